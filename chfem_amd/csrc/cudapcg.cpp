@@ -1,0 +1,873 @@
+// cudapcg.cpp -- the solver plugin C ABI (include/cudapcg.h) over the gfx950
+// kernels.  One global solver per process, as in the reference
+// (chfem/src/femhmg/cudapcg/cudapcg.cu:37); all work is queued on one HIP
+// stream owned by the solver.
+//
+// Krylov loops follow the reference's iteration order and L2 stopping rule
+// (chfem/src/femhmg/cudapcg/solvers/pcg.cu:28-243, pminres.cu:28-247,
+// stopping_criteria.cu:7-14).  Differences by design: alpha/beta and the dot
+// products stay on the device (fused into the vector kernels), and the host
+// reads one 128-byte scalar block per iteration for the stopping test.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <omp.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "solver.h"
+
+namespace chfem {
+
+struct Solver {
+    cudapcgFlag_t analysis_flag = CUDAPCG_THERMAL_3D;
+    cudapcgFlag_t parStrategy_flag = CUDAPCG_NBN;
+    cudapcgModel_t *model = nullptr;
+    unsigned long count = 0;
+    char *header = nullptr;
+    cudapcgTol_t num_tol = CUDAPCG_TOLERANCE;
+    unsigned max_iterations = CUDAPCG_MAX_ITERATIONS;
+    cudapcgFlag_t xreduce_flag = CUDAPCG_XREDUCE_FULL;
+    unsigned xreduce_shift = 0;
+    double xreduce_scale = 1.0, reduce_stab = 0.0001;
+    unsigned solver_flag = CUDAPCG_CG_SOLVER;
+    cudapcgFlag_t precond = CUDAPCG_TRUE;
+    cudapcgFlag_t resnorm = CUDAPCG_L2_NORM;
+    cudapcgFlag_t precision = CHFEM_PREC_FP64;
+    // results
+    double residual = 1.0;
+    unsigned iteration = 0, sweeps = 0;
+    double total_time = 0, mean_time = 0, solve_seconds = 0;
+    cudapcgFlag_t found = CUDAPCG_FALSE;
+    // device state
+    DeviceModel dm;
+    double *r64 = nullptr, *x64 = nullptr, *d64 = nullptr, *q64 = nullptr, *s64 = nullptr, *dinv64 = nullptr;
+    float *x32 = nullptr, *r32 = nullptr, *d32 = nullptr, *q32 = nullptr, *s32 = nullptr, *dinv32 = nullptr;
+    bool arrays = false, user_arrays = false, x0_set = false, dinv_valid = false;
+    double *partials = nullptr, *S = nullptr, *Sh = nullptr;
+    unsigned *counter = nullptr;
+    size_t partials_n = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+static Solver *g = nullptr;
+
+static cudapcgFlag_t ctor_simple(cudapcgModel_t **ptr, const void *data)
+{
+    if (!data) return CUDAPCG_FALSE;
+    *ptr = (cudapcgModel_t *)data;
+    return CUDAPCG_TRUE;
+}
+static cudapcgFlag_t (*g_ctor)(cudapcgModel_t **, const void *) = ctor_simple;
+
+static int make_ops();
+
+DeviceModel *active_model()
+{
+    if (!g) return nullptr;
+    if (g->model) make_ops();
+    return &g->dm;
+}
+hipStream_t active_stream() { return g ? g->stream : nullptr; }
+double *rhs_buffer() { return g ? g->r64 : nullptr; }
+double *solution_buffer() { return g ? g->x64 : nullptr; }
+double *scalars_host() { return g ? g->Sh : nullptr; }
+
+Reduce make_reduce(int fin, int slot)
+{
+    Reduce r;
+    r.partials = g->partials;
+    r.counter = g->counter;
+    r.S = g->S;
+    r.fin = fin;
+    r.slot = slot;
+    return r;
+}
+
+int sync_scalars()
+{
+    if (hipMemcpyAsync(g->Sh, g->S, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, g->stream) != hipSuccess) return 0;
+    if (hipStreamSynchronize(g->stream) != hipSuccess) return 0;
+    return 1;
+}
+
+static int set_scalar(int slot, double v)
+{
+    g->Sh[slot] = v;
+    if (hipMemcpyAsync(g->S + slot, g->Sh + slot, sizeof(double), hipMemcpyHostToDevice, g->stream) != hipSuccess)
+        return 0;
+    return hipStreamSynchronize(g->stream) == hipSuccess;
+}
+
+static int phys(cudapcgFlag_t a)
+{
+    return a == CUDAPCG_THERMAL_3D ? PHY_THERMAL : (a == CUDAPCG_ELASTIC_3D ? PHY_ELASTIC : PHY_FLUID);
+}
+
+static size_t nvars() { return g && g->model ? (size_t)g->model->nvars : 0; }
+
+template <typename T> static void dfree(T *&p)
+{
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+static cudapcgFlag_t free_solver_arrays()
+{
+    if (!g) return CUDAPCG_FALSE;
+    if (!g->x0_set) dfree(g->x64);
+    dfree(g->d64); dfree(g->q64); dfree(g->s64); dfree(g->dinv64);
+    dfree(g->x32); dfree(g->r32); dfree(g->d32); dfree(g->q32); dfree(g->s32); dfree(g->dinv32);
+    dfree(g->partials); dfree(g->S); dfree(g->counter);
+    if (g->Sh) hipHostFree(g->Sh);
+    g->Sh = nullptr;
+    g->arrays = false;
+    g->user_arrays = false;
+    g->dinv_valid = false;
+    return CUDAPCG_TRUE;
+}
+
+static cudapcgFlag_t alloc_solver_arrays()
+{
+    if (!g || !g->model) return CUDAPCG_FALSE;
+    const size_t n = nvars();
+    const size_t b8 = n * sizeof(double), b4 = n * sizeof(float);
+    const bool minres = g->solver_flag == CUDAPCG_MINRES_SOLVER || g->solver_flag == CUDAPCG_MINRES3_SOLVER ||
+                        g->solver_flag == CUDAPCG_MINRES2_SOLVER;
+    if (!g->x64) CHECK_HIP(hipMalloc(&g->x64, b8));
+    if (!g->d64) CHECK_HIP(hipMalloc(&g->d64, b8));
+    if (!g->q64) CHECK_HIP(hipMalloc(&g->q64, b8));
+    if (minres && !g->s64) CHECK_HIP(hipMalloc(&g->s64, b8));
+    if (!g->dinv64) CHECK_HIP(hipMalloc(&g->dinv64, b8));
+    if (g->precision == CHFEM_PREC_MIXED) {
+        if (!g->x32) CHECK_HIP(hipMalloc(&g->x32, b4));
+        if (!g->r32) CHECK_HIP(hipMalloc(&g->r32, b4));
+        if (!g->d32) CHECK_HIP(hipMalloc(&g->d32, b4));
+        if (!g->q32) CHECK_HIP(hipMalloc(&g->q32, b4));
+        if (minres && !g->s32) CHECK_HIP(hipMalloc(&g->s32, b4));
+        if (!g->dinv32) CHECK_HIP(hipMalloc(&g->dinv32, b4));
+    }
+    const size_t pn = (size_t)MAX_PARTIAL_BLOCKS * 8;
+    if (!g->partials) CHECK_HIP(hipMalloc(&g->partials, pn * sizeof(double)));
+    if (!g->S) CHECK_HIP(hipMalloc(&g->S, S_COUNT * sizeof(double)));
+    if (!g->counter) {
+        CHECK_HIP(hipMalloc(&g->counter, sizeof(unsigned)));
+        CHECK_HIP(hipMemsetAsync(g->counter, 0, sizeof(unsigned), g->stream));
+    }
+    if (!g->Sh) CHECK_HIP(hipHostMalloc((void **)&g->Sh, S_COUNT * sizeof(double), hipHostMallocDefault));
+    CHECK_HIP(hipMemsetAsync(g->S, 0, S_COUNT * sizeof(double), g->stream));
+    g->arrays = true;
+    g->dinv_valid = false;
+    return CUDAPCG_TRUE;
+}
+
+// Build the operator descriptors from the model and the local matrices.
+static int make_ops()
+{
+    cudapcgModel_t *m = g->model;
+    OpDesc o;
+    memset(&o, 0, sizeof(o));
+    o.physics = phys(g->analysis_flag);
+    o.ebe = g->parStrategy_flag == CUDAPCG_EBE;
+    o.ny = (int)m->nrows;
+    o.nx = (int)m->ncols;
+    o.nz = (int)m->nlayers;
+    o.N = m->nelem;
+    o.ndof = m->nvars;
+    o.keys = m->image;
+    o.nvel = m->nporenodes;
+    o.nborder = m->nbordernodes;
+    o.dof2node = m->DOF2periodic_map;
+    o.node2dof = m->periodic2DOF_map;
+    o.bkeys = m->border_pore_map;
+    g->dm.op64 = o;
+    g->dm.op64.K = g->dm.K64;
+    g->dm.op32 = o;
+    g->dm.op32.K = g->dm.K32;
+    return 1;
+}
+
+static bool above(double delta, double delta0, double tol) { return fabs(delta) > tol * tol * fabs(delta0); }
+
+// Jacobi-PCG in precision T (pcg.cu:28-243).  r holds b on entry; returns 1 if converged.
+template <typename T>
+static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool x0set, double tol, unsigned maxit,
+                   unsigned &it, double &residual, double *delta0_out)
+{
+    const size_t n = op.ndof;
+    hipStream_t st = g->stream;
+    if (!x0set) launch_fill<T>(x, 0.0, n, st);
+    Reduce r0 = make_reduce(FIN_STORE, S_DELTA0);
+    launch_precond<T>(dinv, r, d, n, &r0, st);
+    if (!sync_scalars()) return -1;
+    const double delta0 = g->Sh[S_DELTA0];
+    if (delta0_out) *delta0_out = delta0;
+    it = 0;
+    residual = 0;
+    if (fabs(delta0) < 1e-12) {
+        launch_fill<T>(x, 0.0, n, st);
+        return 1;
+    }
+    double delta = delta0;
+    if (x0set) {
+        launch_apply<T>(op, x, q, nullptr, st);
+        Reduce rd = make_reduce(FIN_STORE, S_DELTA);
+        launch_sub_precond<T>(r, q, dinv, d, &rd, n, st);
+        if (!sync_scalars()) return -1;
+        delta = g->Sh[S_DELTA];
+        if (!above(delta, delta0, tol)) {
+            residual = sqrt(fabs(delta) / fabs(delta0));
+            return 1;
+        }
+    } else if (!set_scalar(S_DELTA, delta0)) return -1;
+    Reduce ra = make_reduce(FIN_ALPHA, 0), rdl = make_reduce(FIN_DELTA, 0);
+    it = 1;
+    launch_apply<T>(op, d, q, &ra, st);
+    launch_pcg_update<T>(x, r, d, q, dinv, &rdl, n, st);
+    if (!sync_scalars()) return -1;
+    delta = g->Sh[S_DELTA];
+    bool cont = above(delta, delta0, tol);
+    while (cont && it < maxit) {
+        it++;
+        launch_pcg_dir<T>(d, q, g->S, 0, n, st);
+        launch_apply<T>(op, d, q, &ra, st);
+        launch_pcg_update<T>(x, r, d, q, dinv, &rdl, n, st);
+        if (!sync_scalars()) return -1;
+        delta = g->Sh[S_DELTA];
+        cont = above(delta, delta0, tol);
+    }
+    residual = sqrt(fabs(delta) / fabs(delta0));
+    return !above(delta, delta0, tol);
+}
+
+// Jacobi-MINRES in precision T (pminres.cu:28-247).  r holds b on entry.
+template <typename T>
+static int run_minres(const OpDesc &op, T *x, T *r, T *d, T *q, T *s, const T *dinv, bool x0set, double tol,
+                      unsigned maxit, unsigned &it, double &residual)
+{
+    const size_t n = op.ndof;
+    hipStream_t st = g->stream;
+    if (!x0set) launch_fill<T>(x, 0.0, n, st);
+    Reduce rr0 = make_reduce(FIN_STORE, S_RES0), rd0 = make_reduce(FIN_STORE, S_DELTA0);
+    launch_precond<T>(dinv, r, q, n, &rr0, st);        // q = M^-1 r ; res0 = r.q
+    launch_apply<T>(op, q, s, &rd0, st);               // s = A q ; delta0 = q.s
+    if (!sync_scalars()) return -1;
+    const double res0 = g->Sh[S_RES0], delta0 = g->Sh[S_DELTA0];
+    it = 0;
+    residual = 0;
+    if (fabs(res0) < 1e-12) {
+        launch_fill<T>(x, 0.0, n, st);
+        return 1;
+    }
+    double delta = delta0;
+    if (x0set) {
+        Reduce rt0 = make_reduce(FIN_STORE, S_TMP0), rdl = make_reduce(FIN_STORE, S_DELTA);
+        launch_apply<T>(op, x, d, nullptr, st);
+        launch_sub_precond<T>(r, d, dinv, q, &rt0, n, st);
+        launch_apply<T>(op, q, s, &rdl, st);
+        if (!sync_scalars()) return -1;
+        delta = g->Sh[S_DELTA];
+        if (!above(delta, delta0, tol)) {
+            residual = sqrt(fabs(delta) / fabs(delta0));
+            return 1;
+        }
+    } else if (!set_scalar(S_DELTA, delta0)) return -1;
+    // first iteration: d = M^-1 r ; q = A M^-1 r ; r <- M^-1 r kept from here on
+    hipMemcpyAsync(d, q, n * sizeof(T), hipMemcpyDeviceToDevice, st);
+    hipMemcpyAsync(q, s, n * sizeof(T), hipMemcpyDeviceToDevice, st);
+    launch_precond<T>(dinv, r, r, n, nullptr, st);
+    Reduce ra = make_reduce(FIN_ALPHA, 0), rs = make_reduce(FIN_STORE, S_STOP), rdl = make_reduce(FIN_DELTA, 0);
+    launch_precond<T>(dinv, q, nullptr, n, &ra, st);
+    it = 1;
+    launch_minres_update<T>(x, r, d, q, dinv, &rs, n, st);
+    launch_apply<T>(op, r, s, &rdl, st);
+    if (!sync_scalars()) return -1;
+    double stop = g->Sh[S_STOP];
+    bool cont = above(stop, res0, tol);
+    while (cont && it < maxit) {
+        it++;
+        launch_minres_dir<T>(d, q, r, s, dinv, g->S, 0, &ra, n, st);
+        launch_minres_update<T>(x, r, d, q, dinv, &rs, n, st);
+        launch_apply<T>(op, r, s, &rdl, st);
+        if (!sync_scalars()) return -1;
+        stop = g->Sh[S_STOP];
+        cont = above(stop, res0, tol);
+    }
+    residual = sqrt(fabs(stop) / fabs(res0));
+    return residual <= tol;
+}
+
+static bool is_minres()
+{
+    return g->solver_flag == CUDAPCG_MINRES_SOLVER || g->solver_flag == CUDAPCG_MINRES3_SOLVER ||
+           g->solver_flag == CUDAPCG_MINRES2_SOLVER;
+}
+
+static int ensure_dinv()
+{
+    if (g->dinv_valid) return 1;
+    const size_t n = nvars();
+    if (g->precond) launch_jacobi<double>(g->dm.op64, g->dinv64, g->stream);
+    else launch_fill<double>(g->dinv64, 1.0, n, g->stream);
+    if (g->precision == CHFEM_PREC_MIXED) launch_convert<double, float>(g->dinv64, g->dinv32, n, g->stream);
+    g->dinv_valid = true;
+    return 1;
+}
+
+static double inner_tol_floor()
+{
+    const char *e = getenv("CHFEM_INNER_TOL");
+    return e ? atof(e) : 1e-5;
+}
+
+// FP32 inner Krylov + FP64 outer iterative refinement.
+static int run_mixed(bool x0set, unsigned &it_total, unsigned &sweeps, double &residual)
+{
+    const OpDesc &o64 = g->dm.op64, &o32 = g->dm.op32;
+    const size_t n = o64.ndof;
+    hipStream_t st = g->stream;
+    const double tol = g->num_tol;
+    Reduce r0 = make_reduce(FIN_STORE, S_DELTA0), rd = make_reduce(FIN_STORE, S_DELTA);
+    launch_precond<double>(g->dinv64, g->r64, nullptr, n, &r0, st);
+    if (!sync_scalars()) return -1;
+    const double delta0 = g->Sh[S_DELTA0];
+    it_total = 0;
+    sweeps = 0;
+    residual = 0;
+    if (fabs(delta0) < 1e-12) {
+        launch_fill<double>(g->x64, 0.0, n, st);
+        return 1;
+    }
+    double delta = delta0;
+    if (x0set) {
+        launch_apply<double>(o64, g->x64, g->q64, nullptr, st);
+        launch_sub_precond<double>(g->r64, g->q64, g->dinv64, g->d64, &rd, n, st);
+        if (!sync_scalars()) return -1;
+        delta = g->Sh[S_DELTA];
+    } else launch_fill<double>(g->x64, 0.0, n, st);
+    const double floor_tol = inner_tol_floor();
+    while (above(delta, delta0, tol) && it_total < g->max_iterations) {
+        double tau = 0.5 * tol * sqrt(fabs(delta0) / fabs(delta));
+        if (tau < floor_tol) tau = floor_tol;
+        if (tau > 0.5) tau = 0.5;
+        launch_convert<double, float>(g->r64, g->r32, n, st);
+        unsigned it = 0;
+        double res_in = 0;
+        int rc;
+        if (is_minres())
+            rc = run_minres<float>(o32, g->x32, g->r32, g->d32, g->q32, g->s32, g->dinv32, false, tau,
+                                   g->max_iterations - it_total, it, res_in);
+        else
+            rc = run_pcg<float>(o32, g->x32, g->r32, g->d32, g->q32, g->dinv32, false, tau,
+                                g->max_iterations - it_total, it, res_in, nullptr);
+        if (rc < 0) return -1;
+        it_total += it;
+        sweeps++;
+        launch_add_correction(g->x64, g->x32, n, st);
+        launch_convert<float, double>(g->x32, g->d64, n, st);
+        launch_apply<double>(o64, g->d64, g->q64, nullptr, st);
+        launch_sub_precond<double>(g->r64, g->q64, g->dinv64, g->d64, &rd, n, st);
+        if (!sync_scalars()) return -1;
+        const double prev = delta;
+        delta = g->Sh[S_DELTA];
+        if (it == 0 || !(fabs(delta) < fabs(prev))) break;  // no progress: stagnation in FP32
+    }
+    residual = sqrt(fabs(delta) / fabs(delta0));
+    return !above(delta, delta0, tol);
+}
+
+int solve_device(const double *x0_dev)
+{
+    if (!g || !g->model) return -1;
+    if (!g->arrays && alloc_solver_arrays() != CUDAPCG_TRUE) return -1;
+    make_ops();
+    ensure_dinv();
+    const size_t n = nvars();
+    bool x0set = g->x0_set;
+    if (x0_dev) {
+        hipMemcpyAsync(g->x64, x0_dev, n * sizeof(double), hipMemcpyDeviceToDevice, g->stream);
+        x0set = true;
+    }
+    g->count++;
+    hipEventRecord(g->ev0, g->stream);
+    unsigned it = 0, sweeps = 1;
+    double res = 0;
+    int rc;
+    if (g->precision == CHFEM_PREC_MIXED) {
+        rc = run_mixed(x0set, it, sweeps, res);
+    } else if (is_minres()) {
+        rc = run_minres<double>(g->dm.op64, g->x64, g->r64, g->d64, g->q64, g->s64, g->dinv64, x0set, g->num_tol,
+                                g->max_iterations, it, res);
+    } else {
+        rc = run_pcg<double>(g->dm.op64, g->x64, g->r64, g->d64, g->q64, g->dinv64, x0set, g->num_tol,
+                             g->max_iterations, it, res, nullptr);
+    }
+    hipEventRecord(g->ev1, g->stream);
+    hipEventSynchronize(g->ev1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, g->ev0, g->ev1);
+    g->solve_seconds = ms * 1e-3;
+    g->iteration = it;
+    g->sweeps = sweeps;
+    g->residual = res;
+    g->mean_time = it ? g->solve_seconds / it : 0.0;
+    g->found = rc == 1 ? CUDAPCG_TRUE : CUDAPCG_FALSE;
+    if (rc < 0) {
+        fprintf(stderr, "chfem: solver failed on the device\n");
+        return -1;
+    }
+    printf("%siteration: %u, residual: %.3e\n", g->header ? g->header : "", it, res);
+    return rc;
+}
+
+}  // namespace chfem
+
+using namespace chfem;
+
+extern "C" {
+
+cudapcgFlag_t cudapcgSetNumTol(cudapcgTol_t t) { if (!g) return CUDAPCG_FALSE; g->num_tol = t; return CUDAPCG_TRUE; }
+cudapcgFlag_t cudapcgSetMaxIterations(unsigned int n) { if (!g) return CUDAPCG_FALSE; g->max_iterations = n; return CUDAPCG_TRUE; }
+cudapcgFlag_t cudapcgSetXReduceMode(cudapcgFlag_t flag)
+{
+    if (!g) return CUDAPCG_FALSE;
+    g->xreduce_flag = flag <= CUDAPCG_XREDUCE_FULL ? flag : CUDAPCG_XREDUCE_FULL;
+    return CUDAPCG_TRUE;
+}
+cudapcgFlag_t cudapcgSetXReduceShift(unsigned int shift) { if (!g) return CUDAPCG_FALSE; g->xreduce_shift = shift; return CUDAPCG_TRUE; }
+cudapcgFlag_t cudapcgSetXReduceScale(double scl) { if (!g) return CUDAPCG_FALSE; g->xreduce_scale = scl; return CUDAPCG_TRUE; }
+cudapcgFlag_t cudapcgSetReduceStabFactor(double scl) { if (!g) return CUDAPCG_FALSE; g->reduce_stab = scl; return CUDAPCG_TRUE; }
+unsigned int cudapcgGetNumIterations(void) { return g ? g->iteration : 0; }
+unsigned int cudapcgGetMaxNumIterations(void) { return g ? g->max_iterations : 0; }
+cudapcgVar_t cudapcgGetResidual(void) { return g ? g->residual : 0; }
+
+cudapcgFlag_t cudapcgSetPrecision(cudapcgFlag_t flag)
+{
+    if (!g) return CUDAPCG_FALSE;
+    if (flag > CHFEM_PREC_MIXED) return CUDAPCG_FALSE;
+    if (flag != g->precision && g->arrays) {
+        g->precision = flag;
+        return alloc_solver_arrays();
+    }
+    g->precision = flag;
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgGetSolveStats(chfemSolveStats_t *s)
+{
+    if (!g || !s) return CUDAPCG_FALSE;
+    s->iterations = g->iteration;
+    s->sweeps = g->sweeps;
+    s->solve_seconds = g->solve_seconds;
+    s->residual = g->residual;
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgSetSolver(cudapcgFlag_t flag)
+{
+    if (!g) return CUDAPCG_FALSE;
+    if (flag > CUDAPCG_MINRES2_SOLVER) {
+        printf("ERROR: Invalid \"solverFlag\" (%d).\n", flag);
+        return CUDAPCG_FALSE;
+    }
+    const bool was_minres = is_minres();
+    g->solver_flag = flag;
+    if (g->arrays && is_minres() && !was_minres) return alloc_solver_arrays();
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgSetPreconditioner(cudapcgFlag_t flag)
+{
+    if (!g) return CUDAPCG_FALSE;
+    g->precond = flag > 0 ? CUDAPCG_TRUE : CUDAPCG_FALSE;
+    g->dinv_valid = false;
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgSetResNorm(cudapcgFlag_t flag)
+{
+    if (!g) return CUDAPCG_FALSE;
+    g->resnorm = flag > CUDAPCG_ERROR_NORM ? CUDAPCG_L2_NORM : flag;
+    if (g->resnorm != CUDAPCG_L2_NORM)
+        fprintf(stderr, "chfem: only the L2 stopping rule is implemented; using L2\n");
+    return CUDAPCG_TRUE;
+}
+
+static void free_model_device(cudapcgModel_t *m)
+{
+    if (!m) return;
+    dfree(m->image); dfree(m->parametric_density_field); dfree(m->pore_map); dfree(m->border_pore_map);
+    dfree(m->periodic2DOF_map); dfree(m->DOF2periodic_map);
+    if (m->name) free(m->name);
+    m->name = nullptr;
+}
+
+cudapcgFlag_t cudapcgEnd(void)
+{
+    if (!g) return CUDAPCG_FALSE;
+    if (g->stream) hipStreamSynchronize(g->stream);
+    free(g->header);
+    g->x0_set = false;
+    free_solver_arrays();
+    dfree(g->x64);
+    dfree(g->r64);
+    dfree(g->dm.K64);
+    dfree(g->dm.K32);
+    if (g->model && g->model->freeAllowed_flag) {
+        free_model_device(g->model);
+        free(g->model);
+    }
+    if (g->ev0) hipEventDestroy(g->ev0);
+    if (g->ev1) hipEventDestroy(g->ev1);
+    if (g->stream) hipStreamDestroy(g->stream);
+    delete g;
+    g = nullptr;
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgInit(cudapcgFlag_t analysis_flag, cudapcgFlag_t parStrategy_flag)
+{
+    cudapcgEnd();
+    if (analysis_flag != CUDAPCG_THERMAL_3D && analysis_flag != CUDAPCG_ELASTIC_3D && analysis_flag != CUDAPCG_FLUID_3D) {
+        if (analysis_flag <= CUDAPCG_FLUID_3D)
+            printf("ERROR: 2D analyses are not provided by this library.\n");
+        return CUDAPCG_FALSE;
+    }
+    if (analysis_flag == CUDAPCG_FLUID_3D && parStrategy_flag != CUDAPCG_NBN) {
+        if (parStrategy_flag == CUDAPCG_EBE) printf("ERROR: EBE solver is unavailable for FLUID analysis.\n");
+        else printf("ERROR: Invalid \"parStrategy_flag\" (%d) for FLUID analysis.\n", parStrategy_flag);
+        return CUDAPCG_FALSE;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        fprintf(stderr, "chfem: no HIP device available\n");
+        return CUDAPCG_FALSE;
+    }
+    g = new Solver();
+    g->analysis_flag = analysis_flag;
+    g->parStrategy_flag = parStrategy_flag;
+    g->header = (char *)malloc(1);
+    g->header[0] = 0;
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess) {
+        fprintf(stderr, "chfem: failed to create the solver stream\n");
+        delete g;
+        g = nullptr;
+        return CUDAPCG_FALSE;
+    }
+    return CUDAPCG_TRUE;
+}
+
+cudapcgModel_t *cudapcgNewModel(void)
+{
+    cudapcgModel_t *m = (cudapcgModel_t *)calloc(1, sizeof(cudapcgModel_t));
+    if (!m) return nullptr;
+    m->freeAllowed_flag = CUDAPCG_TRUE;
+    m->parStrategy_flag = CUDAPCG_NBN;
+    m->poremap_flag = CUDAPCG_POREMAP_NUM;
+    return m;
+}
+
+static cudapcgFlag_t alloc_model_device()
+{
+    cudapcgModel_t *m = g->model;
+    const size_t n = m->nvars;
+    dfree(g->r64);
+    free_solver_arrays();
+    CHECK_HIP(hipMalloc(&g->r64, n * sizeof(double)));
+    if (g->analysis_flag != CUDAPCG_FLUID_3D) {
+        if (!m->image) CHECK_HIP(hipMalloc(&m->image, sizeof(cudapcgMap_t) * m->nelem));
+    } else {
+        if (!m->periodic2DOF_map) CHECK_HIP(hipMalloc(&m->periodic2DOF_map, sizeof(cudapcgIdMap_t) * m->nelem));
+        if (m->poremap_flag == CUDAPCG_POREMAP_NUM) {
+            if (!m->border_pore_map)
+                CHECK_HIP(hipMalloc(&m->border_pore_map, sizeof(cudapcgFlag_t) * (m->nbordernodes + 1)));
+            if (!m->DOF2periodic_map)
+                CHECK_HIP(hipMalloc(&m->DOF2periodic_map,
+                                    sizeof(cudapcgIdMap_t) * ((size_t)m->nporenodes + m->nbordernodes + 1)));
+        } else {
+            printf("ERROR: image pore mapping is not provided by this library; use DOF-number mapping.\n");
+            return CUDAPCG_FALSE;
+        }
+    }
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgSetModel(cudapcgModel_t *model)
+{
+    if (!g || !model) return CUDAPCG_FALSE;
+    if (g->model && g->model != model && g->model->freeAllowed_flag) {
+        free_model_device(g->model);
+        free(g->model);
+    }
+    g->model = model;
+    return alloc_model_device();
+}
+
+cudapcgFlag_t cudapcgSetModelConstructorFcn(cudapcgFlag_t (*fcn)(cudapcgModel_t **, const void *))
+{
+    if (!fcn) return CUDAPCG_FALSE;
+    g_ctor = fcn;
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgBuildModel(const void *data)
+{
+    if (!g) return CUDAPCG_FALSE;
+    if (!g->model) g->model = cudapcgNewModel();
+    else if (g->model->freeAllowed_flag) {
+        free_model_device(g->model);
+        free(g->model);
+        g->model = cudapcgNewModel();
+    }
+    if (!g_ctor(&g->model, data)) return CUDAPCG_FALSE;
+    return alloc_model_device();
+}
+
+cudapcgFlag_t cudapcgAllocateArrays(void)
+{
+    if (!g || !g->model) return CUDAPCG_FALSE;
+    cudapcgFlag_t ok = alloc_solver_arrays();
+    g->user_arrays = ok == CUDAPCG_TRUE;
+    return ok;
+}
+
+cudapcgFlag_t cudapcgFreeArrays(void)
+{
+    if (!g) return CUDAPCG_FALSE;
+    return free_solver_arrays();
+}
+
+cudapcgFlag_t cudapcgSetRHS(cudapcgVar_t *RHS)
+{
+    if (!g || !g->model || !RHS) return CUDAPCG_FALSE;
+    CHECK_HIP(hipMemcpyAsync(g->r64, RHS, nvars() * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    CHECK_HIP(hipStreamSynchronize(g->stream));
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgSetX0(cudapcgVar_t *x0, cudapcgFlag_t mustInterpolate)
+{
+    if (!g || !g->model) return CUDAPCG_FALSE;
+    const size_t n = nvars();
+    if (!x0) {
+        g->x0_set = false;
+        return CUDAPCG_TRUE;
+    }
+    if (!g->x64) CHECK_HIP(hipMalloc(&g->x64, n * sizeof(double)));
+    if (mustInterpolate) {
+        cudapcgModel_t *m = g->model;
+        const unsigned stride = m->nvars / m->nelem;
+        const size_t nc = (size_t)stride * (m->ncols / 2) * (m->nrows / 2) * (m->nlayers / 2 + (m->nlayers < 2));
+        double *tmp = nullptr;
+        CHECK_HIP(hipMalloc(&tmp, nc * sizeof(double)));
+        CHECK_HIP(hipMemcpyAsync(tmp, x0, nc * sizeof(double), hipMemcpyHostToDevice, g->stream));
+        launch_interpl2(tmp, g->x64, m->nrows, m->ncols, m->nlayers, stride, g->stream);
+        CHECK_HIP(hipStreamSynchronize(g->stream));
+        hipFree(tmp);
+    } else {
+        CHECK_HIP(hipMemcpyAsync(g->x64, x0, n * sizeof(double), hipMemcpyHostToDevice, g->stream));
+        CHECK_HIP(hipStreamSynchronize(g->stream));
+    }
+    g->x0_set = true;
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgSetImage(cudapcgMap_t *img)
+{
+    if (!g || !g->model || !img || !g->model->image) return CUDAPCG_FALSE;
+    CHECK_HIP(hipMemcpyAsync(g->model->image, img, g->model->nelem * sizeof(cudapcgMap_t), hipMemcpyHostToDevice,
+                             g->stream));
+    CHECK_HIP(hipStreamSynchronize(g->stream));
+    g->dinv_valid = false;
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgSetParametricDensityField(parametricScalarField_t *field, double fmin, double fmax)
+{
+    (void)field; (void)fmin; (void)fmax;
+    printf("ERROR: scalar density field input is not provided by this library.\n");
+    return CUDAPCG_FALSE;
+}
+
+cudapcgFlag_t cudapcgSetPoreMap(cudapcgFlag_t *pores)
+{
+    if (!g || !g->model || !pores) return CUDAPCG_FALSE;
+    if (g->model->poremap_flag != CUDAPCG_POREMAP_NUM) return CUDAPCG_FALSE;
+    if (g->model->nbordernodes)
+        CHECK_HIP(hipMemcpyAsync(g->model->border_pore_map, pores, g->model->nbordernodes, hipMemcpyHostToDevice,
+                                 g->stream));
+    CHECK_HIP(hipStreamSynchronize(g->stream));
+    g->dinv_valid = false;
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgSetPeriodic2DOFMap(cudapcgIdMap_t *vars)
+{
+    if (!g || !g->model || !vars) return CUDAPCG_FALSE;
+    CHECK_HIP(hipMemcpyAsync(g->model->periodic2DOF_map, vars, g->model->nelem * sizeof(cudapcgIdMap_t),
+                             hipMemcpyHostToDevice, g->stream));
+    CHECK_HIP(hipStreamSynchronize(g->stream));
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgSetDOF2PeriodicMap(cudapcgIdMap_t *nodes)
+{
+    if (!g || !g->model || !nodes) return CUDAPCG_FALSE;
+    if (g->model->poremap_flag != CUDAPCG_POREMAP_NUM) return CUDAPCG_FALSE;
+    const size_t n = (size_t)g->model->nporenodes + g->model->nbordernodes;
+    if (n) CHECK_HIP(hipMemcpyAsync(g->model->DOF2periodic_map, nodes, n * sizeof(cudapcgIdMap_t),
+                                    hipMemcpyHostToDevice, g->stream));
+    CHECK_HIP(hipStreamSynchronize(g->stream));
+    return CUDAPCG_TRUE;
+}
+
+// Local matrices in the reference layout (chfem/src/femhmg/cudapcg/cudapcg.cu:365-375):
+// thermal 40 stored non-zeros per material, elastic 24x24, Stokes one 32x32.
+// They are expanded to dense blocks and kept in FP64 and FP32.
+cudapcgFlag_t cudapcgSetLclMtxs(cudapcgVar_t *L)
+{
+    if (!g || !g->model || !L) return CUDAPCG_FALSE;
+    const unsigned nk = g->model->nkeys ? g->model->nkeys : 1;
+    const int ph = phys(g->analysis_flag);
+    const size_t ksz = ph == PHY_THERMAL ? 64 : (ph == PHY_ELASTIC ? 576 : 1024);
+    double *K = (double *)calloc(ksz * nk, sizeof(double));
+    float *Kf = (float *)calloc(ksz * nk, sizeof(float));
+    static const int dr[8] = {1, 1, 0, 0, 1, 1, 0, 0}, dc[8] = {0, 1, 1, 0, 0, 1, 1, 0}, dl[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    for (unsigned m = 0; m < nk; m++) {
+        if (ph == PHY_THERMAL) {
+            const double *src = L + (size_t)m * g->model->localmtxdim;
+            int t = 0;
+            for (int i = 0; i < 8; i++)
+                for (int j = 0; j < 8; j++) {
+                    // a conduction matrix of the voxel vanishes between nodes that differ in one coordinate
+                    const int diff = (dr[i] != dr[j]) + (dc[i] != dc[j]) + (dl[i] != dl[j]);
+                    K[m * 64 + i * 8 + j] = diff == 1 ? 0.0 : src[t++];
+                }
+        } else {
+            memcpy(K + m * ksz, L + (size_t)m * ksz, ksz * sizeof(double));
+        }
+    }
+    for (size_t i = 0; i < ksz * nk; i++) Kf[i] = (float)K[i];
+    dfree(g->dm.K64);
+    dfree(g->dm.K32);
+    hipError_t e1 = hipMalloc(&g->dm.K64, ksz * nk * sizeof(double));
+    hipError_t e2 = hipMalloc(&g->dm.K32, ksz * nk * sizeof(float));
+    if (e1 == hipSuccess && e2 == hipSuccess) {
+        e1 = hipMemcpy(g->dm.K64, K, ksz * nk * sizeof(double), hipMemcpyHostToDevice);
+        e2 = hipMemcpy(g->dm.K32, Kf, ksz * nk * sizeof(float), hipMemcpyHostToDevice);
+    }
+    free(K);
+    free(Kf);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        fprintf(stderr, "chfem: failed to upload local matrices\n");
+        return CUDAPCG_FALSE;
+    }
+    g->dm.ksz = ksz;
+    g->dm.nmat = nk;
+    g->dinv_valid = false;
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgSolve(cudapcgVar_t *x)
+{
+    if (!g || !g->model || !x) return CUDAPCG_FALSE;
+    if (g->model->poremap_flag != CUDAPCG_POREMAP_NUM && g->analysis_flag == CUDAPCG_FLUID_3D) return CUDAPCG_FALSE;
+    const bool temp = !g->arrays;
+    double t = omp_get_wtime();
+    int rc = solve_device(nullptr);
+    if (rc < 0) return CUDAPCG_FALSE;
+    const size_t n = nvars();
+    if (g->solver_flag >= CUDAPCG_CG2_SOLVER && g->analysis_flag == CUDAPCG_FLUID_3D) {
+        // two-vector solvers return the effective velocity field only
+        // (chfem/src/femhmg/cudapcg/solvers/xreduce.cu:4-26)
+        Reduce r = make_reduce(FIN_STORE, S_TMP0);
+        launch_fluid_velocity_sum<double>(g->dm.op64, g->x64, &r, g->stream);
+        if (!sync_scalars()) return CUDAPCG_FALSE;
+        for (int a = 0; a < 3; a++) x[a] = g->Sh[S_TMP0 + a] * g->xreduce_scale;
+    } else {
+        CHECK_HIP(hipMemcpyAsync(x, g->x64, n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+        CHECK_HIP(hipStreamSynchronize(g->stream));
+    }
+    if (temp && !g->user_arrays) free_solver_arrays();
+    g->total_time = omp_get_wtime() - t;
+    return g->found;
+}
+
+cudapcgFlag_t cudapcgApplyOperator(const cudapcgVar_t *v, cudapcgVar_t *q)
+{
+    if (!g || !g->model || !v || !q) return CUDAPCG_FALSE;
+    if (!g->arrays && alloc_solver_arrays() != CUDAPCG_TRUE) return CUDAPCG_FALSE;
+    make_ops();
+    const size_t n = nvars();
+    CHECK_HIP(hipMemcpyAsync(g->d64, v, n * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    launch_apply<double>(g->dm.op64, g->d64, g->q64, nullptr, g->stream);
+    CHECK_HIP(hipGetLastError());
+    CHECK_HIP(hipMemcpyAsync(q, g->q64, n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    CHECK_HIP(hipStreamSynchronize(g->stream));
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgApplyPreconditioner(const cudapcgVar_t *v, cudapcgVar_t *z)
+{
+    if (!g || !g->model || !v || !z) return CUDAPCG_FALSE;
+    if (!g->arrays && alloc_solver_arrays() != CUDAPCG_TRUE) return CUDAPCG_FALSE;
+    make_ops();
+    ensure_dinv();
+    const size_t n = nvars();
+    CHECK_HIP(hipMemcpyAsync(g->d64, v, n * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    launch_precond<double>(g->dinv64, g->d64, g->q64, n, nullptr, g->stream);
+    CHECK_HIP(hipMemcpyAsync(z, g->q64, n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    CHECK_HIP(hipStreamSynchronize(g->stream));
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgSetHeaderString(char *header)
+{
+    if (!g) return CUDAPCG_FALSE;
+    free(g->header);
+    const char *h = header ? header : "";
+    g->header = (char *)malloc(strlen(h) + 1);
+    strcpy(g->header, h);
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgPrintSolverMetrics(void)
+{
+    if (!g) return CUDAPCG_FALSE;
+    printf("%sMean time per iteration: %.2e s\n", g->header, g->mean_time);
+    printf("%sElapsed time: %.2e s\n", g->header, g->total_time);
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgPrintSolverReport(void)
+{
+    if (!g) return CUDAPCG_FALSE;
+    if (g->found) printf("%sPCG solver found a solution in %d iterations.\n", g->header, g->iteration);
+    else printf("%sPCG solver did not converge within provided max number of iterations (%d).\n", g->header,
+                g->max_iterations);
+    printf("%sResidual: %e\n", g->header, g->residual);
+    return cudapcgPrintSolverMetrics();
+}
+
+cudapcgFlag_t cudapcgPrintSolverReport2(char *dest)
+{
+    if (!g || !dest) return CUDAPCG_FALSE;
+    int sz = sprintf(dest,
+                     "%sPCG solver ran through %d iterations.\n%sResidual: %e\n%sMean time per iteration: %.2e s\n"
+                     "%sElapsed time: %.2e s\n",
+                     g->header, g->iteration, g->header, g->residual, g->header, g->mean_time, g->header,
+                     g->total_time);
+    return sz < 0 ? CUDAPCG_FALSE : CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgPrintSolverMetrics2(char *dest)
+{
+    if (!g || !dest) return CUDAPCG_FALSE;
+    int sz = sprintf(dest, "%sMean time per iteration: %.2e s\n%sElapsed time: %.2e s\n", g->header, g->mean_time,
+                     g->header, g->total_time);
+    return sz < 0 ? CUDAPCG_FALSE : CUDAPCG_TRUE;
+}
+
+}  // extern "C"

@@ -1,0 +1,1034 @@
+// femhmg.cpp -- the homogenization-model C ABI (include/femhmg.h): neutral-file
+// and image input, node keys / Stokes DOF maps, local matrices, the
+// per-direction solve loop and the effective tensor.  Right-hand sides and
+// effective-tensor columns are computed on the device; the solution vector
+// stays in HBM unless it is exported.
+//
+// Reference behaviour followed: chfem/src/femhmg/femhmg.c:135-1198 (driver),
+// :1456-1758 (input), chfem/src/femhmg/physical_phenomena/femhmg_3D.c,
+// femhmg_thermal_3D.c, femhmg_elastic_3D.c, femhmg_fluid_3D.c (physics),
+// chfem/src/main.c:72-152, :300-323 (runAnalysis / defaults).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <omp.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "solver.h"
+#include "../../include/femhmg.h"
+#include "../../include/chfem_amd.h"
+
+using namespace chfem;
+
+namespace {
+
+// local node k of element (r,c,l) sits at (r+DR[k], c+DC[k], l+DL[k])
+const int DR[8] = {1, 1, 0, 0, 1, 1, 0, 0};
+const int DC[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+const int DL[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+
+struct Model {
+    std::string nf, nf_noext, image_file, x0_file;
+    bool have_sdf = false;
+    int analysis = HMG_THERMAL;
+    int dim = HMG_3D;
+    unsigned hmg_flag = HOMOGENIZE_ALL;
+    bool hmg_flag_set = false, using_x0 = false, export_x = false, thermal_expansion = false;
+    bool save_fields = false;
+    cudapcgFlag_t poremap = CUDAPCG_POREMAP_NUM;
+    cudapcgFlag_t pcg_flag = CUDAPCG_THERMAL_3D;
+    double elem_size = 1.0;
+    unsigned refinement = 1;
+    unsigned C_dim = 9, lclMtx_dim = 40, lclCB_dim = 0;
+    double props[512];
+    double alpha[256];
+    uint8_t props_keys[256];
+    unsigned nmat = 0;
+    unsigned ex = 0, ey = 0, ez = 0;   // element counts (cols, rows, layers)
+    size_t nelem = 0, ndof = 0;
+    unsigned nvel = 0, nborder = 0;
+    size_t nporeelem = 0;
+    double tol = 1e-5;
+    unsigned maxit = 10000;
+    std::vector<uint16_t> elem_map, keys;
+    std::vector<uint8_t> fkeys, bkeys;
+    std::vector<uint32_t> dof_id, dof2node;
+    std::vector<double> Mtxs, CB;        // reference layouts
+    double C[36];
+    std::vector<std::vector<double>> x0;  // per direction (recursive guesses)
+    std::vector<chfemSolveStats_t> stats;
+    std::vector<std::string> report;
+    bool report_on = false;
+};
+
+Model *M = nullptr;
+
+// driver flags (femhmg.c:23-25)
+cudapcgFlag_t g_strategy = CUDAPCG_NBN, g_solver = CUDAPCG_CG_SOLVER, g_precond = CUDAPCG_TRUE;
+cudapcgFlag_t g_poremap_default = CUDAPCG_POREMAP_NUM, g_xreduce = CUDAPCG_XREDUCE_FULL, g_stopcrit = CUDAPCG_L2_NORM;
+cudapcgFlag_t g_precision = CHFEM_PREC_FP64;
+double g_xreduce_stab = 0.0001;
+double g_time_total = 0;
+
+// ---------------------------------------------------------------------------
+// local matrices of the unit voxel from exact 1D integrals of the linear
+// shape functions: mass 1/3, 1/6; stiffness +-1; mixed +-1/2.
+// ---------------------------------------------------------------------------
+inline void frame(int k, bool stokes, int &x, int &y, int &z)
+{
+    x = DC[k];
+    y = 1 - DR[k];
+    z = stokes ? DL[k] : 1 - DL[k];
+}
+
+// int over the unit cube of d_p N_i * d_q N_j  (p or q = -1: no derivative)
+double int_dd(int i, int j, int p, int q, bool stokes)
+{
+    int ci[3], cj[3];
+    frame(i, stokes, ci[0], ci[1], ci[2]);
+    frame(j, stokes, cj[0], cj[1], cj[2]);
+    double v = 1.0;
+    for (int t = 0; t < 3; t++) {
+        const double si = ci[t] ? 1.0 : -1.0, sj = cj[t] ? 1.0 : -1.0;
+        if (t == p && t == q) v *= si * sj;
+        else if (t == p) v *= 0.5 * si;
+        else if (t == q) v *= 0.5 * sj;
+        else v *= ci[t] == cj[t] ? 1.0 / 3.0 : 1.0 / 6.0;
+    }
+    return v;
+}
+
+double lap(int i, int j, bool stokes)
+{
+    return int_dd(i, j, 0, 0, stokes) + int_dd(i, j, 1, 1, stokes) + int_dd(i, j, 2, 2, stokes);
+}
+
+// 24x24 isotropic stiffness, DOFs node-major (x,y,z)
+void elastic_K(double lam, double mu, bool stokes, double *K)
+{
+    for (int i = 0; i < 8; i++)
+        for (int a = 0; a < 3; a++)
+            for (int j = 0; j < 8; j++)
+                for (int b = 0; b < 3; b++)
+                    K[(3 * i + a) * 24 + 3 * j + b] = lam * int_dd(i, j, a, b, stokes) +
+                                                      mu * ((a == b ? lap(i, j, stokes) : 0.0) +
+                                                            int_dd(i, j, b, a, stokes));
+}
+
+// 6x24 centre stress operator, rows xx,yy,zz,yz,xz,xy (engineering shear)
+void elastic_CB(double lam, double mu, double *CB)
+{
+    for (int k = 0; k < 8; k++) {
+        int x, y, z;
+        frame(k, false, x, y, z);
+        const double g[3] = {(x ? 0.25 : -0.25), (y ? 0.25 : -0.25), (z ? 0.25 : -0.25)};
+        for (int b = 0; b < 3; b++) {
+            double eps[6] = {0, 0, 0, 0, 0, 0};
+            eps[b] = g[b];
+            if (b == 1) { eps[3] += g[2]; eps[5] += g[0]; }
+            if (b == 2) { eps[3] += g[1]; eps[4] += g[0]; }
+            if (b == 0) { eps[4] += g[2]; eps[5] += g[1]; }
+            const double tr = eps[0] + eps[1] + eps[2];
+            for (int r = 0; r < 3; r++) CB[r * 24 + 3 * k + b] = lam * tr + 2 * mu * eps[r];
+            for (int r = 3; r < 6; r++) CB[r * 24 + 3 * k + b] = mu * eps[r];
+        }
+    }
+}
+
+void assemble_local(Model &m)
+{
+    if (m.analysis == HMG_THERMAL) {
+        m.Mtxs.assign(40 * m.nmat, 0.0);
+        for (unsigned t = 0; t < m.nmat; t++) {
+            int n = 0;
+            for (int i = 0; i < 8; i++)
+                for (int j = 0; j < 8; j++) {
+                    const int diff = (DR[i] != DR[j]) + (DC[i] != DC[j]) + (DL[i] != DL[j]);
+                    if (diff != 1) m.Mtxs[40 * t + n++] = m.props[t] * lap(i, j, false);
+                }
+        }
+        // thermal flux at the centre, 3x8 per material: k * grad N
+        m.CB.assign(24 * m.nmat, 0.0);
+        for (unsigned t = 0; t < m.nmat; t++)
+            for (int k = 0; k < 8; k++) {
+                int x, y, z;
+                frame(k, false, x, y, z);
+                m.CB[24 * t + 0 * 8 + k] = m.props[t] * (x ? 0.25 : -0.25);
+                m.CB[24 * t + 1 * 8 + k] = m.props[t] * (y ? 0.25 : -0.25);
+                m.CB[24 * t + 2 * 8 + k] = m.props[t] * (z ? 0.25 : -0.25);
+            }
+    } else if (m.analysis == HMG_ELASTIC) {
+        m.Mtxs.assign(576 * m.nmat, 0.0);
+        m.CB.assign(144 * m.nmat, 0.0);
+        for (unsigned t = 0; t < m.nmat; t++) {
+            const double E = m.props[2 * t], nu = m.props[2 * t + 1];
+            const double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
+            elastic_K(lam, mu, false, &m.Mtxs[576 * t]);
+            elastic_CB(lam, mu, &m.CB[144 * t]);
+        }
+    } else {
+        // Stokes [K G; G^T S]: K = 2 eps:eps (mu = 1), G = int dN_i/dx_a N_j,
+        // S = tau lap with tau = 1/6 (femhmg_fluid_3D.c:41-110)
+        m.Mtxs.assign(1024, 0.0);
+        std::vector<double> Kv(576);
+        elastic_K(0.0, 1.0, true, Kv.data());
+        for (int i = 0; i < 24; i++)
+            for (int j = 0; j < 24; j++) m.Mtxs[i * 32 + j] = Kv[i * 24 + j];
+        for (int i = 0; i < 8; i++)
+            for (int a = 0; a < 3; a++)
+                for (int j = 0; j < 8; j++) {
+                    const double gv = int_dd(i, j, a, -1, true);
+                    m.Mtxs[(3 * i + a) * 32 + 24 + j] = gv;
+                    m.Mtxs[(24 + j) * 32 + 3 * i + a] = gv;
+                }
+        for (int i = 0; i < 8; i++)
+            for (int j = 0; j < 8; j++) m.Mtxs[(24 + i) * 32 + 24 + j] = lap(i, j, true) / 6.0;
+        m.CB.assign(80, 0.0);
+        for (int k = 0; k < 8; k++)
+            for (int a = 0; a < 3; a++) m.CB[a * 24 + 3 * k + a] = 0.125;
+        for (int k = 0; k < 8; k++) m.CB[72 + k] = 0.125;
+    }
+}
+
+// dense per-material local matrix (thermal expanded from the 40 stored entries)
+std::vector<double> dense_K(const Model &m)
+{
+    if (m.analysis == HMG_THERMAL) {
+        std::vector<double> K(64 * m.nmat, 0.0);
+        for (unsigned t = 0; t < m.nmat; t++) {
+            int n = 0;
+            for (int i = 0; i < 8; i++)
+                for (int j = 0; j < 8; j++) {
+                    const int diff = (DR[i] != DR[j]) + (DC[i] != DC[j]) + (DL[i] != DL[j]);
+                    K[64 * t + i * 8 + j] = diff == 1 ? 0.0 : m.Mtxs[40 * t + n++];
+                }
+        }
+        return K;
+    }
+    return m.Mtxs;
+}
+
+// ---------------------------------------------------------------------------
+// maps (femhmg_3D.c:4-75, femhmg_fluid_3D.c:370-491), gathered node by node
+// ---------------------------------------------------------------------------
+inline size_t pid(long r, long c, long l, long ny, long nx, long nz)
+{
+    r = (r + ny) % ny; c = (c + nx) % nx; l = (l + nz) % nz;
+    return (size_t)r + (size_t)c * ny + (size_t)l * nx * ny;
+}
+
+void build_keys(Model &m)
+{
+    const long ny = m.ey, nx = m.ex, nz = m.ez;
+    m.keys.assign(m.nelem, 0);
+    #pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)m.nelem; i++) {
+        const long r = i % ny, c = (i / ny) % nx, l = i / (nx * ny);
+        unsigned key = 0;
+        for (int e = 0; e < 8; e++)
+            key |= (unsigned)(m.elem_map[pid(r - DR[e], c - DC[e], l - DL[e], ny, nx, nz)] & MATKEY_BITSTEP_RANGE_3D)
+                   << (MATKEY_BITSTEP_3D * e);
+        m.keys[i] = (uint16_t)key;
+    }
+}
+
+void build_fluid_maps(Model &m)
+{
+    const long ny = m.ey, nx = m.ex, nz = m.ez;
+    m.fkeys.assign(m.nelem, 0);
+    size_t npore = 0;
+    #pragma omp parallel for schedule(static) reduction(+ : npore)
+    for (long i = 0; i < (long)m.nelem; i++) {
+        const long r = i % ny, c = (i / ny) % nx, l = i / (nx * ny);
+        unsigned key = 0;
+        for (int e = 0; e < 8; e++)
+            key |= (unsigned)(m.elem_map[pid(r - DR[e], c - DC[e], l - DL[e], ny, nx, nz)] == 0) << e;
+        m.fkeys[i] = (uint8_t)key;
+        npore += m.elem_map[i] == 0;
+    }
+    m.nporeelem = npore;
+    // DOF numbering: nodes inside the pore space first, then interface nodes,
+    // each in natural order; solid nodes map to 0 (femhmg_fluid_3D.c:370-424)
+    m.dof_id.assign(m.nelem, 0);
+    uint32_t count = 0;
+    for (size_t i = 0; i < m.nelem; i++)
+        if (m.fkeys[i] == 255) m.dof_id[i] = count++;
+    m.nvel = count;
+    for (size_t i = 0; i < m.nelem; i++)
+        if (m.fkeys[i] && m.fkeys[i] != 255) m.dof_id[i] = count++;
+    m.nborder = count - m.nvel;
+    m.ndof = 4 * (size_t)m.nvel + m.nborder;
+    m.dof2node.assign(count, 0);
+    m.bkeys.assign(m.nborder, 0);
+    for (size_t i = 0; i < m.nelem; i++) {
+        if (!m.fkeys[i]) continue;
+        m.dof2node[m.dof_id[i]] = (uint32_t)i;
+        if (m.fkeys[i] != 255) m.bkeys[m.dof_id[i] - m.nvel] = m.fkeys[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// input (femhmg.c:1456-1758)
+// ---------------------------------------------------------------------------
+bool read_data(Model &m, const char *filename)
+{
+    FILE *f = fopen(filename, "r");
+    if (!f) return false;
+    char str[256];
+    unsigned count = 0;
+    const unsigned num_of_info = 12;
+    bool stab_set = false;
+    unsigned nx = 0, ny = 0, nz = 0;
+    while (count < num_of_info && fscanf(f, "%255s", str) == 1) {
+        if (!strcmp(str, "%type_of_analysis")) {
+            unsigned a;
+            if (fscanf(f, "%u", &a) != 1) break;
+            count++;
+            if (a > HMG_FLUID) {
+                fclose(f);
+                printf("ERROR: Provided analysis flag does not match any possible value.\n");
+                return false;
+            }
+            m.analysis = (int)a;
+        } else if (!strcmp(str, "%type_of_solver")) {
+            unsigned s;
+            if (fscanf(f, "%u", &s) != 1) break;
+            g_solver = (cudapcgFlag_t)s;
+            count++;
+        } else if (!strcmp(str, "%voxel_size")) {
+            if (fscanf(f, "%lf", &m.elem_size) != 1) break;
+            count++;
+        } else if (!strcmp(str, "%solver_tolerance")) {
+            if (fscanf(f, "%lf", &m.tol) != 1) break;
+            count++;
+        } else if (!strcmp(str, "%number_of_iterations")) {
+            if (fscanf(f, "%u", &m.maxit) != 1) break;
+            count++;
+        } else if (!strcmp(str, "%image_dimensions")) {
+            if (fscanf(f, "%u %u %u", &nx, &ny, &nz) != 3) break;
+            count++;
+        } else if (!strcmp(str, "%refinement")) {
+            if (fscanf(f, "%u", &m.refinement) != 1) break;
+            count++;
+            if (m.refinement < 1) {
+                fclose(f);
+                return false;
+            }
+        } else if (!strcmp(str, "%number_of_materials")) {
+            if (fscanf(f, "%u", &m.nmat) != 1) break;
+            count++;
+            if (m.nmat > 256) {
+                fclose(f);
+                printf("ERROR: Number of different materials (%d) is greater than max (%d).\n", m.nmat, 255);
+                return false;
+            }
+        } else if (!strcmp(str, "%properties_of_materials")) {
+            for (unsigned i = 0; i < m.nmat; i++) {
+                unsigned key;
+                int ok;
+                if (m.analysis == HMG_ELASTIC) ok = fscanf(f, "%u %lf %lf", &key, &m.props[2 * i], &m.props[2 * i + 1]) == 3;
+                else if (m.analysis == HMG_THERMAL) ok = fscanf(f, "%u %lf", &key, &m.props[i]) == 2;
+                else ok = fscanf(f, "%u", &key) == 1;
+                if (!ok) break;
+                count++;
+                if (key > 255) {
+                    fclose(f);
+                    printf("ERROR: Material %d has color key (%d) greater than max (%d).\n", i, key, 255);
+                    return false;
+                }
+                m.props_keys[key] = (uint8_t)i;
+            }
+        } else if (!strcmp(str, "%thermal_expansion")) {
+            if (m.analysis == HMG_ELASTIC) {
+                for (unsigned i = 0; i < m.nmat; i++)
+                    if (fscanf(f, "%lf", &m.alpha[i]) != 1) break;
+                m.thermal_expansion = true;
+            }
+        } else if (!strcmp(str, "%data_type")) {
+            if (fscanf(f, "%255s", str) != 1) break;
+            count++;
+        } else if (!strcmp(str, "%xreduce_scale_factor")) {
+            if (fscanf(f, "%lf", &g_xreduce_stab) != 1) break;
+            stab_set = true;
+            count++;
+        }
+    }
+    fclose(f);
+    if (nz == 0) {
+        m.dim = HMG_2D;
+        printf("ERROR: 2D images are not provided by this library.\n");
+        return false;
+    }
+    m.dim = HMG_3D;
+    if (m.nmat > MATKEY_BITSTEP_RANGE_3D + 1 && m.analysis != HMG_FLUID)
+        printf("WARNING: Number of specified materials (%d) is greater than what can be represented with "
+               "MATKEY_BITSTEP (%d). Either avoid Node-by-Node and DOF-by-DOF, or recompile with greater MATKEY size.\n",
+               m.nmat, MATKEY_BITSTEP_3D);
+    m.elem_size /= m.refinement;
+    m.ex = nx * m.refinement;
+    m.ey = ny * m.refinement;
+    m.ez = nz * m.refinement;
+    if (!stab_set) g_xreduce_stab = m.tol >= 0.0000005 ? m.tol * m.tol : 1e-12;
+    return true;
+}
+
+bool read_image(Model &m, const uint8_t *img)
+{
+    const unsigned ref = m.refinement;
+    const size_t rows = m.ey / ref, cols = m.ex / ref, slices = m.ez / ref;
+    const size_t rows_ref = m.ey, cols_ref = m.ex;
+    m.elem_map.assign(m.nelem, 0);
+    #pragma omp parallel for schedule(static)
+    for (long k = 0; k < (long)slices; k++)
+        for (size_t i = 0; i < rows; i++)
+            for (size_t j = 0; j < cols; j++) {
+                const uint16_t v = m.props_keys[img[(size_t)k * rows * cols + i * cols + j]];
+                for (size_t kk = ref * k; kk < ref * (k + 1); kk++)
+                    for (size_t ii = ref * i; ii < ref * (i + 1); ii++)
+                        for (size_t jj = ref * j; jj < ref * (j + 1); jj++)
+                            m.elem_map[ii + jj * rows_ref + kk * rows_ref * cols_ref] = v;
+            }
+    return true;
+}
+
+bool read_raw(Model &m, const char *filename)
+{
+    const size_t n = (size_t)(m.ex / m.refinement) * (m.ey / m.refinement) * (m.ez / m.refinement);
+    FILE *f = fopen(filename, "rb");
+    if (!f) {
+        printf(" ERROR: Failed to open file.\n");
+        return false;
+    }
+    std::vector<uint8_t> buf(n);
+    const size_t got = fread(buf.data(), 1, n, f);
+    fclose(f);
+    if (got < n) {
+        printf(" ERROR: Reached end of file before expected.\n");
+        return false;
+    }
+    return read_image(m, buf.data());
+}
+
+// model -> solver model (femhmg.c:54-85)
+cudapcgFlag_t model_ctor(cudapcgModel_t **ptr, const void *data)
+{
+    if (!data) return CUDAPCG_FALSE;
+    const Model *m = (const Model *)data;
+    if (m->nmat == 0 || m->nelem == 0) return CUDAPCG_FALSE;
+    cudapcgModel_t *p = *ptr;
+    p->name = (char *)malloc(m->nf_noext.size() + 1);
+    strcpy(p->name, m->nf_noext.c_str());
+    p->parStrategy_flag = g_strategy;
+    p->poremap_flag = m->poremap;
+    p->parametric_density_field_flag = CUDAPCG_FALSE;
+    p->nrows = m->ey;
+    p->ncols = m->ex;
+    p->nlayers = m->ez;
+    p->nelem = (unsigned)m->nelem;
+    p->nvars = (unsigned)m->ndof;
+    p->nvarspernode = m->dim;
+    p->nkeys = m->analysis == HMG_FLUID ? 1 : m->nmat;
+    p->localmtxdim = m->lclMtx_dim;
+    p->nporenodes = m->nvel;
+    p->nbordernodes = m->nborder;
+    p->nhmgvars = m->analysis == HMG_FLUID ? m->nvel * m->dim : (unsigned)m->ndof;
+    return CUDAPCG_TRUE;
+}
+
+void appendf(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+void appendf(const char *fmt, ...)
+{
+    char buf[2048];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    printf("%s", buf);
+    if (M && M->report_on) M->report.push_back(buf);
+}
+
+// load case table: seam axis, seam local nodes, jumped component
+// (femhmg_thermal_3D.c:144-234, femhmg_elastic_3D.c:152-462)
+LoadCase load_case(const Model &m, unsigned dir)
+{
+    static const int AX[6] = {0, 1, 2, 1, 2, 1};
+    static const int SN[6][4] = {{1, 2, 5, 6}, {2, 3, 6, 7}, {0, 1, 2, 3}, {2, 3, 6, 7}, {0, 1, 2, 3}, {2, 3, 6, 7}};
+    static const int CMP[6] = {0, 1, 2, 2, 0, 0};
+    LoadCase lc;
+    lc.axis = AX[dir];
+    for (int t = 0; t < 4; t++) lc.S[t] = SN[dir][t];
+    lc.comp = m.analysis == HMG_ELASTIC ? CMP[dir] : 0;
+    lc.L = lc.axis == 0 ? m.ex : (lc.axis == 1 ? m.ey : m.ez);
+    if (m.analysis == HMG_FLUID) {
+        double f = 0;
+        for (int k = 0; k < 8; k++) f += m.CB[72 + k];
+        lc.L = f;
+    }
+    return lc;
+}
+
+struct DeviceAux {
+    double *CB = nullptr, *seam = nullptr, *K = nullptr;
+};
+
+bool upload_aux(const Model &m, DeviceAux &a)
+{
+    if (m.analysis == HMG_FLUID) return true;
+    const std::vector<double> K = dense_K(m);
+    if (hipMalloc(&a.CB, m.CB.size() * sizeof(double)) != hipSuccess) return false;
+    if (hipMalloc(&a.seam, 6 * m.nmat * sizeof(double)) != hipSuccess) return false;
+    if (hipMalloc(&a.K, K.size() * sizeof(double)) != hipSuccess) return false;
+    hipMemcpy(a.CB, m.CB.data(), m.CB.size() * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(a.K, K.data(), K.size() * sizeof(double), hipMemcpyHostToDevice);
+    return true;
+}
+
+void free_aux(DeviceAux &a)
+{
+    if (a.CB) hipFree(a.CB);
+    if (a.seam) hipFree(a.seam);
+    if (a.K) hipFree(a.K);
+    a = DeviceAux();
+}
+
+bool set_seam(const Model &m, const DeviceAux &a, const LoadCase &lc)
+{
+    if (m.analysis == HMG_FLUID) return true;
+    const int nd = m.analysis == HMG_ELASTIC ? 3 : 1, ke = 8 * nd, nr = nd == 3 ? 6 : 3;
+    std::vector<double> seam(6 * m.nmat, 0.0);
+    for (unsigned t = 0; t < m.nmat; t++)
+        for (int row = 0; row < nr; row++) {
+            double s = 0;
+            for (int k = 0; k < 4; k++) s += m.CB[(size_t)t * nr * ke + row * ke + nd * lc.S[k] + lc.comp];
+            seam[t * 6 + row] = s * lc.L;
+        }
+    return hipMemcpy(a.seam, seam.data(), seam.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+}
+
+// RHS for direction dir into the solver's RHS buffer
+bool device_rhs(const Model &m, const DeviceAux &a, unsigned dir)
+{
+    const LoadCase lc = load_case(m, dir);
+    DeviceModel *dm = active_model();
+    launch_rhs(dm->op64, a.K, lc, (int)dir, rhs_buffer(), active_stream());
+    return hipStreamSynchronize(active_stream()) == hipSuccess;
+}
+
+// column dir of C from the solution in the solver's x buffer
+bool device_updateC(Model &m, const DeviceAux &a, unsigned dir, const double *x)
+{
+    const LoadCase lc = load_case(m, dir);
+    DeviceModel *dm = active_model();
+    Reduce red = make_reduce(FIN_STORE, S_TMP0);
+    if (m.analysis == HMG_FLUID) {
+        launch_fluid_velocity_sum<double>(dm->op64, x, &red, active_stream());
+        if (!sync_scalars()) return false;
+        const double *S = scalars_host();
+        const double mul = m.elem_size * m.elem_size / (double)m.nelem;
+        for (int row = 0; row < 3; row++) m.C[row * 3 + dir] = S[S_TMP0 + row] * mul;
+        return true;
+    }
+    if (!set_seam(m, a, lc)) return false;
+    launch_updateC(dm->op64, a.CB, a.seam, lc, x, &red, active_stream());
+    if (!sync_scalars()) return false;
+    const double *S = scalars_host();
+    const int nr = m.analysis == HMG_ELASTIC ? 6 : 3;
+    for (int row = 0; row < nr; row++) m.C[row * nr + dir] = S[S_TMP0 + row] / (double)m.nelem;
+    return true;
+}
+
+void directions(const Model &m, std::vector<unsigned> &dirs)
+{
+    dirs.clear();
+    if (m.hmg_flag_set) dirs.push_back(m.hmg_flag);
+    else if (m.analysis == HMG_ELASTIC) for (unsigned d = 0; d < 6; d++) dirs.push_back(d);
+    else for (unsigned d = 0; d < 3; d++) dirs.push_back(d);
+}
+
+// set up the solver for model m (femhmg.c:914-1002)
+bool solver_setup(Model &m, DeviceAux &aux)
+{
+    if (!cudapcgInit(m.pcg_flag, m.analysis == HMG_FLUID ? CUDAPCG_NBN : g_strategy)) return false;
+    cudapcgSetXReduceMode(g_xreduce);
+    cudapcgSetXReduceScale(m.elem_size * m.elem_size / (double)m.nelem);
+    cudapcgSetReduceStabFactor(g_xreduce_stab);
+    cudapcgSetHeaderString((char *)"    | ");
+    cudapcgSetModelConstructorFcn(model_ctor);
+    if (!cudapcgBuildModel(&m)) { cudapcgEnd(); return false; }
+    if (!cudapcgSetSolver(g_solver) || !cudapcgSetPreconditioner(g_precond) || !cudapcgSetResNorm(g_stopcrit)) {
+        cudapcgEnd();
+        return false;
+    }
+    cudapcgSetPrecision(g_precision);
+    cudapcgSetNumTol(m.tol);
+    cudapcgSetMaxIterations(m.maxit);
+    if (m.analysis != HMG_FLUID) {
+        if (g_strategy == CUDAPCG_NBN) cudapcgSetImage(m.keys.data());
+        else cudapcgSetImage(m.elem_map.data());
+    } else {
+        cudapcgSetPeriodic2DOFMap(m.dof_id.data());
+        cudapcgSetPoreMap(m.bkeys.data());
+        cudapcgSetDOF2PeriodicMap(m.dof2node.data());
+    }
+    if (!cudapcgSetLclMtxs(m.Mtxs.data())) { cudapcgEnd(); return false; }
+    if (!cudapcgAllocateArrays()) { cudapcgEnd(); return false; }
+    if (!upload_aux(m, aux)) { cudapcgEnd(); return false; }
+    return true;
+}
+
+bool init_sizes(Model &m)
+{
+    if (m.ex == 0 || m.ey == 0 || m.ez == 0 || m.nmat == 0) {
+        printf("ERROR: Failed to initialize model due to bad input parameters.\nrows:%d\ncols:%d\nslices:%d\nmaterials:%d\n",
+               m.ey, m.ex, m.ez, m.nmat);
+        return false;
+    }
+    m.nelem = (size_t)m.ex * m.ey * m.ez;
+    if (m.nelem >= 0xffffffffull) {
+        printf("ERROR: image too large for 32-bit node indices.\n");
+        return false;
+    }
+    if (m.analysis == HMG_THERMAL) {
+        m.C_dim = 9; m.lclMtx_dim = 40; m.lclCB_dim = 0; m.ndof = m.nelem; m.pcg_flag = CUDAPCG_THERMAL_3D;
+    } else if (m.analysis == HMG_ELASTIC) {
+        m.C_dim = 36; m.lclMtx_dim = 576; m.lclCB_dim = 144; m.ndof = 3 * m.nelem; m.pcg_flag = CUDAPCG_ELASTIC_3D;
+    } else {
+        m.C_dim = 9; m.lclMtx_dim = 1024; m.lclCB_dim = 80; m.ndof = m.nelem; m.pcg_flag = CUDAPCG_FLUID_3D;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+logical hmgInit(char *data_filename, char *elem_filename, char *sdf_filename, uint8_t *npdata)
+{
+    g_time_total = omp_get_wtime();
+    printf("Initializing femhmg...\n");
+    if (M) hmgEnd();
+    if (!data_filename) return HMG_FALSE;
+    M = new Model();
+    Model &m = *M;
+    memset(m.props, 0, sizeof(m.props));
+    memset(m.alpha, 0, sizeof(m.alpha));
+    memset(m.props_keys, 0, sizeof(m.props_keys));
+    memset(m.C, 0, sizeof(m.C));
+    m.nf = data_filename;
+    m.nf_noext = m.nf.size() >= 3 ? m.nf.substr(0, m.nf.size() - 3) : m.nf;
+    if (elem_filename) m.image_file = elem_filename;
+    m.poremap = g_poremap_default;
+    if (!read_data(m, data_filename)) { delete M; M = nullptr; return HMG_FALSE; }
+    if (sdf_filename) {
+        printf("ERROR: scalar density field input (.bin) is not provided by this library.\n");
+        delete M; M = nullptr;
+        return HMG_FALSE;
+    }
+    if (!init_sizes(m)) { delete M; M = nullptr; return HMG_FALSE; }
+    bool ok;
+    if (!elem_filename && npdata) ok = read_image(m, npdata);
+    else if (elem_filename && strlen(elem_filename) > 4 && !strcmp(elem_filename + strlen(elem_filename) - 4, ".raw"))
+        ok = read_raw(m, elem_filename);
+    else {
+        printf("ERROR: file %s is not in a valid format.\n", elem_filename ? elem_filename : "(null)");
+        ok = false;
+    }
+    if (!ok) { delete M; M = nullptr; return HMG_FALSE; }
+    if (m.analysis == HMG_FLUID) build_fluid_maps(m);
+    else build_keys(m);
+    assemble_local(m);
+    if (m.thermal_expansion)
+        printf("WARNING: thermal expansion analysis is not provided by this library; alpha is ignored.\n");
+    printf("Done.(%.2e s)\n", omp_get_wtime() - g_time_total);
+    return HMG_TRUE;
+}
+
+logical hmgEnd(void)
+{
+    if (!M) return HMG_FALSE;
+    delete M;
+    M = nullptr;
+    printf("Elapsed time (total): %.2e s\n", omp_get_wtime() - g_time_total);
+    return HMG_TRUE;
+}
+
+void hmgSetXReduceFlag(cudapcgFlag_t flag) { g_xreduce = flag <= CUDAPCG_XREDUCE_FULL ? flag : CUDAPCG_XREDUCE_FULL; }
+void hmgSetParallelStrategyFlag(cudapcgFlag_t flag) { g_strategy = flag ? CUDAPCG_EBE : CUDAPCG_NBN; }
+void hmgSetSolverFlag(cudapcgFlag_t flag) { g_solver = flag; }
+void hmgSetPreConditionerFlag(cudapcgFlag_t flag) { g_precond = flag > 0 ? CUDAPCG_TRUE : CUDAPCG_FALSE; }
+void hmgSetStoppingCriteria(cudapcgFlag_t flag) { g_stopcrit = flag; }
+void hmgSetPrecisionFlag(cudapcgFlag_t flag) { g_precision = flag ? CHFEM_PREC_MIXED : CHFEM_PREC_FP64; }
+
+logical hmgSetPoreMappingStrategy(cudapcgFlag_t flag)
+{
+    if (!M) {
+        g_poremap_default = flag;
+        return HMG_TRUE;
+    }
+    if (flag == M->poremap || flag > CUDAPCG_POREMAP_NUM) return HMG_TRUE;
+    if (M->analysis != HMG_FLUID) {
+        M->poremap = flag;
+        return HMG_TRUE;
+    }
+    printf("ERROR: image pore mapping is not provided by this library; use DOF-number mapping.\n");
+    return HMG_FALSE;
+}
+
+void hmgSetHomogenizationFlag(hmgFlag_t flag)
+{
+    if (!M) return;
+    if (flag > HOMOGENIZE_XY) { M->hmg_flag_set = false; return; }
+    if ((M->analysis == HMG_THERMAL || M->analysis == HMG_FLUID) && flag > HOMOGENIZE_Z) {
+        M->hmg_flag_set = false;
+        return;
+    }
+    M->hmg_flag_set = true;
+    M->hmg_flag = flag;
+}
+
+void hmgExportX(logical flag) { if (M) M->export_x = flag > 0; }
+void hmgImportX(char *file)
+{
+    if (!M || !file) return;
+    M->x0_file = file;
+    M->using_x0 = true;
+}
+
+// Recursive coarse-mesh initial guesses (femhmg.c:561-846): every level halves
+// the element map (taking every second voxel, femhmg.c:1844-1916), is solved
+// with the fine model's settings, and its solution is interpolated onto the
+// next finer level through cudapcgSetX0(x0, mustInterpolate=TRUE).
+void hmgFindInitialGuesses(unsigned int nlevels)
+{
+    if (!nlevels || !M) return;
+    if (M->analysis == HMG_FLUID) {
+        printf("WARNING: Initial guess estimation is not yet available for permeability analysis.\n");
+        printf("         Null initial guess will be considered (x0 = [0]).\n");
+        return;
+    }
+    if (g_solver > CUDAPCG_MINRES3_SOLVER)
+        g_solver = g_solver == CUDAPCG_CG2_SOLVER ? CUDAPCG_CG3_SOLVER : CUDAPCG_MINRES3_SOLVER;
+    std::vector<Model *> lv;
+    lv.push_back(M);
+    for (unsigned i = 0; i < nlevels; i++) {
+        const Model &f = *lv.back();
+        Model *c = new Model(f);
+        c->ex = f.ex / 2; c->ey = f.ey / 2; c->ez = f.ez / 2;
+        c->report_on = false;
+        c->x0.clear();
+        c->using_x0 = false;
+        c->x0_file.clear();
+        c->export_x = false;
+        if (c->ex == 0 || c->ey == 0 || c->ez == 0) { delete c; break; }
+        c->nelem = (size_t)c->ex * c->ey * c->ez;
+        c->ndof = c->analysis == HMG_ELASTIC ? 3 * c->nelem : c->nelem;
+        c->elem_map.assign(c->nelem, 0);
+        for (unsigned r = 0; r < c->ey; r++)
+            for (unsigned col = 0; col < c->ex; col++)
+                for (unsigned l = 0; l < c->ez; l++)
+                    c->elem_map[r + (size_t)col * c->ey + (size_t)l * c->ex * c->ey] =
+                        f.elem_map[2 * r + (size_t)2 * col * f.ey + (size_t)2 * l * f.ex * f.ey];
+        build_keys(*c);
+        lv.push_back(c);
+    }
+    std::vector<unsigned> dirs;
+    directions(*M, dirs);
+    for (size_t i = lv.size() - 1; i > 0; i--) {
+        Model &c = *lv[i];
+        Model &finer = *lv[i - 1];
+        printf("Computing initial guesses in coarse mesh %zu...\n", i);
+        DeviceAux aux;
+        if (!solver_setup(c, aux)) continue;
+        finer.x0.assign(6, std::vector<double>());
+        for (unsigned d : dirs) {
+            if (!c.x0.empty() && !c.x0[d].empty()) cudapcgSetX0(c.x0[d].data(), CUDAPCG_TRUE);
+            else cudapcgSetX0(nullptr, CUDAPCG_FALSE);
+            device_rhs(c, aux, d);
+            if (solve_device(nullptr) == 1) {
+                finer.x0[d].resize(c.ndof);
+                hipMemcpy(finer.x0[d].data(), solution_buffer(), c.ndof * sizeof(double), hipMemcpyDeviceToHost);
+            }
+        }
+        free_aux(aux);
+        cudapcgEnd();
+        if (i > 0) finer.using_x0 = true;
+    }
+    for (size_t i = 1; i < lv.size(); i++) delete lv[i];
+}
+
+void hmgSolveHomogenization(void)
+{
+    if (!M) return;
+    Model &m = *M;
+    if (m.analysis != HMG_FLUID && g_solver > CUDAPCG_MINRES3_SOLVER) {
+        printf("WARNING: CG2 and MINRES2 are only available for FLUID.\n");
+        g_solver = g_solver == CUDAPCG_CG2_SOLVER ? CUDAPCG_CG3_SOLVER : CUDAPCG_MINRES3_SOLVER;
+    }
+    for (unsigned i = 0; i < m.C_dim; i++) m.C[i] = 0.0;
+    m.stats.clear();
+    const double t_hmg = omp_get_wtime();
+    appendf("Running homogenization...\n");
+    DeviceAux aux;
+    if (!solver_setup(m, aux)) return;
+    static const char sim_str[6][11] = {"X", "Y", "Z", "YZ (shear)", "XZ (shear)", "XY (shear)"};
+    std::vector<unsigned> dirs;
+    directions(m, dirs);
+    std::vector<double> xfile;
+    for (unsigned d : dirs) {
+        appendf("    ------------------------------------------------------\n    Analysis on %s:\n", sim_str[d]);
+        m.hmg_flag = d;
+        device_rhs(m, aux, d);
+        if (m.using_x0) {
+            if (m.x0_file.empty()) {
+                if (!m.x0.empty() && !m.x0[d].empty()) cudapcgSetX0(m.x0[d].data(), CUDAPCG_TRUE);
+                else cudapcgSetX0(nullptr, CUDAPCG_FALSE);
+            } else {
+                char fb[2048];
+                snprintf(fb, sizeof(fb), "%s_%d.bin", m.x0_file.c_str(), d);
+                FILE *f = fopen(fb, "rb");
+                if (!f) {
+                    printf("    | Reading initial guess from %s...Failed!...\n", fb);
+                    cudapcgSetX0(nullptr, CUDAPCG_FALSE);
+                } else {
+                    xfile.assign(m.ndof, 0.0);
+                    if (fread(xfile.data(), sizeof(double), m.ndof, f) < m.ndof)
+                        printf("WARNING: Failed to read all %zu items from %s...", m.ndof, fb);
+                    fclose(f);
+                    cudapcgSetX0(xfile.data(), CUDAPCG_FALSE);
+                }
+            }
+        }
+        const int rc = solve_device(nullptr);
+        chfemSolveStats_t st;
+        cudapcgGetSolveStats(&st);
+        m.stats.push_back(st);
+        appendf("    | Solver ran %u iterations (%u sweeps), residual %.3e, %.3e s\n", st.iterations, st.sweeps,
+                st.residual, st.solve_seconds);
+        if (rc < 0) break;
+        if (m.export_x) {
+            std::vector<double> xh(m.ndof);
+            hipMemcpy(xh.data(), solution_buffer(), m.ndof * sizeof(double), hipMemcpyDeviceToHost);
+            char fb[2048];
+            snprintf(fb, sizeof(fb), "%s_X_%d.bin", m.nf_noext.c_str(), d);
+            FILE *f = fopen(fb, "wb");
+            if (f) {
+                fwrite(xh.data(), sizeof(double), m.ndof, f);
+                fclose(f);
+            }
+        }
+        if (g_solver >= CUDAPCG_CG2_SOLVER && m.analysis == HMG_FLUID) {
+            // two-vector solvers report the effective field directly
+            device_updateC(m, aux, d, solution_buffer());
+            printf("    Effective fields:\n");
+            for (unsigned jj = 0; jj < 3; jj++) printf("        %.8e\n", m.C[jj * 3 + d]);
+        } else {
+            device_updateC(m, aux, d, solution_buffer());
+        }
+    }
+    if (m.save_fields) printf("WARNING: field export is not provided by this library.\n");
+    appendf("    ------------------------------------------------------\n");
+    free_aux(aux);
+    cudapcgFreeArrays();
+    cudapcgSetX0(nullptr, CUDAPCG_FALSE);
+    cudapcgEnd();
+    printf("Done.(%.2e s)\n", omp_get_wtime() - t_hmg);
+}
+
+void hmgPrintModelData(void)
+{
+    if (!M) return;
+    static const char *an[3] = {"THERMAL_3D", "ELASTIC_3D", "FLUID_3D"};
+    printf("*******************************************************\nMODEL DATA:\nAnalysis: %s\n", an[M->analysis]);
+    printf("Mesh: [%u,%u,%u] elements, %zu DOFs\n", M->ex, M->ey, M->ez, M->ndof);
+    if (M->analysis == HMG_FLUID)
+        printf("  | porosity: %05.2f%%\n  | within pores: %u\n  | at interfaces: %u\n",
+               100.0 * M->nporeelem / (double)M->nelem, M->nvel, M->nborder);
+    printf("Solver tolerance: %.3e, max iterations: %u\n*******************************************************\n", M->tol,
+           M->maxit);
+}
+
+void hmgPrintConstitutiveMtx(void)
+{
+    if (!M) return;
+    const unsigned n = M->analysis == HMG_ELASTIC ? 6 : 3;
+    printf("-------------------------------------------------------\nHomogenized Constitutive Matrix:\n");
+    for (unsigned i = 0; i < n; i++) {
+        printf(" ");
+        for (unsigned j = 0; j < n; j++) printf(" %.8e", M->C[i * n + j]);
+        printf("\n");
+    }
+    printf("-------------------------------------------------------\n");
+}
+
+var *hmgGetConstitutiveMtx(void) { return M ? M->C : nullptr; }
+unsigned int hmgGetConstitutiveMtxDim(void) { return M ? M->C_dim : 0; }
+var *hmgGetThermalExpansion(void) { return nullptr; }
+unsigned int hmgGetThermalExpansionDim(void) { return 0; }
+
+logical hmgSaveConstitutiveMtx(const char *filename)
+{
+    if (!M || !filename) return HMG_FALSE;
+    FILE *f = fopen(filename, "wb");
+    if (!f) return HMG_FALSE;
+    fwrite(M->C, sizeof(double), M->C_dim, f);
+    fclose(f);
+    return HMG_TRUE;
+}
+
+void hmgSaveFields(logical mustExport_flag, logical byElems_flag)
+{
+    (void)byElems_flag;
+    if (M) M->save_fields = mustExport_flag > 0;
+}
+
+void hmgKeepTrackOfAnalysisReport(reportFlag_t flag) { if (M) M->report_on = flag > 0; }
+
+reportFlag_t hmgPrintReport(const char *filename)
+{
+    if (!M || !M->report_on || !filename) return 0;
+    FILE *f = fopen(filename, "w");
+    if (!f) return 0;
+    for (auto &s : M->report) fputs(s.c_str(), f);
+    fclose(f);
+    return 1;
+}
+
+cudapcgFlag_t hmgGetSolveStats(chfemSolveStats_t *stats, unsigned int *count)
+{
+    if (!M || !stats || !count) return CUDAPCG_FALSE;
+    *count = (unsigned)M->stats.size();
+    for (size_t i = 0; i < M->stats.size(); i++) stats[i] = M->stats[i];
+    return CUDAPCG_TRUE;
+}
+
+logical hmgGetSizes(unsigned int *nelem, unsigned int *ndof, unsigned int *nvel, unsigned int *nborder)
+{
+    if (!M) return HMG_FALSE;
+    if (nelem) *nelem = (unsigned)M->nelem;
+    if (ndof) *ndof = (unsigned)M->ndof;
+    if (nvel) *nvel = M->nvel;
+    if (nborder) *nborder = M->nborder;
+    return HMG_TRUE;
+}
+
+logical hmgGetLocalMatrices(double *Mtxs, double *CB)
+{
+    if (!M) return HMG_FALSE;
+    if (Mtxs) memcpy(Mtxs, M->Mtxs.data(), M->Mtxs.size() * sizeof(double));
+    if (CB) memcpy(CB, M->CB.data(), M->CB.size() * sizeof(double));
+    return HMG_TRUE;
+}
+
+logical hmgGetMaps(uint16_t *keys, uint8_t *fluid_keys, uint32_t *dof_id, uint32_t *dof2node, uint8_t *border_keys)
+{
+    if (!M) return HMG_FALSE;
+    if (keys && !M->keys.empty()) memcpy(keys, M->keys.data(), M->keys.size() * 2);
+    if (fluid_keys && !M->fkeys.empty()) memcpy(fluid_keys, M->fkeys.data(), M->fkeys.size());
+    if (dof_id && !M->dof_id.empty()) memcpy(dof_id, M->dof_id.data(), M->dof_id.size() * 4);
+    if (dof2node && !M->dof2node.empty()) memcpy(dof2node, M->dof2node.data(), M->dof2node.size() * 4);
+    if (border_keys && !M->bkeys.empty()) memcpy(border_keys, M->bkeys.data(), M->bkeys.size());
+    return HMG_TRUE;
+}
+
+// RHS / C column on the device for the current model, through a temporary
+// solver set up as hmgSolveHomogenization does.
+logical hmgAssembleRHS(unsigned int dir, cudapcgVar_t *b)
+{
+    if (!M || !b) return HMG_FALSE;
+    DeviceAux aux;
+    if (!solver_setup(*M, aux)) return HMG_FALSE;
+    bool ok = device_rhs(*M, aux, dir) &&
+              hipMemcpy(b, rhs_buffer(), M->ndof * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess;
+    free_aux(aux);
+    cudapcgEnd();
+    return ok ? HMG_TRUE : HMG_FALSE;
+}
+
+logical hmgUpdateC(unsigned int dir, const cudapcgVar_t *x, var *C)
+{
+    if (!M || !x || !C) return HMG_FALSE;
+    DeviceAux aux;
+    if (!solver_setup(*M, aux)) return HMG_FALSE;
+    bool ok = hipMemcpy(solution_buffer(), x, M->ndof * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+              device_updateC(*M, aux, dir, solution_buffer());
+    if (ok) memcpy(C, M->C, M->C_dim * sizeof(double));
+    free_aux(aux);
+    cudapcgEnd();
+    return ok ? HMG_TRUE : HMG_FALSE;
+}
+
+// chfem/src/main.c:300-323
+void initDefaultInput(chfemgpuInput_t *u)
+{
+    memset(u, 0, sizeof(*u));
+    u->hmg_direction_flag = HOMOGENIZE_ALL;
+    u->solver_flag = 0;
+    u->preconditioner_flag = 1;
+    u->parallel_flag = 0;
+    u->stopcrit_flag = 0;
+    u->poremap_flag = 1;
+    u->xreduce_flag = 2;
+}
+
+// chfem/src/main.c:72-152
+int runAnalysis(chfemgpuInput_t *u)
+{
+    if (u->writeReport_flag) hmgKeepTrackOfAnalysisReport(1);
+    hmgSaveFields(u->exportFields_flag, u->fieldsByElem_flag);
+    hmgSetParallelStrategyFlag(u->parallel_flag);
+    if (!hmgSetPoreMappingStrategy(u->poremap_flag)) {
+        printf("ERROR: Failed to set \"poremap_flag\"=%d.\nProcess aborted.\n", u->poremap_flag);
+        hmgEnd();
+        return -1;
+    }
+    hmgSetSolverFlag(u->solver_flag);
+    hmgSetPreConditionerFlag(u->preconditioner_flag);
+    hmgSetXReduceFlag(u->xreduce_flag);
+    hmgSetHomogenizationFlag(u->hmg_direction_flag);
+    hmgPrintModelData();
+    hmgSetStoppingCriteria(u->stopcrit_flag);
+    if (u->importX_flag) hmgImportX(u->x0_file);
+    else hmgFindInitialGuesses(u->num_of_recursions_initguess);
+    hmgExportX(u->exportX_flag);
+    hmgSolveHomogenization();
+    hmgPrintConstitutiveMtx();
+    hmgPrintReport(u->report_file);
+    if (u->save2binary_flag) hmgSaveConstitutiveMtx(u->binary_file);
+    const double *C = hmgGetConstitutiveMtx();
+    const unsigned cd = hmgGetConstitutiveMtxDim();
+    for (unsigned i = 0; i < cd; i++) u->eff_coeff[i] = C[i];
+    u->num_of_thermal_expansion_coeffs = 0;
+    hmgEnd();
+    printf("#######################################################\n");
+    return 0;
+}
+
+// chfem/src/chfem_wrapper.c:7-89, as a plain C ABI
+int chfem_run(const uint8_t *img, size_t img_size, const char *nf_file, int analysis, int direction, int solver,
+              int precondition, int output_fields, double *eff_out, unsigned *n_out, unsigned *alpha_out)
+{
+    (void)img_size;
+    chfemgpuInput_t u;
+    initDefaultInput(&u);
+    u.hmg_direction_flag = (unsigned)direction;
+    u.solver_flag = (unsigned)solver;
+    u.preconditioner_flag = (unsigned)precondition;
+    u.exportFields_flag = (unsigned char)output_fields;
+    if (!hmgInit((char *)nf_file, nullptr, nullptr, (uint8_t *)img)) {
+        printf("Failed to properly read input files.\nProcess aborted.\n");
+        return -1;
+    }
+    if (runAnalysis(&u) != 0) return -1;
+    const unsigned n = (analysis == 0 || analysis == 2) ? 3 : 6;
+    for (unsigned i = 0; i < n * n; i++) eff_out[i] = u.eff_coeff[i];
+    if (n_out) *n_out = n;
+    if (alpha_out) *alpha_out = u.num_of_thermal_expansion_coeffs;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+// kernels.h -- host-side launchers of the gfx950 kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace chfem {
+
+// upper bound on the blocks of any reducing launch (partials buffer size)
+#define MAX_PARTIAL_BLOCKS 8192
+
+enum Physics { PHY_THERMAL = 0, PHY_ELASTIC = 1, PHY_FLUID = 2 };
+
+// Device-resident scalars of one Krylov solve (FP64).
+enum ScalarSlot {
+    S_DELTA = 0, S_DELTA_OLD, S_DELTA0, S_DQ, S_ALPHA, S_STOP, S_RES0, S_TMP0, S_TMP1,
+    S_TMP2, S_TMP3, S_TMP4, S_TMP5, S_TMP6, S_TMP7, S_COUNT = 16
+};
+
+// How the last block of a fused reduction publishes its total.
+enum Finalize {
+    FIN_STORE = 0,      // S[slot] = sum
+    FIN_ALPHA = 1,      // S[S_DQ] = sum; S[S_ALPHA] = S[S_DELTA] / sum
+    FIN_DELTA = 2,      // S[S_DELTA_OLD] = S[S_DELTA]; S[S_DELTA] = sum
+};
+
+struct Reduce {
+    double *partials;    // >= gridDim.x * 8 doubles
+    unsigned *counter;   // zero-initialised
+    double *S;           // scalar slots
+    int fin;
+    int slot;
+};
+
+// Geometry and material data of one operator, all device pointers.
+struct OpDesc {
+    int physics;         // Physics
+    int ebe;             // 0 NBN (keys), 1 EBE (element map)
+    int ny, nx, nz;      // rows, cols, layers (elements)
+    uint32_t N;          // nodes = elements
+    size_t ndof;
+    const uint16_t *keys;      // NBN packed keys, or EBE element material map
+    const void *K;             // per-material dense local matrices in the operator precision
+    // Stokes
+    uint32_t nvel, nborder;
+    const uint32_t *dof2node;  // DOF id -> periodic node
+    const uint32_t *node2dof;  // periodic node -> DOF id
+    const uint8_t *bkeys;      // border node fluid keys
+};
+
+template <typename T> void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s);
+
+// dinv[i] = 1 / diag(A)_i in precision T, from the keys (thermal/elastic) or the
+// Stokes pore/border structure.
+template <typename T> void launch_jacobi(const OpDesc &op, T *dinv, hipStream_t s);
+
+// Vector kernels of the solvers (n entries, precision T, reductions FP64).
+template <typename T> void launch_fill(T *x, double val, size_t n, hipStream_t s);
+template <typename Tin, typename Tout> void launch_convert(const Tin *in, Tout *out, size_t n, hipStream_t s);
+// out = dinv*v ; optional reduction of v.out
+template <typename T> void launch_precond(const T *dinv, const T *v, T *out, size_t n, const Reduce *red, hipStream_t s);
+// d = z + beta*d, beta = S[DELTA]/S[DELTA_OLD] (first: d = z)
+template <typename T> void launch_pcg_dir(T *d, const T *z, const double *S, int first, size_t n, hipStream_t s);
+// x += a d ; r -= a q ; z = dinv r ; reduce r.z  (a = S[ALPHA])
+template <typename T> void launch_pcg_update(T *x, T *r, const T *d, T *z, const T *dinv, const Reduce *red,
+                                             size_t n, hipStream_t s);
+// r -= q ; z = dinv r ; reduce r.z
+template <typename T> void launch_sub_precond(T *r, const T *q, const T *dinv, T *z, const Reduce *red, size_t n,
+                                              hipStream_t s);
+// MINRES: d = rt + b d ; q = s + b q ; reduce q.(dinv q) ; b = S[DELTA]/S[DELTA_OLD] (first: d=rt, q=s)
+template <typename T> void launch_minres_dir(T *d, T *q, const T *rt, const T *sv, const T *dinv, const double *S,
+                                             int first, const Reduce *red, size_t n, hipStream_t s);
+// MINRES: rn = rt/dinv - a q ; rt = dinv rn ; x += a d ; reduce rn.rt
+template <typename T> void launch_minres_update(T *x, T *rt, const T *d, const T *q, const T *dinv, const Reduce *red,
+                                                size_t n, hipStream_t s);
+// FP64 outer refinement: x64 += e32
+void launch_add_correction(double *x, const float *e, size_t n, hipStream_t s);
+// dot reduction of two vectors
+template <typename T> void launch_dot(const T *a, const T *b, size_t n, const Reduce *red, hipStream_t s);
+
+// Right-hand side of load case dir (FP64), and effective-tensor column.
+struct LoadCase { int axis; int S[4]; int comp; double L; };
+void launch_rhs(const OpDesc &op, const double *K64, const LoadCase &lc, int dir, double *b, hipStream_t s);
+// seamCB: per material, per C row: sum over seam columns of CB (already x L)
+void launch_updateC(const OpDesc &op, const double *CB64, const double *seamCB, const LoadCase &lc,
+                    const double *x, const Reduce *red, hipStream_t s);
+template <typename T> void launch_fluid_velocity_sum(const OpDesc &op, const T *x, const Reduce *red, hipStream_t s);
+
+// x0 from a half-resolution solution (rows/cols/layers = fine element counts)
+void launch_interpl2(const double *coarse, double *fine, int rows, int cols, int layers, int stride, hipStream_t s);
+
+int reduce_blocks(size_t n);
+
+}  // namespace chfem

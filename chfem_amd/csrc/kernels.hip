@@ -1,0 +1,879 @@
+// kernels.hip -- gfx950 kernels of the chfem matrix-free solve path.
+//
+// Operators (q = A v) for the periodic voxel mesh:
+//   thermal / elastic node-by-node (NBN): one work-item per node, the 27-node
+//     neighbourhood gathered into registers once, and for each of the node's 8
+//     elements the rows of that element's local matrix applied to the element's
+//     nodal values (chfem/src/femhmg/cudapcg/kernels/matrix_free.cu:191-323,
+//     :416-459).  The material of an element varies per lane; the kernel walks
+//     the distinct materials present in the wavefront (ballot + readlane), so
+//     every local-matrix entry is a wave-uniform scalar load instead of a
+//     divergent constant-memory read.
+//   thermal / elastic element-by-element (EBE): one work-item per element,
+//     K_e u_e scattered with FP64/FP32 atomics (matrix_free.cu:2187-2268,
+//     :3164-3262).
+//   Stokes DOF-number pore mapping: one work-item per pore/border DOF id
+//     (matrix_free.cu:674-732, :927-977).
+// Reductions are FP64 and deterministic: per-block tree sums, then the last
+// block to finish sums the block partials in a fixed order.
+#include <hip/hip_runtime.h>
+#include "kernels.h"
+
+namespace chfem {
+
+#define BS 256
+#define MAX_RED_BLOCKS MAX_PARTIAL_BLOCKS
+
+__constant__ int cDR[8] = {1, 1, 0, 0, 1, 1, 0, 0};
+__constant__ int cDC[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+__constant__ int cDL[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+// compile-time copies for index arithmetic that must fold after unrolling
+#define LDR(k) ((k) == 0 || (k) == 1 || (k) == 4 || (k) == 5 ? 1 : 0)
+#define LDC(k) ((k) == 1 || (k) == 2 || (k) == 5 || (k) == 6 ? 1 : 0)
+#define LDL(k) ((k) >= 4 ? 1 : 0)
+
+int reduce_blocks(size_t n)
+{
+    size_t b = (n + BS - 1) / BS;
+    if (b > MAX_RED_BLOCKS) b = MAX_RED_BLOCKS;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+// ---------------------------------------------------------------------------
+// deterministic reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// block sum; result valid in thread 0
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV])
+{
+    __shared__ double lds[NV][BS / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double t = wave_sum(v[k]);
+        if (lane == 0) lds[k][w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double t = 0;
+            for (int i = 0; i < BS / 64; i++) t += lds[k][i];
+            v[k] = t;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ double load_coherent(const double *p)
+{
+    unsigned long long u = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)u);
+}
+
+template <int NV>
+__device__ void reduce_finalize(const Reduce &red, double (&v)[NV])
+{
+    __shared__ int amLast;
+    block_sum<NV>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) red.partials[(size_t)k * gridDim.x + blockIdx.x] = v[k];
+        __threadfence();
+        unsigned prev = atomicAdd(red.counter, 1u);
+        amLast = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (amLast) {
+        __threadfence();
+        double s[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            s[k] = 0;
+            for (unsigned b = threadIdx.x; b < gridDim.x; b += BS) s[k] += load_coherent(red.partials + (size_t)k * gridDim.x + b);
+        }
+        block_sum<NV>(s);
+        if (threadIdx.x == 0) {
+            double *S = red.S;
+            if (red.fin == FIN_ALPHA) {
+                S[S_DQ] = s[0];
+                S[S_ALPHA] = S[S_DELTA] / s[0];
+            } else if (red.fin == FIN_DELTA) {
+                S[S_DELTA_OLD] = S[S_DELTA];
+                S[S_DELTA] = s[0];
+            } else {
+#pragma unroll
+                for (int k = 0; k < NV; k++) S[red.slot + k] = s[k];
+            }
+            __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// neighbourhood helpers
+// ---------------------------------------------------------------------------
+struct Nbh {
+    uint32_t ro[3];   // row index r-1, r, r+1 (periodic)
+    uint32_t co[3];   // column offset (c-1, c, c+1) * ny
+    size_t lo[3];     // layer offset (l-1, l, l+1) * nx*ny
+};
+
+__device__ __forceinline__ void node_rcl(uint32_t i, int ny, int nx, int &r, int &c, int &l)
+{
+    uint32_t t = i / (uint32_t)ny;
+    r = (int)(i - t * (uint32_t)ny);
+    l = (int)(t / (uint32_t)nx);
+    c = (int)(t - (uint32_t)l * (uint32_t)nx);
+}
+
+__device__ __forceinline__ Nbh make_nbh(int r, int c, int l, int ny, int nx, int nz)
+{
+    Nbh h;
+    const size_t nxy = (size_t)nx * ny;
+    h.ro[0] = r == 0 ? ny - 1 : r - 1; h.ro[1] = r; h.ro[2] = r == ny - 1 ? 0 : r + 1;
+    h.co[0] = (uint32_t)(c == 0 ? nx - 1 : c - 1) * ny; h.co[1] = (uint32_t)c * ny;
+    h.co[2] = (uint32_t)(c == nx - 1 ? 0 : c + 1) * ny;
+    h.lo[0] = (size_t)(l == 0 ? nz - 1 : l - 1) * nxy; h.lo[1] = (size_t)l * nxy;
+    h.lo[2] = (size_t)(l == nz - 1 ? 0 : l + 1) * nxy;
+    return h;
+}
+
+// View of read-only device data in the constant address space: loads with a
+// wave-uniform address become scalar (SMEM) loads.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T *p) { return (cptr<T>)p; }
+
+// Visit the distinct values of m across the active lanes of the wavefront;
+// body(mu) runs with mu wave-uniform and only the lanes with m == mu active,
+// so data indexed by mu is read with scalar loads.
+template <typename F>
+__device__ __forceinline__ void for_each_uniform(const int m, F &&body)
+{
+    unsigned long long pend = __ballot(1);
+    while (pend) {
+        const int lead = __ffsll((long long)pend) - 1;
+        const int mu = __builtin_amdgcn_readlane(m, lead);
+        if (m == mu) {
+            // re-read as a wave-uniform value inside the branch: the compiler
+            // may otherwise rewrite mu as the per-lane m (equal on the active
+            // lanes) and address the local matrix with vector loads
+            body(__builtin_amdgcn_readfirstlane(mu));
+        }
+        pend &= ~__ballot(m == mu);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// thermal NBN: K layout [mat][8][8]
+// ---------------------------------------------------------------------------
+template <typename T, bool DOT>
+__global__ void __launch_bounds__(BS) k_thermal_nbn(const T *__restrict__ v, T *__restrict__ q,
+                                                    const uint16_t *__restrict__ keys, const T *__restrict__ K,
+                                                    int ny, int nx, int nz, uint32_t N, Reduce red)
+{
+    double acc = 0.0;
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS) {
+        int r, c, l;
+        node_rcl(i, ny, nx, r, c, l);
+        const Nbh h = make_nbh(r, c, l, ny, nx, nz);
+        T u[3][3][3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) u[a][b][d] = v[h.lo[a] + h.co[b] + h.ro[d]];
+        const unsigned key = keys[i];
+        T res = 0;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int me = (key >> (2 * e)) & 3;
+            for_each_uniform(me, [&](const int mu) {
+                cptr<T> Km = as_const(K) + mu * 64 + e * 8;
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    res = fma(Km[j], u[LDL(j) - LDL(e) + 1][LDC(j) - LDC(e) + 1][LDR(j) - LDR(e) + 1], res);
+            });
+        }
+        q[i] = res;
+        if (DOT) acc += (double)u[1][1][1] * (double)res;
+    }
+    if (DOT) {
+        double vv[1] = {acc};
+        reduce_finalize<1>(red, vv);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// elastic NBN: K layout [mat][24][24], DOFs node-major (x,y,z)
+// ---------------------------------------------------------------------------
+template <typename T, bool DOT>
+__global__ void __launch_bounds__(BS) k_elastic_nbn(const T *__restrict__ v, T *__restrict__ q,
+                                                    const uint16_t *__restrict__ keys, const T *__restrict__ K,
+                                                    int ny, int nx, int nz, uint32_t N, Reduce red)
+{
+    double acc = 0.0;
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS) {
+        int r, c, l;
+        node_rcl(i, ny, nx, r, c, l);
+        const Nbh h = make_nbh(r, c, l, ny, nx, nz);
+        T u[3][3][3][3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const T *p = v + 3 * (h.lo[a] + h.co[b] + h.ro[d]);
+                    u[a][b][d][0] = p[0]; u[a][b][d][1] = p[1]; u[a][b][d][2] = p[2];
+                }
+        const unsigned key = keys[i];
+        T rx = 0, ry = 0, rz = 0;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int me = (key >> (2 * e)) & 3;
+            for_each_uniform(me, [&](const int mu) {
+                cptr<T> Km = as_const(K) + mu * 576 + e * 72;
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const T *uu = u[LDL(j) - LDL(e) + 1][LDC(j) - LDC(e) + 1][LDR(j) - LDR(e) + 1];
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        rx = fma(Km[3 * j + b], uu[b], rx);
+                        ry = fma(Km[24 + 3 * j + b], uu[b], ry);
+                        rz = fma(Km[48 + 3 * j + b], uu[b], rz);
+                    }
+                }
+            });
+        }
+        T *o = q + 3 * (size_t)i;
+        o[0] = rx; o[1] = ry; o[2] = rz;
+        if (DOT) acc += (double)u[1][1][1][0] * rx + (double)u[1][1][1][1] * ry + (double)u[1][1][1][2] * rz;
+    }
+    if (DOT) {
+        double vv[1] = {acc};
+        reduce_finalize<1>(red, vv);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// EBE: one element per work-item; element map gives the material; the
+// element's 8*ND outputs are added to q with atomics (q zeroed beforehand).
+// ---------------------------------------------------------------------------
+template <typename T, int ND>
+__global__ void __launch_bounds__(BS) k_ebe(const T *__restrict__ v, T *__restrict__ q,
+                                            const uint16_t *__restrict__ emap, const T *__restrict__ K,
+                                            int ny, int nx, int nz, uint32_t N)
+{
+    constexpr int KE = 8 * ND;
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS) {
+        int r, c, l;
+        node_rcl(i, ny, nx, r, c, l);
+        const Nbh h = make_nbh(r, c, l, ny, nx, nz);
+        size_t idx[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) idx[k] = h.lo[1 + LDL(k)] + h.co[1 + LDC(k)] + h.ro[1 + LDR(k)];
+        T ue[KE];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+#pragma unroll
+            for (int a = 0; a < ND; a++) ue[ND * k + a] = v[ND * idx[k] + a];
+        const int me = emap[i];
+        T fe[KE];
+#pragma unroll
+        for (int t = 0; t < KE; t++) fe[t] = 0;
+        for_each_uniform(me, [&](const int mu) {
+            cptr<T> Km = as_const(K) + mu * KE * KE;
+#pragma unroll
+            for (int t = 0; t < KE; t++) {
+                T s = 0;
+#pragma unroll
+                for (int j = 0; j < KE; j++) s = fma(Km[t * KE + j], ue[j], s);
+                fe[t] = s;
+            }
+        });
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+#pragma unroll
+            for (int a = 0; a < ND; a++) atomicAdd(q + ND * idx[k] + a, fe[ND * k + a]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stokes, DOF-number pore mapping.  Vector layout [3*nvel velocities |
+// nvel+nborder pressures]; K is the single 32x32 [K G; G^T S] matrix; pressure
+// rows use -S (matrix_free.cu:674-732).
+// ---------------------------------------------------------------------------
+template <typename T, bool DOT>
+__global__ void __launch_bounds__(BS) k_fluid(const T *__restrict__ v, T *__restrict__ q,
+                                              const uint32_t *__restrict__ dof2node,
+                                              const uint32_t *__restrict__ node2dof,
+                                              const uint8_t *__restrict__ bkeys, const T *__restrict__ K,
+                                              int ny, int nx, int nz, uint32_t nvel, uint32_t nborder, Reduce red)
+{
+    const uint32_t ntot = nvel + nborder;
+    double acc = 0.0;
+    for (uint32_t t = blockIdx.x * BS + threadIdx.x; t < ntot; t += gridDim.x * BS) {
+        const uint32_t node = dof2node[t];
+        int r, c, l;
+        node_rcl(node, ny, nx, r, c, l);
+        const Nbh h = make_nbh(r, c, l, ny, nx, nz);
+        const bool pore = t < nvel;
+        const unsigned fkey = pore ? 255u : bkeys[t - nvel];
+        T uv[3][3][3][3], up[3][3][3];
+        bool hv[3][3][3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const uint32_t id = node2dof[h.lo[a] + h.co[b] + h.ro[d]];
+                    const bool isv = id < nvel;
+                    hv[a][b][d] = isv;
+                    const size_t vi = isv ? 3 * (size_t)id : 0;
+                    uv[a][b][d][0] = isv ? v[vi] : T(0);
+                    uv[a][b][d][1] = isv ? v[vi + 1] : T(0);
+                    uv[a][b][d][2] = isv ? v[vi + 2] : T(0);
+                    up[a][b][d] = v[3 * (size_t)nvel + (id < ntot ? id : 0)];
+                }
+        T rx = 0, ry = 0, rz = 0, rp = 0;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            if ((fkey >> e) & 1u) {
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int A = LDL(j) - LDL(e) + 1, B = LDC(j) - LDC(e) + 1, D = LDR(j) - LDR(e) + 1;
+                    const T *uu = uv[A][B][D];
+                    const T p = up[A][B][D];
+                    if (pore) {
+                        const T *k0 = K + (3 * e) * 32, *k1 = k0 + 32, *k2 = k0 + 64;
+                        rx += k0[3 * j] * uu[0] + k0[3 * j + 1] * uu[1] + k0[3 * j + 2] * uu[2] + k0[24 + j] * p;
+                        ry += k1[3 * j] * uu[0] + k1[3 * j + 1] * uu[1] + k1[3 * j + 2] * uu[2] + k1[24 + j] * p;
+                        rz += k2[3 * j] * uu[0] + k2[3 * j + 1] * uu[1] + k2[3 * j + 2] * uu[2] + k2[24 + j] * p;
+                    }
+                    const T *kp = K + (24 + e) * 32;
+                    rp += kp[3 * j] * uu[0] + kp[3 * j + 1] * uu[1] + kp[3 * j + 2] * uu[2] - kp[24 + j] * p;
+                }
+            }
+        }
+        (void)hv;
+        if (pore) {
+            T *o = q + 3 * (size_t)t;
+            o[0] = rx; o[1] = ry; o[2] = rz;
+            if (DOT) acc += (double)uv[1][1][1][0] * rx + (double)uv[1][1][1][1] * ry + (double)uv[1][1][1][2] * rz;
+        }
+        q[3 * (size_t)nvel + t] = rp;
+        if (DOT) acc += (double)up[1][1][1] * rp;
+    }
+    if (DOT) {
+        double vv[1] = {acc};
+        reduce_finalize<1>(red, vv);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(BS) k_fill(T *x, T val, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) x[i] = val;
+}
+
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(BS) k_convert(const Tin *in, Tout *out, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) out[i] = (Tout)in[i];
+}
+
+template <typename T, bool RED>
+__global__ void __launch_bounds__(BS) k_precond(const T *__restrict__ dinv, const T *v, T *out, size_t n,
+                                                Reduce red)
+{
+    double acc = 0;
+    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
+        T vi = v[i];
+        T o = dinv[i] * vi;
+        if (out) out[i] = o;
+        if (RED) acc += (double)vi * (double)o;
+    }
+    if (RED) {
+        double vv[1] = {acc};
+        reduce_finalize<1>(red, vv);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BS) k_dot(const T *__restrict__ a, const T *__restrict__ b, size_t n, Reduce red)
+{
+    double acc = 0;
+    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS)
+        acc += (double)a[i] * (double)b[i];
+    double vv[1] = {acc};
+    reduce_finalize<1>(red, vv);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BS) k_pcg_dir(T *d, const T *__restrict__ z, const double *__restrict__ S, int first,
+                                                size_t n)
+{
+    const T beta = first ? T(0) : (T)(S[S_DELTA] / S[S_DELTA_OLD]);
+    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS)
+        d[i] = first ? z[i] : z[i] + beta * d[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BS) k_pcg_update(T *x, T *r, const T *__restrict__ d, T *z,
+                                                   const T *__restrict__ dinv, size_t n, Reduce red)
+{
+    const T a = (T)red.S[S_ALPHA];
+    double acc = 0;
+    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
+        const T di = d[i];
+        x[i] += a * di;
+        const T ri = r[i] - a * z[i];   // z holds q = A d on entry
+        r[i] = ri;
+        const T zi = dinv[i] * ri;
+        z[i] = zi;
+        acc += (double)ri * (double)zi;
+    }
+    double vv[1] = {acc};
+    reduce_finalize<1>(red, vv);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BS) k_sub_precond(T *r, const T *__restrict__ q, const T *__restrict__ dinv, T *z,
+                                                    size_t n, Reduce red)
+{
+    double acc = 0;
+    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
+        const T ri = r[i] - q[i];
+        r[i] = ri;
+        const T zi = dinv[i] * ri;
+        z[i] = zi;
+        acc += (double)ri * (double)zi;
+    }
+    double vv[1] = {acc};
+    reduce_finalize<1>(red, vv);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BS) k_minres_dir(T *d, T *q, const T *__restrict__ rt, const T *__restrict__ sv,
+                                                   const T *__restrict__ dinv, const double *__restrict__ S, int first,
+                                                   size_t n, Reduce red)
+{
+    const T b = first ? T(0) : (T)(S[S_DELTA] / S[S_DELTA_OLD]);
+    double acc = 0;
+    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
+        const T di = first ? rt[i] : rt[i] + b * d[i];
+        const T qi = first ? sv[i] : sv[i] + b * q[i];
+        d[i] = di;
+        q[i] = qi;
+        acc += (double)qi * (double)(dinv[i] * qi);
+    }
+    double vv[1] = {acc};
+    reduce_finalize<1>(red, vv);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BS) k_minres_update(T *x, T *rt, const T *__restrict__ d, const T *__restrict__ q,
+                                                      const T *__restrict__ dinv, size_t n, Reduce red)
+{
+    const T a = (T)red.S[S_ALPHA];
+    double acc = 0;
+    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
+        const T di = dinv[i];
+        const T rn = rt[i] / di - a * q[i];
+        const T rtn = di * rn;
+        rt[i] = rtn;
+        x[i] += a * d[i];
+        acc += (double)rn * (double)rtn;
+    }
+    double vv[1] = {acc};
+    reduce_finalize<1>(red, vv);
+}
+
+__global__ void __launch_bounds__(BS) k_add_correction(double *x, const float *__restrict__ e, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) x[i] += (double)e[i];
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi diagonal
+// ---------------------------------------------------------------------------
+template <typename T, int ND>
+__global__ void __launch_bounds__(BS) k_jacobi_keys(const uint16_t *__restrict__ keys, const T *__restrict__ K,
+                                                    T *dinv, uint32_t N, int ebe, int ny, int nx, int nz)
+{
+    constexpr int KE = 8 * ND;
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS) {
+        double dsum[ND];
+#pragma unroll
+        for (int a = 0; a < ND; a++) dsum[a] = 0;
+        unsigned key = 0;
+        if (ebe) {
+            // EBE keeps the element map: the material of the element with this
+            // node as local node e sits at the element origin.
+            int r, c, l;
+            node_rcl(i, ny, nx, r, c, l);
+            for (int e = 0; e < 8; e++) {
+                int re = r - cDR[e], ce = c - cDC[e], le = l - cDL[e];
+                re += re < 0 ? ny : 0; ce += ce < 0 ? nx : 0; le += le < 0 ? nz : 0;
+                key |= (unsigned)(keys[(size_t)re + (size_t)ce * ny + (size_t)le * nx * ny] & 3u) << (2 * e);
+            }
+        } else key = keys[i];
+        for (int e = 0; e < 8; e++) {
+            const int m = (key >> (2 * e)) & 3;
+#pragma unroll
+            for (int a = 0; a < ND; a++) dsum[a] += (double)K[m * KE * KE + (ND * e + a) * KE + ND * e + a];
+        }
+#pragma unroll
+        for (int a = 0; a < ND; a++) dinv[(size_t)ND * i + a] = (T)(1.0 / dsum[a]);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BS) k_jacobi_fluid(const uint8_t *__restrict__ bkeys, const T *__restrict__ K,
+                                                     T *dinv, uint32_t nvel, uint32_t nborder)
+{
+    double dv = 0, dp = 0;
+    for (int e = 0; e < 8; e++) {
+        dv += (double)K[(3 * e) * 32 + 3 * e];
+        dp -= (double)K[(24 + e) * 32 + 24 + e];
+    }
+    const uint32_t ntot = nvel + nborder;
+    for (uint32_t t = blockIdx.x * BS + threadIdx.x; t < ntot; t += gridDim.x * BS) {
+        if (t < nvel) {
+            dinv[3 * (size_t)t] = (T)(1.0 / dv);
+            dinv[3 * (size_t)t + 1] = (T)(1.0 / dv);
+            dinv[3 * (size_t)t + 2] = (T)(1.0 / dv);
+            dinv[3 * (size_t)nvel + t] = (T)(1.0 / dp);
+        } else {
+            const unsigned fk = bkeys[t - nvel];
+            double d = 0;
+            for (int e = 0; e < 8; e++)
+                if ((fk >> e) & 1u) d -= (double)K[(24 + e) * 32 + 24 + e];
+            dinv[3 * (size_t)nvel + t] = (T)(1.0 / d);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// right-hand side of a seam-jump load case (femhmg_thermal_3D.c:122-236,
+// femhmg_elastic_3D.c:130-462), gathered node by node; Stokes body force
+// (femhmg_fluid_3D.c:112-181).
+// ---------------------------------------------------------------------------
+template <int ND>
+__global__ void __launch_bounds__(BS) k_rhs_seam(double *b, const uint16_t *__restrict__ keys,
+                                                 const double *__restrict__ K, int ebe, int ny, int nx, int nz,
+                                                 uint32_t N, LoadCase lc)
+{
+    constexpr int KE = 8 * ND;
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS) {
+        int r, c, l;
+        node_rcl(i, ny, nx, r, c, l);
+        double acc[ND];
+#pragma unroll
+        for (int a = 0; a < ND; a++) acc[a] = 0;
+        const bool plane = lc.axis == 0 ? (c == nx - 1 || c == 0) : (lc.axis == 1 ? (r == 0 || r == 1 % ny) :
+                                                                     (l == 0 || l == 1 % nz));
+        if (plane) {
+            unsigned key = ebe ? 0u : keys[i];
+            for (int e = 0; e < 8; e++) {
+                int re = r - cDR[e], ce = c - cDC[e], le = l - cDL[e];
+                re += re < 0 ? ny : 0; ce += ce < 0 ? nx : 0; le += le < 0 ? nz : 0;
+                const bool seam = lc.axis == 0 ? ce == nx - 1 : (lc.axis == 1 ? re == 0 : le == 0);
+                if (!seam) continue;
+                const int m = ebe ? (int)keys[(size_t)re + (size_t)ce * ny + (size_t)le * nx * ny]
+                                  : (int)((key >> (2 * e)) & 3u);
+                const double *Km = K + (size_t)m * KE * KE;
+#pragma unroll
+                for (int a = 0; a < ND; a++) {
+                    double s = 0;
+                    for (int t = 0; t < 4; t++) s += Km[(ND * e + a) * KE + ND * lc.S[t] + lc.comp];
+                    acc[a] -= s * lc.L;
+                }
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < ND; a++) b[(size_t)ND * i + a] = acc[a];
+    }
+}
+
+__global__ void __launch_bounds__(BS) k_rhs_fluid(double *b, uint32_t nvel, uint32_t ntot, int dir, double f)
+{
+    for (uint32_t t = blockIdx.x * BS + threadIdx.x; t < ntot; t += gridDim.x * BS) {
+        if (t < nvel) {
+            b[3 * (size_t)t] = dir == 0 ? f : 0.0;
+            b[3 * (size_t)t + 1] = dir == 1 ? f : 0.0;
+            b[3 * (size_t)t + 2] = dir == 2 ? f : 0.0;
+        }
+        b[3 * (size_t)nvel + t] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// effective tensor column: sum_e CB_e u_e (+ seam term), FP64
+// (femhmg_thermal_3D.c:355-500, femhmg_elastic_3D.c:1075-1342)
+// ---------------------------------------------------------------------------
+template <int ND>
+__global__ void __launch_bounds__(BS) k_updateC(const double *__restrict__ x, const uint16_t *__restrict__ keys,
+                                                const double *__restrict__ CB, const double *__restrict__ seamCB,
+                                                int ebe, int ny, int nx, int nz, uint32_t N, LoadCase lc, Reduce red)
+{
+    constexpr int KE = 8 * ND, NR = ND == 3 ? 6 : 3;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS) {
+        int r, c, l;
+        node_rcl(i, ny, nx, r, c, l);
+        const Nbh h = make_nbh(r, c, l, ny, nx, nz);
+        const int m = ebe ? (int)keys[i] : (int)((keys[i] >> 6) & 3u);  // node i is local node 3 of element i
+        double ue[KE];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const size_t id = h.lo[1 + LDL(k)] + h.co[1 + LDC(k)] + h.ro[1 + LDR(k)];
+#pragma unroll
+            for (int a = 0; a < ND; a++) ue[ND * k + a] = x[ND * id + a];
+        }
+        const bool seam = lc.axis == 0 ? c == nx - 1 : (lc.axis == 1 ? r == 0 : l == 0);
+        for_each_uniform(m, [&](const int mu) {
+            cptr<double> CBm = as_const(CB) + (size_t)mu * NR * KE;
+#pragma unroll
+            for (int row = 0; row < NR; row++) {
+                double s = 0;
+#pragma unroll
+                for (int j = 0; j < KE; j++) s = fma(CBm[row * KE + j], ue[j], s);
+                if (seam) s += seamCB[mu * 6 + row];
+                acc[row] += s;
+            }
+        });
+    }
+    reduce_finalize<6>(red, acc);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BS) k_fluid_vsum(const T *__restrict__ x, uint32_t nvel, Reduce red)
+{
+    double acc[3] = {0, 0, 0};
+    for (uint32_t t = blockIdx.x * BS + threadIdx.x; t < nvel; t += gridDim.x * BS) {
+        acc[0] += (double)x[3 * (size_t)t];
+        acc[1] += (double)x[3 * (size_t)t + 1];
+        acc[2] += (double)x[3 * (size_t)t + 2];
+    }
+    reduce_finalize<3>(red, acc);
+}
+
+// ---------------------------------------------------------------------------
+// coarse -> fine initial guess (chfem/src/femhmg/cudapcg/kernels/image.cu:19-121)
+// ---------------------------------------------------------------------------
+__global__ void k_project2(double *res, const double *v, uint32_t nx, uint32_t ny, uint32_t nz, int stride)
+{
+    const uint32_t nxc = nx / 2, nyc = ny / 2, nzc = nz / 2;
+    const size_t nc = (size_t)nxc * nyc * (nzc + (nzc < 1));
+    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < nc; i += (size_t)gridDim.x * BS) {
+        size_t ii = i % nyc, jj = (i % ((size_t)nxc * nyc)) / nyc, kk = i / ((size_t)nxc * nyc);
+        size_t idx = 2 * ii + 2 * jj * ny + 2 * kk * (size_t)nx * ny;
+        for (int j = 0; j < stride; j++) res[stride * idx + j] = 2.0 * v[stride * i + j];
+    }
+}
+
+__global__ void k_interp(double *v, uint32_t nx, uint32_t ny, uint32_t nz, int stride, int pass)
+{
+    const size_t nxy = (size_t)nx * ny, N = nxy * nz;
+    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < N; i += (size_t)gridDim.x * BS) {
+        size_t ii = i % ny, jj = (i % nxy) / ny, kk = i / nxy;
+        size_t a, b;
+        if (pass == 0) {
+            if (!((ii & 1) && !(jj & 1) && !(kk & 1))) continue;
+            a = ii == 0 ? i - 1 + ny : i - 1;
+            b = (ii + 1) == ny ? i + 1 - ny : i + 1;
+        } else if (pass == 1) {
+            if (!((jj & 1) && !(kk & 1))) continue;
+            size_t loc = i % nxy;
+            a = (loc + nxy - ny) % nxy + kk * nxy;
+            b = (loc + ny) % nxy + kk * nxy;
+        } else {
+            if (!(kk & 1)) continue;
+            a = (i + N - nxy) % N;
+            b = (i + nxy) % N;
+        }
+        for (int j = 0; j < stride; j++) v[stride * i + j] = 0.5 * v[stride * a + j] + 0.5 * v[stride * b + j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int grid_for(size_t n) { return reduce_blocks(n); }
+
+static Reduce noreduce() { Reduce r; r.partials = nullptr; r.counter = nullptr; r.S = nullptr; r.fin = 0; r.slot = 0; return r; }
+
+template <typename T>
+void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s)
+{
+    const Reduce rr = red ? *red : noreduce();
+    const T *K = (const T *)op.K;
+    if (op.physics == PHY_FLUID) {
+        const uint32_t nt = op.nvel + op.nborder;
+        if (red) k_fluid<T, true><<<grid_for(nt), BS, 0, s>>>(v, q, op.dof2node, op.node2dof, op.bkeys, K, op.ny, op.nx, op.nz, op.nvel, op.nborder, rr);
+        else k_fluid<T, false><<<grid_for(nt), BS, 0, s>>>(v, q, op.dof2node, op.node2dof, op.bkeys, K, op.ny, op.nx, op.nz, op.nvel, op.nborder, rr);
+        return;
+    }
+    if (op.ebe) {
+        k_fill<T><<<grid_for(op.ndof), BS, 0, s>>>(q, T(0), op.ndof);
+        if (op.physics == PHY_THERMAL) k_ebe<T, 1><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N);
+        else k_ebe<T, 3><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N);
+        if (red) k_dot<T><<<grid_for(op.ndof), BS, 0, s>>>(v, q, op.ndof, rr);
+        return;
+    }
+    if (op.physics == PHY_THERMAL) {
+        if (red) k_thermal_nbn<T, true><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N, rr);
+        else k_thermal_nbn<T, false><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N, rr);
+    } else {
+        if (red) k_elastic_nbn<T, true><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N, rr);
+        else k_elastic_nbn<T, false><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N, rr);
+    }
+}
+
+template <typename T>
+void launch_jacobi(const OpDesc &op, T *dinv, hipStream_t s)
+{
+    const T *K = (const T *)op.K;
+    if (op.physics == PHY_FLUID)
+        k_jacobi_fluid<T><<<grid_for(op.nvel + op.nborder), BS, 0, s>>>(op.bkeys, K, dinv, op.nvel, op.nborder);
+    else if (op.physics == PHY_THERMAL)
+        k_jacobi_keys<T, 1><<<grid_for(op.N), BS, 0, s>>>(op.keys, K, dinv, op.N, op.ebe, op.ny, op.nx, op.nz);
+    else
+        k_jacobi_keys<T, 3><<<grid_for(op.N), BS, 0, s>>>(op.keys, K, dinv, op.N, op.ebe, op.ny, op.nx, op.nz);
+}
+
+template <typename T>
+void launch_fill(T *x, double val, size_t n, hipStream_t s) { k_fill<T><<<grid_for(n), BS, 0, s>>>(x, (T)val, n); }
+
+template <typename Tin, typename Tout>
+void launch_convert(const Tin *in, Tout *out, size_t n, hipStream_t s) { k_convert<Tin, Tout><<<grid_for(n), BS, 0, s>>>(in, out, n); }
+
+template <typename T>
+void launch_precond(const T *dinv, const T *v, T *out, size_t n, const Reduce *red, hipStream_t s)
+{
+    if (red) k_precond<T, true><<<grid_for(n), BS, 0, s>>>(dinv, v, out, n, *red);
+    else k_precond<T, false><<<grid_for(n), BS, 0, s>>>(dinv, v, out, n, noreduce());
+}
+
+template <typename T>
+void launch_pcg_dir(T *d, const T *z, const double *S, int first, size_t n, hipStream_t s)
+{
+    k_pcg_dir<T><<<grid_for(n), BS, 0, s>>>(d, z, S, first, n);
+}
+
+template <typename T>
+void launch_pcg_update(T *x, T *r, const T *d, T *z, const T *dinv, const Reduce *red, size_t n, hipStream_t s)
+{
+    k_pcg_update<T><<<grid_for(n), BS, 0, s>>>(x, r, d, z, dinv, n, *red);
+}
+
+template <typename T>
+void launch_sub_precond(T *r, const T *q, const T *dinv, T *z, const Reduce *red, size_t n, hipStream_t s)
+{
+    k_sub_precond<T><<<grid_for(n), BS, 0, s>>>(r, q, dinv, z, n, *red);
+}
+
+template <typename T>
+void launch_minres_dir(T *d, T *q, const T *rt, const T *sv, const T *dinv, const double *S, int first,
+                       const Reduce *red, size_t n, hipStream_t s)
+{
+    k_minres_dir<T><<<grid_for(n), BS, 0, s>>>(d, q, rt, sv, dinv, S, first, n, *red);
+}
+
+template <typename T>
+void launch_minres_update(T *x, T *rt, const T *d, const T *q, const T *dinv, const Reduce *red, size_t n,
+                          hipStream_t s)
+{
+    k_minres_update<T><<<grid_for(n), BS, 0, s>>>(x, rt, d, q, dinv, n, *red);
+}
+
+void launch_add_correction(double *x, const float *e, size_t n, hipStream_t s)
+{
+    k_add_correction<<<grid_for(n), BS, 0, s>>>(x, e, n);
+}
+
+template <typename T>
+void launch_dot(const T *a, const T *b, size_t n, const Reduce *red, hipStream_t s)
+{
+    k_dot<T><<<grid_for(n), BS, 0, s>>>(a, b, n, *red);
+}
+
+void launch_rhs(const OpDesc &op, const double *K64, const LoadCase &lc, int dir, double *b, hipStream_t s)
+{
+    if (op.physics == PHY_FLUID) {
+        k_rhs_fluid<<<grid_for(op.nvel + op.nborder), BS, 0, s>>>(b, op.nvel, op.nvel + op.nborder, dir, lc.L);
+        return;
+    }
+    if (op.physics == PHY_THERMAL)
+        k_rhs_seam<1><<<grid_for(op.N), BS, 0, s>>>(b, op.keys, K64, op.ebe, op.ny, op.nx, op.nz, op.N, lc);
+    else
+        k_rhs_seam<3><<<grid_for(op.N), BS, 0, s>>>(b, op.keys, K64, op.ebe, op.ny, op.nx, op.nz, op.N, lc);
+}
+
+void launch_updateC(const OpDesc &op, const double *CB64, const double *seamCB, const LoadCase &lc, const double *x,
+                    const Reduce *red, hipStream_t s)
+{
+    if (op.physics == PHY_THERMAL)
+        k_updateC<1><<<grid_for(op.N), BS, 0, s>>>(x, op.keys, CB64, seamCB, op.ebe, op.ny, op.nx, op.nz, op.N, lc, *red);
+    else
+        k_updateC<3><<<grid_for(op.N), BS, 0, s>>>(x, op.keys, CB64, seamCB, op.ebe, op.ny, op.nx, op.nz, op.N, lc, *red);
+}
+
+template <typename T>
+void launch_fluid_velocity_sum(const OpDesc &op, const T *x, const Reduce *red, hipStream_t s)
+{
+    k_fluid_vsum<T><<<grid_for(op.nvel ? op.nvel : 1), BS, 0, s>>>(x, op.nvel, *red);
+}
+
+void launch_interpl2(const double *coarse, double *fine, int rows, int cols, int layers, int stride, hipStream_t s)
+{
+    const bool is2d = layers == 0;
+    const int lay = layers < 1 ? 1 : layers;
+    const size_t n = (size_t)rows * cols * lay;
+    k_fill<double><<<grid_for(n * stride), BS, 0, s>>>(fine, 0.0, n * stride);
+    k_project2<<<grid_for(n), BS, 0, s>>>(fine, coarse, cols, rows, lay, stride);
+    k_interp<<<grid_for(n), BS, 0, s>>>(fine, cols, rows, lay, stride, 0);
+    k_interp<<<grid_for(n), BS, 0, s>>>(fine, cols, rows, lay, stride, 1);
+    if (!is2d) k_interp<<<grid_for(n), BS, 0, s>>>(fine, cols, rows, lay, stride, 2);
+}
+
+// explicit instantiations
+#define INST(T)                                                                                                \
+    template void launch_apply<T>(const OpDesc &, const T *, T *, const Reduce *, hipStream_t);               \
+    template void launch_jacobi<T>(const OpDesc &, T *, hipStream_t);                                         \
+    template void launch_fill<T>(T *, double, size_t, hipStream_t);                                            \
+    template void launch_precond<T>(const T *, const T *, T *, size_t, const Reduce *, hipStream_t);          \
+    template void launch_pcg_dir<T>(T *, const T *, const double *, int, size_t, hipStream_t);                \
+    template void launch_pcg_update<T>(T *, T *, const T *, T *, const T *, const Reduce *, size_t, hipStream_t); \
+    template void launch_sub_precond<T>(T *, const T *, const T *, T *, const Reduce *, size_t, hipStream_t); \
+    template void launch_minres_dir<T>(T *, T *, const T *, const T *, const T *, const double *, int,        \
+                                       const Reduce *, size_t, hipStream_t);                                  \
+    template void launch_minres_update<T>(T *, T *, const T *, const T *, const T *, const Reduce *, size_t,  \
+                                          hipStream_t);                                                       \
+    template void launch_dot<T>(const T *, const T *, size_t, const Reduce *, hipStream_t);                   \
+    template void launch_fluid_velocity_sum<T>(const OpDesc &, const T *, const Reduce *, hipStream_t);
+INST(float)
+INST(double)
+template void launch_convert<double, float>(const double *, float *, size_t, hipStream_t);
+template void launch_convert<float, double>(const float *, double *, size_t, hipStream_t);
+template void launch_convert<double, double>(const double *, double *, size_t, hipStream_t);
+
+}  // namespace chfem
