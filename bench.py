@@ -1,0 +1,206 @@
+"""Benchmark: seconds to the effective stiffness C_eff of a 256^3 two-phase
+elastic image within 1e-4 of the FP64 result, on one MI355X.
+
+Workload: overlapping spheres (radius 6-12 voxels, inclusion fraction 0.3,
+seed 3) in a matrix; matrix E=1, nu=0.3, inclusion E=10, nu=0.2; trilinear
+hexahedra, periodic boundaries, all 6 load cases.  A step is one full
+homogenization (6 x [RHS on the device -> Jacobi-preconditioned Krylov solve ->
+C column on the device]) with the model already resident in HBM.
+
+Methods timed in the same run:
+  plain : FP64 Jacobi-PCG (the reference method) at --tol
+  mixed : FP32 operator + inner PCG, FP64 dots/alpha/beta, FP64 outer residual
+          and iterative refinement, at --tol   (reported as `value`)
+Both are checked against C_ref, FP64 PCG at tolerance 1e-9 on the same
+image (untimed).  The operator kernel of the timed method is timed with HIP
+events on the solver's stream (roofline); the CPU FP64 oracle is timed on a
+bounded sample of the same workload (cpu_baseline).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "s to C_eff @1e-4 err, 256³ elastic: plain vs learned+mixed PCG; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_image(n, seed):
+    from chfem_amd import synthetic
+    return synthetic.spheres((n, n, n), rmin=6, rmax=12, fraction=0.3, seed=seed)
+
+
+def hip_sync():
+    try:
+        C.CDLL("libamdhip64.so").hipDeviceSynchronize()
+    except OSError:
+        pass
+
+
+def run_method(L, precision, tol, steps, warmup, label):
+    """Prepares the resident solver and times `steps` homogenizations."""
+    from chfem_amd.lib import SolveStats
+    L.hmgSetPrecisionFlag(precision)
+    L.hmgSetSolverTolerance(tol)
+    if L.hmgPrepare() != 1:
+        raise RuntimeError("hmgPrepare failed")
+    Cm = np.zeros(36)
+    try:
+        for w in range(warmup):
+            log(f"[bench] {label}: warmup {w + 1}/{warmup}")
+            L.hmgSolveResident(Cm.ctypes.data_as(C.c_void_p))
+        hip_sync()
+        times = []
+        for s in range(steps):
+            t0 = time.perf_counter()
+            ok = L.hmgSolveResident(Cm.ctypes.data_as(C.c_void_p))
+            hip_sync()
+            times.append(time.perf_counter() - t0)
+            log(f"[bench] {label}: step {s + 1}/{steps} {times[-1]:.3f} s")
+            if ok != 1:
+                raise RuntimeError("hmgSolveResident failed")
+        st = (SolveStats * 6)()
+        cnt = C.c_uint(0)
+        L.hmgGetSolveStats(st, C.byref(cnt))
+        its = [st[i].iterations for i in range(cnt.value)]
+        sweeps = [st[i].sweeps for i in range(cnt.value)]
+        op_ms = C.c_double(0)
+        op_n = C.c_uint(0)
+        L.hmgGetKernelTiming(C.byref(op_ms), C.byref(op_n))
+    finally:
+        L.hmgRelease()
+    return {"seconds": float(np.mean(times)) if times else None, "times": times, "C": Cm.reshape(6, 6).copy(),
+            "iterations": its, "sweeps": sweeps, "op_ms_total": op_ms.value, "op_launches": op_n.value}
+
+
+def cpu_baseline(img, n, gpu_iterations, iters_sample=3):
+    """CPU FP64 oracle (OpenMP, all host threads) on the same image: PCG
+    iterations of load case X timed, scaled by the FP64 GPU iteration count of
+    the full homogenization."""
+    from oracle import oracle as O
+    from chfem_amd import synthetic
+    em = synthetic.elem_map(img, [0, 255])
+    t0 = time.perf_counter()
+    m = O.Model(O.ELASTIC, (n, n, n), em, [1.0, 0.3, 10.0, 0.2])
+    setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    m.solve(0, tol=1e-30, maxit=1)
+    t1 = time.perf_counter()
+    m.solve(0, tol=1e-30, maxit=1 + iters_sample)
+    t2 = time.perf_counter()
+    per_iter = ((t2 - t1) - (t1 - t0)) / iters_sample
+    total_it = int(sum(gpu_iterations))
+    return {"value": per_iter * total_it, "unit": "s", "cores": int(O.lib().orc_num_threads()), "kind": "port",
+            "sample": (f"oracle FP64 Jacobi-PCG (NBN) on the same {n}^3 image, load case X, {iters_sample} "
+                       f"iterations timed ({per_iter:.3f} s/iteration, model setup {setup:.1f} s), "
+                       f"x {total_it} iterations = FP64 GPU iteration count of the 6 load cases")}
+
+
+def read_traffic(kernel_key):
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get(kernel_key)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--tol", type=float, default=1e-5)
+    ap.add_argument("--seed", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X only; --gpus {args.gpus} is not supported (run with --gpus 1).",
+              file=sys.stderr)
+        sys.exit(2)
+    n = args.size
+    from chfem_amd import io as cio
+    from chfem_amd.lib import lib, ptr
+    L = lib()
+    log(f"[bench] generating {n}^3 sphere image (seed {args.seed})")
+    img = build_image(n, args.seed)
+    tmpd = tempfile.mkdtemp(prefix="chfem_bench_")
+    nf = os.path.join(tmpd, f"spheres_{n}.raw")
+    cio.export_for_chfem(nf, img, 1, {0: (1.0, 0.3), 255: (10.0, 0.2)}, voxel_size=1.0, export_raw=False,
+                         solver_tolerance=args.tol, solver_maxiter=20000)
+    nf = nf[:-4] + ".nf"
+    flat = np.ascontiguousarray(img).reshape(-1)
+    if L.hmgInit(nf.encode(), None, None, ptr(flat)) != 1:
+        raise RuntimeError("hmgInit failed")
+    L.hmgSetSolverFlag(0)
+    L.hmgSetPreConditionerFlag(1)
+    L.hmgSetParallelStrategyFlag(0)
+    L.hmgSetHomogenizationFlag(6)
+    try:
+        log("[bench] reference C: FP64 PCG, tol 1e-9")
+        ref = run_method(L, 0, 1e-9, 1, 0, "reference")
+        Cref = ref["C"]
+        plain = run_method(L, 0, args.tol, args.steps, args.warmup, "plain-fp64")
+        fast = run_method(L, 1, args.tol, args.steps, args.warmup, "mixed")
+    finally:
+        L.hmgEnd()
+
+    def err(Cm):
+        return float(np.abs(Cm - Cref).max() / np.abs(Cref).max())
+
+    N = n ** 3
+    op_avg_ms = fast["op_ms_total"] / max(fast["op_launches"], 1)
+    bytes_per_launch = 25.0 * N  # FP32 elastic apply: 1 B image + 12 B read + 12 B written per node
+    achieved = bytes_per_launch / (op_avg_ms * 1e-3) / 1e9
+    traffic = read_traffic(f"elastic_walsh_fp32_{n}")
+    log(f"[bench] cpu baseline (oracle)")
+    cpu = cpu_baseline(img, n, plain["iterations"])
+    out = {
+        "metric": METRIC,
+        "value": fast["seconds"],
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": fast["seconds"] * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"elastic_{n}^3_spheres_r6-12_f0.3_seed{args.seed}_E1-10_nu0.3-0.2_6loads",
+                   "method": "mixed (FP32 inner PCG, FP64 reductions + iterative refinement), Jacobi, NBN",
+                   "solver_tolerance": args.tol, "reference": "FP64 PCG tol 1e-9"},
+        "err_vs_fp64": err(fast["C"]),
+        "plain_fp64": {"seconds": plain["seconds"], "err_vs_fp64": err(plain["C"]),
+                       "iterations": plain["iterations"],
+                       "op_avg_ms": plain["op_ms_total"] / max(plain["op_launches"], 1),
+                       "op_GBps": 49.0 * N / (plain["op_ms_total"] / max(plain["op_launches"], 1) * 1e-3) / 1e9},
+        "mixed": {"seconds": fast["seconds"], "iterations": fast["iterations"], "sweeps": fast["sweeps"]},
+        "speedup_vs_plain": plain["seconds"] / fast["seconds"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_elastic_walsh<float,32,8,true> (FP32 elastic NBN apply + fused d.Ad)",
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": op_avg_ms,
+                     "launches": fast["op_launches"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

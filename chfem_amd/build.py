@@ -21,7 +21,7 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
-SOURCES = ["kernels.hip", "cudapcg.cpp", "femhmg.cpp"]
+SOURCES = ["kernels.hip", "walsh.hip", "cudapcg.cpp", "femhmg.cpp"]
 
 
 def _run(cmd, verbose):

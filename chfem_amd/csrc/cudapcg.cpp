@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include "solver.h"
+#include "voxel.h"
 
 namespace chfem {
 
@@ -49,6 +50,15 @@ struct Solver {
     size_t partials_n = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // isotropic elastic materials for the Walsh-basis operator
+    double *lamu64 = nullptr;
+    float *lamu32 = nullptr;
+    bool walsh_ok = false;
+    // operator-launch timing
+    bool timing = false;
+    hipEvent_t opA = nullptr, opB = nullptr;
+    double op_ms = 0;
+    unsigned op_count = 0;
 };
 
 static Solver *g = nullptr;
@@ -98,6 +108,34 @@ static int set_scalar(int slot, double v)
     if (hipMemcpyAsync(g->S + slot, g->Sh + slot, sizeof(double), hipMemcpyHostToDevice, g->stream) != hipSuccess)
         return 0;
     return hipStreamSynchronize(g->stream) == hipSuccess;
+}
+
+// q = A v inside a Krylov loop; with timing on the launch is bracketed by
+// events whose interval is collected at the next host synchronisation.
+static bool op_pending = false;
+template <typename T>
+static void timed_apply(const OpDesc &op, const T *v, T *q, const Reduce *red)
+{
+    if (g->timing) hipEventRecord(g->opA, g->stream);
+    launch_apply<T>(op, v, q, red, g->stream);
+    if (g->timing) {
+        hipEventRecord(g->opB, g->stream);
+        op_pending = true;
+    }
+}
+
+static int sync_loop()
+{
+    if (!sync_scalars()) return 0;
+    if (op_pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, g->opA, g->opB) == hipSuccess) {
+            g->op_ms += ms;
+            g->op_count++;
+        }
+        op_pending = false;
+    }
+    return 1;
 }
 
 static int phys(cudapcgFlag_t a)
@@ -181,10 +219,19 @@ static int make_ops()
     o.dof2node = m->DOF2periodic_map;
     o.node2dof = m->periodic2DOF_map;
     o.bkeys = m->border_pore_map;
+    o.lamu = nullptr;
+    o.tile_tr = 32;
+    o.tile_tc = 8;
+    o.tile_lz = 16;
+    if (const char *t = getenv("CHFEM_WALSH_TILE")) sscanf(t, "%dx%dx%d", &o.tile_tr, &o.tile_tc, &o.tile_lz);
+    const char *opsel = getenv("CHFEM_OPERATOR");
+    const bool use_walsh = g->walsh_ok && !(opsel && !strcmp(opsel, "generic"));
     g->dm.op64 = o;
     g->dm.op64.K = g->dm.K64;
+    g->dm.op64.lamu = use_walsh ? g->lamu64 : nullptr;
     g->dm.op32 = o;
     g->dm.op32.K = g->dm.K32;
+    g->dm.op32.lamu = use_walsh ? g->lamu32 : nullptr;
     return 1;
 }
 
@@ -223,17 +270,17 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
     } else if (!set_scalar(S_DELTA, delta0)) return -1;
     Reduce ra = make_reduce(FIN_ALPHA, 0), rdl = make_reduce(FIN_DELTA, 0);
     it = 1;
-    launch_apply<T>(op, d, q, &ra, st);
+    timed_apply<T>(op, d, q, &ra);
     launch_pcg_update<T>(x, r, d, q, dinv, &rdl, n, st);
-    if (!sync_scalars()) return -1;
+    if (!sync_loop()) return -1;
     delta = g->Sh[S_DELTA];
     bool cont = above(delta, delta0, tol);
     while (cont && it < maxit) {
         it++;
         launch_pcg_dir<T>(d, q, g->S, 0, n, st);
-        launch_apply<T>(op, d, q, &ra, st);
+        timed_apply<T>(op, d, q, &ra);
         launch_pcg_update<T>(x, r, d, q, dinv, &rdl, n, st);
-        if (!sync_scalars()) return -1;
+        if (!sync_loop()) return -1;
         delta = g->Sh[S_DELTA];
         cont = above(delta, delta0, tol);
     }
@@ -281,16 +328,16 @@ static int run_minres(const OpDesc &op, T *x, T *r, T *d, T *q, T *s, const T *d
     launch_precond<T>(dinv, q, nullptr, n, &ra, st);
     it = 1;
     launch_minres_update<T>(x, r, d, q, dinv, &rs, n, st);
-    launch_apply<T>(op, r, s, &rdl, st);
-    if (!sync_scalars()) return -1;
+    timed_apply<T>(op, r, s, &rdl);
+    if (!sync_loop()) return -1;
     double stop = g->Sh[S_STOP];
     bool cont = above(stop, res0, tol);
     while (cont && it < maxit) {
         it++;
         launch_minres_dir<T>(d, q, r, s, dinv, g->S, 0, &ra, n, st);
         launch_minres_update<T>(x, r, d, q, dinv, &rs, n, st);
-        launch_apply<T>(op, r, s, &rdl, st);
-        if (!sync_scalars()) return -1;
+        timed_apply<T>(op, r, s, &rdl);
+        if (!sync_loop()) return -1;
         stop = g->Sh[S_STOP];
         cont = above(stop, res0, tol);
     }
@@ -454,6 +501,23 @@ cudapcgFlag_t cudapcgSetPrecision(cudapcgFlag_t flag)
     return CUDAPCG_TRUE;
 }
 
+cudapcgFlag_t cudapcgSetKernelTiming(cudapcgFlag_t on)
+{
+    if (!g) return CUDAPCG_FALSE;
+    g->timing = on != 0;
+    g->op_ms = 0;
+    g->op_count = 0;
+    return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgGetKernelTiming(double *op_ms_total, unsigned int *op_launches)
+{
+    if (!g) return CUDAPCG_FALSE;
+    if (op_ms_total) *op_ms_total = g->op_ms;
+    if (op_launches) *op_launches = g->op_count;
+    return CUDAPCG_TRUE;
+}
+
 cudapcgFlag_t cudapcgGetSolveStats(chfemSolveStats_t *s)
 {
     if (!g || !s) return CUDAPCG_FALSE;
@@ -514,12 +578,16 @@ cudapcgFlag_t cudapcgEnd(void)
     dfree(g->r64);
     dfree(g->dm.K64);
     dfree(g->dm.K32);
+    dfree(g->lamu64);
+    dfree(g->lamu32);
     if (g->model && g->model->freeAllowed_flag) {
         free_model_device(g->model);
         free(g->model);
     }
     if (g->ev0) hipEventDestroy(g->ev0);
     if (g->ev1) hipEventDestroy(g->ev1);
+    if (g->opA) hipEventDestroy(g->opA);
+    if (g->opB) hipEventDestroy(g->opB);
     if (g->stream) hipStreamDestroy(g->stream);
     delete g;
     g = nullptr;
@@ -550,7 +618,8 @@ cudapcgFlag_t cudapcgInit(cudapcgFlag_t analysis_flag, cudapcgFlag_t parStrategy
     g->header = (char *)malloc(1);
     g->header[0] = 0;
     if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess) {
+        hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess ||
+        hipEventCreate(&g->opA) != hipSuccess || hipEventCreate(&g->opB) != hipSuccess) {
         fprintf(stderr, "chfem: failed to create the solver stream\n");
         delete g;
         g = nullptr;
@@ -767,6 +836,22 @@ cudapcgFlag_t cudapcgSetLclMtxs(cudapcgVar_t *L)
     g->dm.ksz = ksz;
     g->dm.nmat = nk;
     g->dinv_valid = false;
+    // isotropic elastic materials: the operator can use the Walsh-basis kernel
+    g->walsh_ok = false;
+    if (ph == PHY_ELASTIC && nk <= 4) {
+        double lm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        float lmf[8];
+        bool ok = true;
+        for (unsigned m = 0; m < nk && ok; m++) ok = voxel::isotropic_params(L + (size_t)m * 576, lm[2 * m], lm[2 * m + 1]);
+        for (int i = 0; i < 8; i++) lmf[i] = (float)lm[i];
+        if (ok) {
+            if (!g->lamu64) CHECK_HIP(hipMalloc(&g->lamu64, 8 * sizeof(double)));
+            if (!g->lamu32) CHECK_HIP(hipMalloc(&g->lamu32, 8 * sizeof(float)));
+            CHECK_HIP(hipMemcpy(g->lamu64, lm, sizeof(lm), hipMemcpyHostToDevice));
+            CHECK_HIP(hipMemcpy(g->lamu32, lmf, sizeof(lmf), hipMemcpyHostToDevice));
+            g->walsh_ok = true;
+        }
+    }
     return CUDAPCG_TRUE;
 }
 
@@ -821,6 +906,32 @@ cudapcgFlag_t cudapcgApplyPreconditioner(const cudapcgVar_t *v, cudapcgVar_t *z)
     CHECK_HIP(hipMemcpyAsync(z, g->q64, n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     CHECK_HIP(hipStreamSynchronize(g->stream));
     return CUDAPCG_TRUE;
+}
+
+double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps)
+{
+    if (!g || !g->model) return -1.0;
+    if (!g->arrays && alloc_solver_arrays() != CUDAPCG_TRUE) return -1.0;
+    if (precision == CHFEM_PREC_MIXED && !g->d32) {
+        g->precision = CHFEM_PREC_MIXED;
+        if (alloc_solver_arrays() != CUDAPCG_TRUE) return -1.0;
+    }
+    make_ops();
+    const size_t n = nvars();
+    launch_fill<double>(g->d64, 1.0, n, g->stream);
+    if (precision == CHFEM_PREC_MIXED) launch_convert<double, float>(g->d64, g->d32, n, g->stream);
+    auto once = [&]() {
+        if (precision == CHFEM_PREC_MIXED) launch_apply<float>(g->dm.op32, g->d32, g->q32, nullptr, g->stream);
+        else launch_apply<double>(g->dm.op64, g->d64, g->q64, nullptr, g->stream);
+    };
+    once();
+    hipEventRecord(g->ev0, g->stream);
+    for (unsigned r = 0; r < reps; r++) once();
+    hipEventRecord(g->ev1, g->stream);
+    if (hipEventSynchronize(g->ev1) != hipSuccess) return -1.0;
+    float ms = 0;
+    hipEventElapsedTime(&ms, g->ev0, g->ev1);
+    return reps ? ms / reps : 0.0;
 }
 
 cudapcgFlag_t cudapcgSetHeaderString(char *header)

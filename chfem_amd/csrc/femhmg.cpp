@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "solver.h"
+#include "voxel.h"
 #include "../../include/femhmg.h"
 #include "../../include/chfem_amd.h"
 
@@ -76,49 +77,12 @@ double g_xreduce_stab = 0.0001;
 double g_time_total = 0;
 
 // ---------------------------------------------------------------------------
-// local matrices of the unit voxel from exact 1D integrals of the linear
-// shape functions: mass 1/3, 1/6; stiffness +-1; mixed +-1/2.
+// local matrices of the unit voxel (voxel.h)
 // ---------------------------------------------------------------------------
-inline void frame(int k, bool stokes, int &x, int &y, int &z)
-{
-    x = DC[k];
-    y = 1 - DR[k];
-    z = stokes ? DL[k] : 1 - DL[k];
-}
-
-// int over the unit cube of d_p N_i * d_q N_j  (p or q = -1: no derivative)
-double int_dd(int i, int j, int p, int q, bool stokes)
-{
-    int ci[3], cj[3];
-    frame(i, stokes, ci[0], ci[1], ci[2]);
-    frame(j, stokes, cj[0], cj[1], cj[2]);
-    double v = 1.0;
-    for (int t = 0; t < 3; t++) {
-        const double si = ci[t] ? 1.0 : -1.0, sj = cj[t] ? 1.0 : -1.0;
-        if (t == p && t == q) v *= si * sj;
-        else if (t == p) v *= 0.5 * si;
-        else if (t == q) v *= 0.5 * sj;
-        else v *= ci[t] == cj[t] ? 1.0 / 3.0 : 1.0 / 6.0;
-    }
-    return v;
-}
-
-double lap(int i, int j, bool stokes)
-{
-    return int_dd(i, j, 0, 0, stokes) + int_dd(i, j, 1, 1, stokes) + int_dd(i, j, 2, 2, stokes);
-}
-
-// 24x24 isotropic stiffness, DOFs node-major (x,y,z)
-void elastic_K(double lam, double mu, bool stokes, double *K)
-{
-    for (int i = 0; i < 8; i++)
-        for (int a = 0; a < 3; a++)
-            for (int j = 0; j < 8; j++)
-                for (int b = 0; b < 3; b++)
-                    K[(3 * i + a) * 24 + 3 * j + b] = lam * int_dd(i, j, a, b, stokes) +
-                                                      mu * ((a == b ? lap(i, j, stokes) : 0.0) +
-                                                            int_dd(i, j, b, a, stokes));
-}
+using voxel::elastic_K;
+using voxel::frame;
+using voxel::int_dd;
+using voxel::lap;
 
 // 6x24 centre stress operator, rows xx,yy,zz,yz,xz,xy (engineering shear)
 void elastic_CB(double lam, double mu, double *CB)
@@ -651,6 +615,7 @@ logical hmgInit(char *data_filename, char *elem_filename, char *sdf_filename, ui
 logical hmgEnd(void)
 {
     if (!M) return HMG_FALSE;
+    hmgRelease();
     delete M;
     M = nullptr;
     printf("Elapsed time (total): %.2e s\n", omp_get_wtime() - g_time_total);
@@ -761,28 +726,53 @@ void hmgFindInitialGuesses(unsigned int nlevels)
     for (size_t i = 1; i < lv.size(); i++) delete lv[i];
 }
 
-void hmgSolveHomogenization(void)
+void hmgSetSolverTolerance(double tol) { if (M) M->tol = tol; }
+void hmgSetMaxIterationsOverride(unsigned int maxit) { if (M) M->maxit = maxit; }
+
+static DeviceAux g_aux;
+static bool g_prepared = false;
+static double g_op_ms = 0;
+static unsigned g_op_count = 0;
+
+logical hmgPrepare(void)
 {
-    if (!M) return;
-    Model &m = *M;
-    if (m.analysis != HMG_FLUID && g_solver > CUDAPCG_MINRES3_SOLVER) {
+    if (!M) return HMG_FALSE;
+    if (g_prepared) hmgRelease();
+    if (M->analysis != HMG_FLUID && g_solver > CUDAPCG_MINRES3_SOLVER) {
         printf("WARNING: CG2 and MINRES2 are only available for FLUID.\n");
         g_solver = g_solver == CUDAPCG_CG2_SOLVER ? CUDAPCG_CG3_SOLVER : CUDAPCG_MINRES3_SOLVER;
     }
+    if (!solver_setup(*M, g_aux)) return HMG_FALSE;
+    g_prepared = true;
+    return HMG_TRUE;
+}
+
+void hmgRelease(void)
+{
+    if (!g_prepared) return;
+    free_aux(g_aux);
+    cudapcgFreeArrays();
+    cudapcgSetX0(nullptr, CUDAPCG_FALSE);
+    cudapcgEnd();
+    g_prepared = false;
+}
+
+logical hmgSolveResident(var *Cout)
+{
+    if (!M || !g_prepared) return HMG_FALSE;
+    Model &m = *M;
     for (unsigned i = 0; i < m.C_dim; i++) m.C[i] = 0.0;
     m.stats.clear();
-    const double t_hmg = omp_get_wtime();
-    appendf("Running homogenization...\n");
-    DeviceAux aux;
-    if (!solver_setup(m, aux)) return;
+    cudapcgSetKernelTiming(CUDAPCG_TRUE);
     static const char sim_str[6][11] = {"X", "Y", "Z", "YZ (shear)", "XZ (shear)", "XY (shear)"};
     std::vector<unsigned> dirs;
     directions(m, dirs);
     std::vector<double> xfile;
+    bool ok = true;
     for (unsigned d : dirs) {
         appendf("    ------------------------------------------------------\n    Analysis on %s:\n", sim_str[d]);
         m.hmg_flag = d;
-        device_rhs(m, aux, d);
+        device_rhs(m, g_aux, d);
         if (m.using_x0) {
             if (m.x0_file.empty()) {
                 if (!m.x0.empty() && !m.x0[d].empty()) cudapcgSetX0(m.x0[d].data(), CUDAPCG_TRUE);
@@ -809,7 +799,10 @@ void hmgSolveHomogenization(void)
         m.stats.push_back(st);
         appendf("    | Solver ran %u iterations (%u sweeps), residual %.3e, %.3e s\n", st.iterations, st.sweeps,
                 st.residual, st.solve_seconds);
-        if (rc < 0) break;
+        if (rc < 0) {
+            ok = false;
+            break;
+        }
         if (m.export_x) {
             std::vector<double> xh(m.ndof);
             hipMemcpy(xh.data(), solution_buffer(), m.ndof * sizeof(double), hipMemcpyDeviceToHost);
@@ -821,21 +814,37 @@ void hmgSolveHomogenization(void)
                 fclose(f);
             }
         }
+        if (!device_updateC(m, g_aux, d, solution_buffer())) {
+            ok = false;
+            break;
+        }
         if (g_solver >= CUDAPCG_CG2_SOLVER && m.analysis == HMG_FLUID) {
-            // two-vector solvers report the effective field directly
-            device_updateC(m, aux, d, solution_buffer());
             printf("    Effective fields:\n");
             for (unsigned jj = 0; jj < 3; jj++) printf("        %.8e\n", m.C[jj * 3 + d]);
-        } else {
-            device_updateC(m, aux, d, solution_buffer());
         }
     }
+    cudapcgGetKernelTiming(&g_op_ms, &g_op_count);
     if (m.save_fields) printf("WARNING: field export is not provided by this library.\n");
     appendf("    ------------------------------------------------------\n");
-    free_aux(aux);
-    cudapcgFreeArrays();
-    cudapcgSetX0(nullptr, CUDAPCG_FALSE);
-    cudapcgEnd();
+    if (Cout) memcpy(Cout, m.C, m.C_dim * sizeof(double));
+    return ok ? HMG_TRUE : HMG_FALSE;
+}
+
+cudapcgFlag_t hmgGetKernelTiming(double *op_ms_total, unsigned int *op_launches)
+{
+    if (op_ms_total) *op_ms_total = g_op_ms;
+    if (op_launches) *op_launches = g_op_count;
+    return CUDAPCG_TRUE;
+}
+
+void hmgSolveHomogenization(void)
+{
+    if (!M) return;
+    const double t_hmg = omp_get_wtime();
+    appendf("Running homogenization...\n");
+    if (!hmgPrepare()) return;
+    hmgSolveResident(nullptr);
+    hmgRelease();
     printf("Done.(%.2e s)\n", omp_get_wtime() - t_hmg);
 }
 

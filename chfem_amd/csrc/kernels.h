@@ -6,8 +6,11 @@
 
 namespace chfem {
 
-// upper bound on the blocks of any reducing launch (partials buffer size)
-#define MAX_PARTIAL_BLOCKS 8192
+// upper bound on the blocks of any reducing launch: 1-D vector kernels use at
+// most MAX_RED_GRID blocks; tiled operators at most MAX_PARTIAL_BLOCKS (the
+// partials buffer holds 8 values per block)
+#define MAX_RED_GRID 8192
+#define MAX_PARTIAL_BLOCKS 65536
 
 enum Physics { PHY_THERMAL = 0, PHY_ELASTIC = 1, PHY_FLUID = 2 };
 
@@ -46,6 +49,10 @@ struct OpDesc {
     const uint32_t *dof2node;  // DOF id -> periodic node
     const uint32_t *node2dof;  // periodic node -> DOF id
     const uint8_t *bkeys;      // border node fluid keys
+    // isotropic elastic materials: {lambda, mu} per material (precision of K);
+    // non-null selects the Walsh-basis operator (walsh.hip)
+    const void *lamu;
+    int tile_tr, tile_tc, tile_lz;
 };
 
 template <typename T> void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s);
@@ -91,4 +98,14 @@ void launch_interpl2(const double *coarse, double *fine, int rows, int cols, int
 
 int reduce_blocks(size_t n);
 
+}  // namespace chfem
+
+namespace chfem {
+// Isotropic elastic materials in the form the Walsh-basis operator uses:
+// per material {lambda, mu}.  Filled when every local matrix equals
+// lambda*A + mu*B for the unit-voxel trilinear element.
+struct WalshTiles { int TR, TC, LZ; };
+template <typename T>
+void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduce *red, WalshTiles t,
+                          hipStream_t s);
 }  // namespace chfem

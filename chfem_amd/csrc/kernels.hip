@@ -18,11 +18,12 @@
 // block to finish sums the block partials in a fixed order.
 #include <hip/hip_runtime.h>
 #include "kernels.h"
+#include "reduce.h"
 
 namespace chfem {
 
 #define BS 256
-#define MAX_RED_BLOCKS MAX_PARTIAL_BLOCKS
+#define MAX_RED_BLOCKS MAX_RED_GRID
 
 __constant__ int cDR[8] = {1, 1, 0, 0, 1, 1, 0, 0};
 __constant__ int cDC[8] = {0, 1, 1, 0, 0, 1, 1, 0};
@@ -38,85 +39,6 @@ int reduce_blocks(size_t n)
     if (b > MAX_RED_BLOCKS) b = MAX_RED_BLOCKS;
     if (b < 1) b = 1;
     return (int)b;
-}
-
-// ---------------------------------------------------------------------------
-// deterministic reductions
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// block sum; result valid in thread 0
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV])
-{
-    __shared__ double lds[NV][BS / 64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-        double t = wave_sum(v[k]);
-        if (lane == 0) lds[k][w] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) {
-            double t = 0;
-            for (int i = 0; i < BS / 64; i++) t += lds[k][i];
-            v[k] = t;
-        }
-    }
-    __syncthreads();
-}
-
-__device__ __forceinline__ double load_coherent(const double *p)
-{
-    unsigned long long u = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)u);
-}
-
-template <int NV>
-__device__ void reduce_finalize(const Reduce &red, double (&v)[NV])
-{
-    __shared__ int amLast;
-    block_sum<NV>(v);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) red.partials[(size_t)k * gridDim.x + blockIdx.x] = v[k];
-        __threadfence();
-        unsigned prev = atomicAdd(red.counter, 1u);
-        amLast = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (amLast) {
-        __threadfence();
-        double s[NV];
-#pragma unroll
-        for (int k = 0; k < NV; k++) {
-            s[k] = 0;
-            for (unsigned b = threadIdx.x; b < gridDim.x; b += BS) s[k] += load_coherent(red.partials + (size_t)k * gridDim.x + b);
-        }
-        block_sum<NV>(s);
-        if (threadIdx.x == 0) {
-            double *S = red.S;
-            if (red.fin == FIN_ALPHA) {
-                S[S_DQ] = s[0];
-                S[S_ALPHA] = S[S_DELTA] / s[0];
-            } else if (red.fin == FIN_DELTA) {
-                S[S_DELTA_OLD] = S[S_DELTA];
-                S[S_DELTA] = s[0];
-            } else {
-#pragma unroll
-                for (int k = 0; k < NV; k++) S[red.slot + k] = s[k];
-            }
-            __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -399,16 +321,66 @@ __global__ void __launch_bounds__(BS) k_convert(const Tin *in, Tout *out, size_t
     for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) out[i] = (Tout)in[i];
 }
 
+// Streaming vector kernels.  Arrays are processed as 16-byte vectors
+// (2 doubles / 4 floats); block b owns one contiguous range of vectors and its
+// work-items walk it with a stride of one block, so every wavefront reads and
+// writes whole, consecutive 1-KiB segments.  The ragged tail (n not a
+// multiple of the vector width) is handled by the last block entry by entry.
+template <typename T>
+struct V16 {
+    static constexpr int W = 16 / sizeof(T);
+    T e[W];
+};
+template <typename T>
+__device__ __forceinline__ V16<T> ld16(const T *p)
+{
+    V16<T> r;
+    const uint4 q = *reinterpret_cast<const uint4 *>(p);
+    __builtin_memcpy(&r, &q, 16);
+    return r;
+}
+template <typename T>
+__device__ __forceinline__ void st16(T *p, const V16<T> &r)
+{
+    uint4 q;
+    __builtin_memcpy(&q, &r, 16);
+    *reinterpret_cast<uint4 *>(p) = q;
+}
+
+// range of 16-byte vectors owned by this block; tail entries [nv*W, n)
+#define STREAM_RANGE(T, n)                                                                        \
+    constexpr int W_ = V16<T>::W;                                                                 \
+    const size_t nv_ = (n) / W_;                                                                  \
+    const size_t per_ = (nv_ + gridDim.x - 1) / gridDim.x;                                        \
+    const size_t vb_ = (size_t)blockIdx.x * per_;                                                 \
+    const size_t ve_ = vb_ + per_ < nv_ ? vb_ + per_ : nv_;
+#define FOR_VECTORS for (size_t j_ = vb_ + threadIdx.x; j_ < ve_; j_ += BS)
+#define FOR_TAIL(n)                                                                               \
+    if (blockIdx.x == gridDim.x - 1)                                                              \
+        for (size_t i = nv_ * W_ + threadIdx.x; i < (n); i += BS)
+
 template <typename T, bool RED>
 __global__ void __launch_bounds__(BS) k_precond(const T *__restrict__ dinv, const T *v, T *out, size_t n,
                                                 Reduce red)
 {
     double acc = 0;
-    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
-        T vi = v[i];
-        T o = dinv[i] * vi;
+    STREAM_RANGE(T, n)
+    FOR_VECTORS
+    {
+        const V16<T> a = ld16(v + j_ * W_), m = ld16(dinv + j_ * W_);
+        V16<T> o;
+#pragma unroll
+        for (int k = 0; k < W_; k++) {
+            o.e[k] = m.e[k] * a.e[k];
+            if (RED) acc += (double)a.e[k] * (double)o.e[k];
+        }
+        if (out) st16(out + j_ * W_, o);
+    }
+    FOR_TAIL(n)
+    {
+        const T o = dinv[i] * v[i];
+        if (RED) acc += (double)v[i] * (double)o;
         if (out) out[i] = o;
-        if (RED) acc += (double)vi * (double)o;
     }
     if (RED) {
         double vv[1] = {acc};
@@ -420,84 +392,175 @@ template <typename T>
 __global__ void __launch_bounds__(BS) k_dot(const T *__restrict__ a, const T *__restrict__ b, size_t n, Reduce red)
 {
     double acc = 0;
-    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS)
-        acc += (double)a[i] * (double)b[i];
+    STREAM_RANGE(T, n)
+    FOR_VECTORS
+    {
+        const V16<T> x = ld16(a + j_ * W_), y = ld16(b + j_ * W_);
+#pragma unroll
+        for (int k = 0; k < W_; k++) acc += (double)x.e[k] * (double)y.e[k];
+    }
+    FOR_TAIL(n) acc += (double)a[i] * (double)b[i];
     double vv[1] = {acc};
     reduce_finalize<1>(red, vv);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(BS) k_pcg_dir(T *d, const T *__restrict__ z, const double *__restrict__ S, int first,
-                                                size_t n)
+__global__ void __launch_bounds__(BS) k_pcg_dir(T *__restrict__ d, const T *__restrict__ z,
+                                                const double *__restrict__ S, int first, size_t n)
 {
     const T beta = first ? T(0) : (T)(S[S_DELTA] / S[S_DELTA_OLD]);
-    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS)
-        d[i] = first ? z[i] : z[i] + beta * d[i];
+    STREAM_RANGE(T, n)
+    FOR_VECTORS
+    {
+        const V16<T> zz = ld16(z + j_ * W_);
+        V16<T> dd;
+        if (first) dd = zz;
+        else {
+            dd = ld16(d + j_ * W_);
+#pragma unroll
+            for (int k = 0; k < W_; k++) dd.e[k] = zz.e[k] + beta * dd.e[k];
+        }
+        st16(d + j_ * W_, dd);
+    }
+    FOR_TAIL(n) d[i] = first ? z[i] : z[i] + beta * d[i];
 }
 
+// x += a d ; r -= a q ; z = dinv r (z holds q = A d on entry) ; reduce r.z
 template <typename T>
-__global__ void __launch_bounds__(BS) k_pcg_update(T *x, T *r, const T *__restrict__ d, T *z,
-                                                   const T *__restrict__ dinv, size_t n, Reduce red)
+__global__ void __launch_bounds__(BS) k_pcg_update(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ d,
+                                                   T *__restrict__ z, const T *__restrict__ dinv, size_t n,
+                                                   Reduce red)
 {
     const T a = (T)red.S[S_ALPHA];
     double acc = 0;
-    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
-        const T di = d[i];
-        x[i] += a * di;
-        const T ri = r[i] - a * z[i];   // z holds q = A d on entry
-        r[i] = ri;
-        const T zi = dinv[i] * ri;
-        z[i] = zi;
-        acc += (double)ri * (double)zi;
+    STREAM_RANGE(T, n)
+    FOR_VECTORS
+    {
+        const size_t o = j_ * W_;
+        V16<T> xv = ld16(x + o), rv = ld16(r + o);
+        const V16<T> dv = ld16(d + o), qv = ld16(z + o), mv = ld16(dinv + o);
+        V16<T> zv;
+#pragma unroll
+        for (int k = 0; k < W_; k++) {
+            xv.e[k] += a * dv.e[k];
+            rv.e[k] -= a * qv.e[k];
+            zv.e[k] = mv.e[k] * rv.e[k];
+            acc += (double)rv.e[k] * (double)zv.e[k];
+        }
+        st16(x + o, xv);
+        st16(r + o, rv);
+        st16(z + o, zv);
+    }
+    FOR_TAIL(n)
+    {
+        x[i] += a * d[i];
+        const T rn = r[i] - a * z[i];
+        r[i] = rn;
+        const T zn = dinv[i] * rn;
+        z[i] = zn;
+        acc += (double)rn * (double)zn;
     }
     double vv[1] = {acc};
     reduce_finalize<1>(red, vv);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(BS) k_sub_precond(T *r, const T *__restrict__ q, const T *__restrict__ dinv, T *z,
-                                                    size_t n, Reduce red)
+__global__ void __launch_bounds__(BS) k_sub_precond(T *__restrict__ r, const T *__restrict__ q,
+                                                    const T *__restrict__ dinv, T *__restrict__ z, size_t n,
+                                                    Reduce red)
 {
     double acc = 0;
-    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
-        const T ri = r[i] - q[i];
-        r[i] = ri;
-        const T zi = dinv[i] * ri;
-        z[i] = zi;
-        acc += (double)ri * (double)zi;
+    STREAM_RANGE(T, n)
+    FOR_VECTORS
+    {
+        const size_t o = j_ * W_;
+        V16<T> rv = ld16(r + o);
+        const V16<T> qv = ld16(q + o), mv = ld16(dinv + o);
+        V16<T> zv;
+#pragma unroll
+        for (int k = 0; k < W_; k++) {
+            rv.e[k] -= qv.e[k];
+            zv.e[k] = mv.e[k] * rv.e[k];
+            acc += (double)rv.e[k] * (double)zv.e[k];
+        }
+        st16(r + o, rv);
+        st16(z + o, zv);
+    }
+    FOR_TAIL(n)
+    {
+        const T rn = r[i] - q[i];
+        r[i] = rn;
+        const T zn = dinv[i] * rn;
+        z[i] = zn;
+        acc += (double)rn * (double)zn;
     }
     double vv[1] = {acc};
     reduce_finalize<1>(red, vv);
 }
 
+// MINRES: d = rt + b d ; q = s + b q ; reduce q.(dinv q)
 template <typename T>
-__global__ void __launch_bounds__(BS) k_minres_dir(T *d, T *q, const T *__restrict__ rt, const T *__restrict__ sv,
-                                                   const T *__restrict__ dinv, const double *__restrict__ S, int first,
-                                                   size_t n, Reduce red)
+__global__ void __launch_bounds__(BS) k_minres_dir(T *__restrict__ d, T *__restrict__ q, const T *__restrict__ rt,
+                                                   const T *__restrict__ sv, const T *__restrict__ dinv,
+                                                   const double *__restrict__ S, int first, size_t n, Reduce red)
 {
     const T b = first ? T(0) : (T)(S[S_DELTA] / S[S_DELTA_OLD]);
     double acc = 0;
-    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
-        const T di = first ? rt[i] : rt[i] + b * d[i];
-        const T qi = first ? sv[i] : sv[i] + b * q[i];
-        d[i] = di;
-        q[i] = qi;
-        acc += (double)qi * (double)(dinv[i] * qi);
+    STREAM_RANGE(T, n)
+    FOR_VECTORS
+    {
+        const size_t o = j_ * W_;
+        const V16<T> rv = ld16(rt + o), s_ = ld16(sv + o), mv = ld16(dinv + o);
+        V16<T> dv = ld16(d + o), qv = ld16(q + o);
+#pragma unroll
+        for (int k = 0; k < W_; k++) {
+            dv.e[k] = first ? rv.e[k] : rv.e[k] + b * dv.e[k];
+            qv.e[k] = first ? s_.e[k] : s_.e[k] + b * qv.e[k];
+            acc += (double)qv.e[k] * (double)(mv.e[k] * qv.e[k]);
+        }
+        st16(d + o, dv);
+        st16(q + o, qv);
+    }
+    FOR_TAIL(n)
+    {
+        const T dn = first ? rt[i] : rt[i] + b * d[i];
+        const T qn = first ? sv[i] : sv[i] + b * q[i];
+        d[i] = dn;
+        q[i] = qn;
+        acc += (double)qn * (double)(dinv[i] * qn);
     }
     double vv[1] = {acc};
     reduce_finalize<1>(red, vv);
 }
 
+// MINRES: rn = rt/dinv - a q ; rt = dinv rn ; x += a d ; reduce rn.rt
 template <typename T>
-__global__ void __launch_bounds__(BS) k_minres_update(T *x, T *rt, const T *__restrict__ d, const T *__restrict__ q,
-                                                      const T *__restrict__ dinv, size_t n, Reduce red)
+__global__ void __launch_bounds__(BS) k_minres_update(T *__restrict__ x, T *__restrict__ rt, const T *__restrict__ d,
+                                                      const T *__restrict__ q, const T *__restrict__ dinv, size_t n,
+                                                      Reduce red)
 {
     const T a = (T)red.S[S_ALPHA];
     double acc = 0;
-    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
-        const T di = dinv[i];
-        const T rn = rt[i] / di - a * q[i];
-        const T rtn = di * rn;
+    STREAM_RANGE(T, n)
+    FOR_VECTORS
+    {
+        const size_t o = j_ * W_;
+        V16<T> xv = ld16(x + o), rv = ld16(rt + o);
+        const V16<T> dv = ld16(d + o), qv = ld16(q + o), mv = ld16(dinv + o);
+#pragma unroll
+        for (int k = 0; k < W_; k++) {
+            const T rn = rv.e[k] / mv.e[k] - a * qv.e[k];
+            rv.e[k] = mv.e[k] * rn;
+            xv.e[k] += a * dv.e[k];
+            acc += (double)rn * (double)rv.e[k];
+        }
+        st16(x + o, xv);
+        st16(rt + o, rv);
+    }
+    FOR_TAIL(n)
+    {
+        const T rn = rt[i] / dinv[i] - a * q[i];
+        const T rtn = dinv[i] * rn;
         rt[i] = rtn;
         x[i] += a * d[i];
         acc += (double)rn * (double)rtn;
@@ -732,6 +795,11 @@ void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStre
         else k_fluid<T, false><<<grid_for(nt), BS, 0, s>>>(v, q, op.dof2node, op.node2dof, op.bkeys, K, op.ny, op.nx, op.nz, op.nvel, op.nborder, rr);
         return;
     }
+    if (op.physics == PHY_ELASTIC && op.lamu) {
+        WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
+        launch_elastic_walsh<T>(op, (const T *)op.lamu, v, q, red, t, s);
+        return;
+    }
     if (op.ebe) {
         k_fill<T><<<grid_for(op.ndof), BS, 0, s>>>(q, T(0), op.ndof);
         if (op.physics == PHY_THERMAL) k_ebe<T, 1><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N);
@@ -877,3 +945,36 @@ template void launch_convert<float, double>(const float *, double *, size_t, hip
 template void launch_convert<double, double>(const double *, double *, size_t, hipStream_t);
 
 }  // namespace chfem
+
+// ---------------------------------------------------------------------------
+// calibration: plain grid-stride FP64 copy (known traffic: 8 B read + 8 B
+// written per entry) for scaling hardware byte counters
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BS) k_copy_calibration(const double *__restrict__ a, double *__restrict__ b, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) b[i] = a[i];
+}
+
+extern "C" double chfemCopyKernelBench(size_t n, unsigned int reps)
+{
+    double *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * sizeof(double)) != hipSuccess) return -1.0;
+    if (hipMalloc(&b, n * sizeof(double)) != hipSuccess) { hipFree(a); return -1.0; }
+    hipMemset(a, 0, n * sizeof(double));
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    const int grid = (int)((n + BS - 1) / BS < 65536 ? (n + BS - 1) / BS : 65536);
+    k_copy_calibration<<<grid, BS>>>(a, b, n);
+    hipEventRecord(e0, 0);
+    for (unsigned r = 0; r < reps; r++) k_copy_calibration<<<grid, BS>>>(a, b, n);
+    hipEventRecord(e1, 0);
+    hipEventSynchronize(e1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    return reps ? ms / reps : 0.0;
+}

@@ -1,0 +1,75 @@
+// voxel.h -- local matrices of the unit-voxel trilinear hexahedron from exact
+// 1D integrals of the linear shape functions (mass 1/3 and 1/6, stiffness +-1,
+// mixed +-1/2).  Local node k of an element sits at (row + DR[k], col + DC[k],
+// layer + DL[k]) (chfem/src/femhmg/physical_phenomena/femhmg_3D.c:26-73); the
+// element frame is x = col offset, y = 1 - row offset and z = 1 - layer offset
+// for thermal/elastic, z = layer offset for Stokes.
+#pragma once
+
+namespace chfem {
+namespace voxel {
+
+static const int DR[8] = {1, 1, 0, 0, 1, 1, 0, 0};
+static const int DC[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+static const int DL[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+
+inline void frame(int k, bool stokes, int &x, int &y, int &z)
+{
+    x = DC[k];
+    y = 1 - DR[k];
+    z = stokes ? DL[k] : 1 - DL[k];
+}
+
+// int over the unit cube of d_p N_i * d_q N_j  (p or q = -1: no derivative)
+inline double int_dd(int i, int j, int p, int q, bool stokes)
+{
+    int ci[3], cj[3];
+    frame(i, stokes, ci[0], ci[1], ci[2]);
+    frame(j, stokes, cj[0], cj[1], cj[2]);
+    double v = 1.0;
+    for (int t = 0; t < 3; t++) {
+        const double si = ci[t] ? 1.0 : -1.0, sj = cj[t] ? 1.0 : -1.0;
+        if (t == p && t == q) v *= si * sj;
+        else if (t == p) v *= 0.5 * si;
+        else if (t == q) v *= 0.5 * sj;
+        else v *= ci[t] == cj[t] ? 1.0 / 3.0 : 1.0 / 6.0;
+    }
+    return v;
+}
+
+inline double lap(int i, int j, bool stokes)
+{
+    return int_dd(i, j, 0, 0, stokes) + int_dd(i, j, 1, 1, stokes) + int_dd(i, j, 2, 2, stokes);
+}
+
+// 24x24 isotropic stiffness lambda*A + mu*B, DOFs node-major (x,y,z)
+inline void elastic_K(double lam, double mu, bool stokes, double *K)
+{
+    for (int i = 0; i < 8; i++)
+        for (int a = 0; a < 3; a++)
+            for (int j = 0; j < 8; j++)
+                for (int b = 0; b < 3; b++)
+                    K[(3 * i + a) * 24 + 3 * j + b] =
+                        lam * int_dd(i, j, a, b, stokes) +
+                        mu * ((a == b ? lap(i, j, stokes) : 0.0) + int_dd(i, j, b, a, stokes));
+}
+
+// (lambda, mu) of a 24x24 matrix if it is an isotropic unit-voxel stiffness:
+// K[0x,0x] = (lambda + 4 mu)/9 and K[0x,0y] = (lambda + mu)/12 for node 0.
+inline bool isotropic_params(const double *K, double &lam, double &mu)
+{
+    const double a = 9.0 * K[0], b = 12.0 * K[1];
+    mu = (a - b) / 3.0;
+    lam = b - mu;
+    double R[576], kmax = 0.0, dmax = 0.0;
+    elastic_K(lam, mu, false, R);
+    for (int i = 0; i < 576; i++) {
+        kmax = fabs(K[i]) > kmax ? fabs(K[i]) : kmax;
+        const double d = fabs(K[i] - R[i]);
+        dmax = d > dmax ? d : dmax;
+    }
+    return kmax > 0.0 && dmax <= 1e-12 * kmax;
+}
+
+}  // namespace voxel
+}  // namespace chfem

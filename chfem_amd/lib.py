@@ -100,6 +100,16 @@ SIGNATURES = [
     ("hmgGetMaps", u8, [vp, vp, vp, vp, vp]),
     ("hmgGetLocalMatrices", u8, [vp, vp]),
     ("hmgGetSizes", u8, [C.POINTER(uint), C.POINTER(uint), C.POINTER(uint), C.POINTER(uint)]),
+    ("hmgSetSolverTolerance", None, [dbl]),
+    ("hmgSetMaxIterationsOverride", None, [uint]),
+    ("hmgPrepare", u8, []),
+    ("hmgSolveResident", u8, [vp]),
+    ("hmgRelease", None, []),
+    ("cudapcgSetKernelTiming", u8, [u8]),
+    ("cudapcgGetKernelTiming", u8, [C.POINTER(dbl), C.POINTER(uint)]),
+    ("hmgGetKernelTiming", u8, [C.POINTER(dbl), C.POINTER(uint)]),
+    ("chfemCopyKernelBench", dbl, [C.c_size_t, uint]),
+    ("cudapcgOperatorBench", dbl, [u8, uint]),
 ]
 
 

@@ -1,0 +1,58 @@
+"""Operator micro-benchmark: q = A d for a seeded two-phase sphere image, and a
+plain copy kernel of known traffic for scaling hardware byte counters.
+
+    python -m chfem_amd.opbench --physics elastic --size 256 --precision mixed --reps 20
+"""
+import argparse
+import json
+import os
+import tempfile
+
+import numpy as np
+
+from . import io as cio
+from . import synthetic
+from .lib import lib, ptr
+
+PHYS = {"thermal": 0, "elastic": 1}
+BYTES = {("thermal", 0): 17.0, ("thermal", 1): 9.0, ("elastic", 0): 49.0, ("elastic", 1): 25.0}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--physics", default="elastic", choices=list(PHYS))
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--strategy", type=int, default=0)
+    ap.add_argument("--copy", action="store_true", help="also time the calibration copy kernel")
+    a = ap.parse_args()
+    n = a.size
+    img = synthetic.spheres((n, n, n), rmin=6, rmax=12, fraction=0.3, seed=3)
+    props = {0: 1.0, 255: 10.0} if a.physics == "thermal" else {0: (1.0, 0.3), 255: (10.0, 0.2)}
+    d = tempfile.mkdtemp(prefix="chfem_op_")
+    f = os.path.join(d, "img.raw")
+    cio.export_for_chfem(f, img, PHYS[a.physics], props, export_raw=False)
+    L = lib()
+    assert L.hmgInit((f[:-4] + ".nf").encode(), None, None, ptr(np.ascontiguousarray(img).reshape(-1))) == 1
+    prec = 1 if a.precision == "mixed" else 0
+    L.hmgSetParallelStrategyFlag(a.strategy)
+    L.hmgSetPrecisionFlag(prec)
+    assert L.hmgPrepare() == 1
+    ms = L.cudapcgOperatorBench(prec, a.reps)
+    N = n ** 3
+    out = {"physics": a.physics, "size": n, "precision": a.precision, "strategy": a.strategy,
+           "avg_launch_ms": ms, "algorithmic_bytes": BYTES[(a.physics, prec)] * N,
+           "GBps": BYTES[(a.physics, prec)] * N / (ms * 1e-3) / 1e9}
+    L.hmgRelease()
+    L.hmgEnd()
+    if a.copy:
+        nc = 64 * 1024 * 1024
+        cms = L.chfemCopyKernelBench(nc, a.reps)
+        out["copy"] = {"entries": nc, "bytes_read": 8 * nc, "bytes_written": 8 * nc, "avg_launch_ms": cms,
+                       "GBps": 16 * nc / (cms * 1e-3) / 1e9}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

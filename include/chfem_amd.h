@@ -73,6 +73,38 @@ logical hmgGetLocalMatrices(double *Mtxs, double *CB);
 /* Sizes: nelem, ndof, velocity nodes, border nodes. */
 logical hmgGetSizes(unsigned int *nelem, unsigned int *ndof, unsigned int *nvel, unsigned int *nborder);
 
+/* Overrides the solver tolerance / iteration cap read from the .nf file. */
+void hmgSetSolverTolerance(double tol);
+void hmgSetMaxIterationsOverride(unsigned int maxit);
+
+/* Resident solves: hmgPrepare uploads the current model (maps, local
+ * matrices) and allocates the solver once; hmgSolveResident then runs every
+ * configured load case (device RHS -> solve -> device C column) against the
+ * HBM-resident model and writes C (C_dim entries); hmgRelease frees the
+ * solver.  hmgSolveHomogenization is prepare + one resident solve + release. */
+logical hmgPrepare(void);
+logical hmgSolveResident(var *C);
+void hmgRelease(void);
+
+/* Operator-launch timing inside the Krylov loops: with timing on, every
+ * operator launch of the inner solve is bracketed with HIP events on the
+ * solver's stream and the durations are summed. */
+cudapcgFlag_t cudapcgSetKernelTiming(cudapcgFlag_t on);
+cudapcgFlag_t cudapcgGetKernelTiming(double *op_ms_total, unsigned int *op_launches);
+/* Same counters for the last hmgSolveResident / hmgSolveHomogenization. */
+cudapcgFlag_t hmgGetKernelTiming(double *op_ms_total, unsigned int *op_launches);
+
+/* Operator micro-benchmark on the current solver (after cudapcgAllocateArrays
+ * or hmgPrepare): q = A d, reps launches, precision CHFEM_PREC_FP64 or
+ * CHFEM_PREC_MIXED (FP32 operator); returns the mean launch time in ms from
+ * HIP events on the solver's stream, or a negative value on failure. */
+double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps);
+
+/* Calibration kernel for hardware-counter traffic: a grid-stride FP64 copy of
+ * n entries, run reps times on device buffers it allocates; returns the mean
+ * duration in ms (HIP events), or a negative value on failure. */
+double chfemCopyKernelBench(size_t n, unsigned int reps);
+
 #ifdef __cplusplus
 }
 #endif

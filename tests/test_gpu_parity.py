@@ -143,3 +143,52 @@ def test_mixed_precision_elastic(ragged_elastic):
         assert st.sweeps >= 1
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("tile", ["32x8x16", "16x16x4", "64x4x7", "32x16x3"])
+@pytest.mark.parametrize("shape", [(40, 70, 9), (3, 1, 2), (17, 33, 65)])
+def test_walsh_operator_tiles(tile, shape, monkeypatch):
+    """Walsh-basis elastic operator (isotropic materials) on ragged, tiny and
+    multi-chunk grids for every tile shape, against the oracle's 24x24 form."""
+    monkeypatch.setenv("CHFEM_WALSH_TILE", tile)
+    img = three_phase(shape, 21)
+    m = H.model(H.ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25], colors=(0, 100, 255))
+    for strategy in (0, 1):
+        s = H.GpuSolver(m, strategy=strategy)
+        try:
+            v = np.random.default_rng(7).standard_normal(m.ndof)
+            assert rel(s.apply(v), m.apply(v)) < 1e-12
+        finally:
+            s.close()
+
+
+def test_generic_operator_path(ragged_elastic, monkeypatch):
+    """The 24x24 per-material kernel (used for non-isotropic local matrices)."""
+    monkeypatch.setenv("CHFEM_OPERATOR", "generic")
+    m = ragged_elastic
+    s = H.GpuSolver(m)
+    try:
+        v = np.random.default_rng(8).standard_normal(m.ndof)
+        assert rel(s.apply(v), m.apply(v)) < 1e-12
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_walsh_solve_multiblock(precision, monkeypatch):
+    """Solve with the Walsh operator on a grid of several blocks along each of
+    rows, columns and layers, so its fused d.Ad reduction runs on a 3-D grid."""
+    monkeypatch.setenv("CHFEM_WALSH_TILE", "16x16x4")
+    img = three_phase((14, 40, 33), 31)
+    m = H.model(H.ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25], colors=(0, 100, 255))
+    s = H.GpuSolver(m, precision=precision, tol=1e-9)
+    try:
+        b = m.rhs(4)
+        x, ok, st = s.solve(b)
+        xr, okr, itr, _ = m.solve(4, tol=1e-9)
+        assert ok and okr
+        assert rel(m.update_c(4, x), m.update_c(4, xr)) < 1e-7
+        if precision == 0:
+            assert abs(int(st.iterations) - itr) <= 2
+    finally:
+        s.close()
