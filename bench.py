@@ -132,6 +132,10 @@ def main():
         print(f"bench.py measures one MI355X only; --gpus {args.gpus} is not supported (run with --gpus 1).",
               file=sys.stderr)
         sys.exit(2)
+    # the library prints its solver reports on stdout: send them to stderr so
+    # that stdout carries only the JSON line
+    json_fd = os.dup(1)
+    os.dup2(2, 1)
     n = args.size
     from chfem_amd import io as cio
     from chfem_amd.lib import lib, ptr
@@ -199,6 +203,9 @@ def main():
                      "launches": fast["op_launches"]},
         "cpu_baseline": cpu,
     }
+    C.CDLL(None).fflush(None)
+    sys.stdout.flush()
+    os.dup2(json_fd, 1)
     print(json.dumps(out), flush=True)
 
 

@@ -46,7 +46,6 @@ struct Solver {
     float *x32 = nullptr, *r32 = nullptr, *d32 = nullptr, *q32 = nullptr, *s32 = nullptr, *dinv32 = nullptr;
     bool arrays = false, user_arrays = false, x0_set = false, dinv_valid = false;
     double *partials = nullptr, *S = nullptr, *Sh = nullptr;
-    unsigned *counter = nullptr;
     size_t partials_n = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -88,7 +87,6 @@ Reduce make_reduce(int fin, int slot)
 {
     Reduce r;
     r.partials = g->partials;
-    r.counter = g->counter;
     r.S = g->S;
     r.fin = fin;
     r.slot = slot;
@@ -157,7 +155,7 @@ static cudapcgFlag_t free_solver_arrays()
     if (!g->x0_set) dfree(g->x64);
     dfree(g->d64); dfree(g->q64); dfree(g->s64); dfree(g->dinv64);
     dfree(g->x32); dfree(g->r32); dfree(g->d32); dfree(g->q32); dfree(g->s32); dfree(g->dinv32);
-    dfree(g->partials); dfree(g->S); dfree(g->counter);
+    dfree(g->partials); dfree(g->S);
     if (g->Sh) hipHostFree(g->Sh);
     g->Sh = nullptr;
     g->arrays = false;
@@ -189,10 +187,6 @@ static cudapcgFlag_t alloc_solver_arrays()
     const size_t pn = (size_t)MAX_PARTIAL_BLOCKS * 8;
     if (!g->partials) CHECK_HIP(hipMalloc(&g->partials, pn * sizeof(double)));
     if (!g->S) CHECK_HIP(hipMalloc(&g->S, S_COUNT * sizeof(double)));
-    if (!g->counter) {
-        CHECK_HIP(hipMalloc(&g->counter, sizeof(unsigned)));
-        CHECK_HIP(hipMemsetAsync(g->counter, 0, sizeof(unsigned), g->stream));
-    }
     if (!g->Sh) CHECK_HIP(hipHostMalloc((void **)&g->Sh, S_COUNT * sizeof(double), hipHostMallocDefault));
     CHECK_HIP(hipMemsetAsync(g->S, 0, S_COUNT * sizeof(double), g->stream));
     g->arrays = true;

@@ -20,7 +20,7 @@ enum ScalarSlot {
     S_TMP2, S_TMP3, S_TMP4, S_TMP5, S_TMP6, S_TMP7, S_COUNT = 16
 };
 
-// How the last block of a fused reduction publishes its total.
+// How the finishing kernel of a fused reduction publishes its total.
 enum Finalize {
     FIN_STORE = 0,      // S[slot] = sum
     FIN_ALPHA = 1,      // S[S_DQ] = sum; S[S_ALPHA] = S[S_DELTA] / sum
@@ -28,8 +28,7 @@ enum Finalize {
 };
 
 struct Reduce {
-    double *partials;    // >= gridDim.x * 8 doubles
-    unsigned *counter;   // zero-initialised
+    double *partials;    // >= (blocks of the launch) * (values reduced) doubles
     double *S;           // scalar slots
     int fin;
     int slot;
@@ -56,6 +55,11 @@ struct OpDesc {
 };
 
 template <typename T> void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s);
+
+// Sums the nv x nb partials a reducing launch of nb blocks left in
+// red.partials (fixed order) and publishes them as red.fin says.  Every
+// launcher that takes a Reduce calls it after its kernel.
+void launch_reduce_finish(const Reduce &red, unsigned nb, int nv, hipStream_t s);
 
 // dinv[i] = 1 / diag(A)_i in precision T, from the keys (thermal/elastic) or the
 // Stokes pore/border structure.

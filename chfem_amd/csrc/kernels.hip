@@ -17,6 +17,7 @@
 // Reductions are FP64 and deterministic: per-block tree sums, then the last
 // block to finish sums the block partials in a fixed order.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 #include "kernels.h"
 #include "reduce.h"
 
@@ -134,7 +135,7 @@ __global__ void __launch_bounds__(BS) k_thermal_nbn(const T *__restrict__ v, T *
     }
     if (DOT) {
         double vv[1] = {acc};
-        reduce_finalize<1>(red, vv);
+        store_partials<1>(red, vv);
     }
 }
 
@@ -186,7 +187,7 @@ __global__ void __launch_bounds__(BS) k_elastic_nbn(const T *__restrict__ v, T *
     }
     if (DOT) {
         double vv[1] = {acc};
-        reduce_finalize<1>(red, vv);
+        store_partials<1>(red, vv);
     }
 }
 
@@ -302,7 +303,7 @@ __global__ void __launch_bounds__(BS) k_fluid(const T *__restrict__ v, T *__rest
     }
     if (DOT) {
         double vv[1] = {acc};
-        reduce_finalize<1>(red, vv);
+        store_partials<1>(red, vv);
     }
 }
 
@@ -384,7 +385,7 @@ __global__ void __launch_bounds__(BS) k_precond(const T *__restrict__ dinv, cons
     }
     if (RED) {
         double vv[1] = {acc};
-        reduce_finalize<1>(red, vv);
+        store_partials<1>(red, vv);
     }
 }
 
@@ -401,7 +402,7 @@ __global__ void __launch_bounds__(BS) k_dot(const T *__restrict__ a, const T *__
     }
     FOR_TAIL(n) acc += (double)a[i] * (double)b[i];
     double vv[1] = {acc};
-    reduce_finalize<1>(red, vv);
+    store_partials<1>(red, vv);
 }
 
 template <typename T>
@@ -461,7 +462,7 @@ __global__ void __launch_bounds__(BS) k_pcg_update(T *__restrict__ x, T *__restr
         acc += (double)rn * (double)zn;
     }
     double vv[1] = {acc};
-    reduce_finalize<1>(red, vv);
+    store_partials<1>(red, vv);
 }
 
 template <typename T>
@@ -495,7 +496,7 @@ __global__ void __launch_bounds__(BS) k_sub_precond(T *__restrict__ r, const T *
         acc += (double)rn * (double)zn;
     }
     double vv[1] = {acc};
-    reduce_finalize<1>(red, vv);
+    store_partials<1>(red, vv);
 }
 
 // MINRES: d = rt + b d ; q = s + b q ; reduce q.(dinv q)
@@ -530,7 +531,7 @@ __global__ void __launch_bounds__(BS) k_minres_dir(T *__restrict__ d, T *__restr
         acc += (double)qn * (double)(dinv[i] * qn);
     }
     double vv[1] = {acc};
-    reduce_finalize<1>(red, vv);
+    store_partials<1>(red, vv);
 }
 
 // MINRES: rn = rt/dinv - a q ; rt = dinv rn ; x += a d ; reduce rn.rt
@@ -566,7 +567,7 @@ __global__ void __launch_bounds__(BS) k_minres_update(T *__restrict__ x, T *__re
         acc += (double)rn * (double)rtn;
     }
     double vv[1] = {acc};
-    reduce_finalize<1>(red, vv);
+    store_partials<1>(red, vv);
 }
 
 __global__ void __launch_bounds__(BS) k_add_correction(double *x, const float *__restrict__ e, size_t n)
@@ -724,7 +725,7 @@ __global__ void __launch_bounds__(BS) k_updateC(const double *__restrict__ x, co
             }
         });
     }
-    reduce_finalize<6>(red, acc);
+    store_partials<6>(red, acc);
 }
 
 template <typename T>
@@ -736,7 +737,7 @@ __global__ void __launch_bounds__(BS) k_fluid_vsum(const T *__restrict__ x, uint
         acc[1] += (double)x[3 * (size_t)t + 1];
         acc[2] += (double)x[3 * (size_t)t + 2];
     }
-    reduce_finalize<3>(red, acc);
+    store_partials<3>(red, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -782,7 +783,69 @@ __global__ void k_interp(double *v, uint32_t nx, uint32_t ny, uint32_t nz, int s
 // ---------------------------------------------------------------------------
 static inline int grid_for(size_t n) { return reduce_blocks(n); }
 
-static Reduce noreduce() { Reduce r; r.partials = nullptr; r.counter = nullptr; r.S = nullptr; r.fin = 0; r.slot = 0; return r; }
+// Streaming vector kernels: one contiguous range of 16-byte vectors per
+// block, at most stream_cap() blocks (env CHFEM_STREAM_GRID, default 2048 =
+// 8 blocks of 4 wavefronts per CU on 256 CUs).
+static int stream_cap()
+{
+    static int cap = 0;
+    if (!cap) {
+        const char *e = getenv("CHFEM_STREAM_GRID");
+        cap = e ? atoi(e) : 2048;
+        if (cap < 1) cap = 1;
+        if (cap > MAX_RED_GRID) cap = MAX_RED_GRID;
+    }
+    return cap;
+}
+static inline int stream_grid(size_t n, size_t elem_bytes)
+{
+    const size_t nv = n * elem_bytes / 16;
+    size_t b = (nv + BS - 1) / BS;
+    if (b > (size_t)stream_cap()) b = stream_cap();
+    return b < 1 ? 1 : (int)b;
+}
+
+static Reduce noreduce() { Reduce r; r.partials = nullptr; r.S = nullptr; r.fin = 0; r.slot = 0; return r; }
+
+template <int NV>
+__global__ void __launch_bounds__(1024) k_reduce_finish(Reduce red, unsigned nb)
+{
+    double s[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        s[k] = 0;
+        for (unsigned b = threadIdx.x; b < nb; b += blockDim.x) s[k] += red.partials[(size_t)k * nb + b];
+    }
+    block_sum<NV>(s);
+    if (threadIdx.x == 0) {
+        double *S = red.S;
+        if (red.fin == FIN_ALPHA) {
+            S[S_DQ] = s[0];
+            S[S_ALPHA] = S[S_DELTA] / s[0];
+        } else if (red.fin == FIN_DELTA) {
+            S[S_DELTA_OLD] = S[S_DELTA];
+            S[S_DELTA] = s[0];
+        } else {
+#pragma unroll
+            for (int k = 0; k < NV; k++) S[red.slot + k] = s[k];
+        }
+    }
+}
+
+void launch_reduce_finish(const Reduce &red, unsigned nb, int nv, hipStream_t s)
+{
+    if (nv == 1) k_reduce_finish<1><<<1, 1024, 0, s>>>(red, nb);
+    else if (nv == 3) k_reduce_finish<3><<<1, 1024, 0, s>>>(red, nb);
+    else k_reduce_finish<6><<<1, 1024, 0, s>>>(red, nb);
+}
+
+// reducing launch over a 1-D grid, followed by its finishing kernel
+#define LAUNCH_RED(NV, KERNEL, GRID, s, red, ...)                                                  \
+    do {                                                                                           \
+        const int g_ = (GRID);                                                                     \
+        KERNEL<<<g_, BS, 0, s>>>(__VA_ARGS__);                                                     \
+        launch_reduce_finish(red, (unsigned)g_, NV, s);                                            \
+    } while (0)
 
 template <typename T>
 void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s)
@@ -791,7 +854,7 @@ void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStre
     const T *K = (const T *)op.K;
     if (op.physics == PHY_FLUID) {
         const uint32_t nt = op.nvel + op.nborder;
-        if (red) k_fluid<T, true><<<grid_for(nt), BS, 0, s>>>(v, q, op.dof2node, op.node2dof, op.bkeys, K, op.ny, op.nx, op.nz, op.nvel, op.nborder, rr);
+        if (red) LAUNCH_RED(1, (k_fluid<T, true>), grid_for(nt), s, rr, v, q, op.dof2node, op.node2dof, op.bkeys, K, op.ny, op.nx, op.nz, op.nvel, op.nborder, rr);
         else k_fluid<T, false><<<grid_for(nt), BS, 0, s>>>(v, q, op.dof2node, op.node2dof, op.bkeys, K, op.ny, op.nx, op.nz, op.nvel, op.nborder, rr);
         return;
     }
@@ -804,14 +867,14 @@ void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStre
         k_fill<T><<<grid_for(op.ndof), BS, 0, s>>>(q, T(0), op.ndof);
         if (op.physics == PHY_THERMAL) k_ebe<T, 1><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N);
         else k_ebe<T, 3><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N);
-        if (red) k_dot<T><<<grid_for(op.ndof), BS, 0, s>>>(v, q, op.ndof, rr);
+        if (red) LAUNCH_RED(1, k_dot<T>, stream_grid(op.ndof, sizeof(T)), s, rr, v, q, op.ndof, rr);
         return;
     }
     if (op.physics == PHY_THERMAL) {
-        if (red) k_thermal_nbn<T, true><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N, rr);
+        if (red) LAUNCH_RED(1, (k_thermal_nbn<T, true>), grid_for(op.N), s, rr, v, q, op.keys, K, op.ny, op.nx, op.nz, op.N, rr);
         else k_thermal_nbn<T, false><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N, rr);
     } else {
-        if (red) k_elastic_nbn<T, true><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N, rr);
+        if (red) LAUNCH_RED(1, (k_elastic_nbn<T, true>), grid_for(op.N), s, rr, v, q, op.keys, K, op.ny, op.nx, op.nz, op.N, rr);
         else k_elastic_nbn<T, false><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N, rr);
     }
 }
@@ -837,40 +900,40 @@ void launch_convert(const Tin *in, Tout *out, size_t n, hipStream_t s) { k_conve
 template <typename T>
 void launch_precond(const T *dinv, const T *v, T *out, size_t n, const Reduce *red, hipStream_t s)
 {
-    if (red) k_precond<T, true><<<grid_for(n), BS, 0, s>>>(dinv, v, out, n, *red);
-    else k_precond<T, false><<<grid_for(n), BS, 0, s>>>(dinv, v, out, n, noreduce());
+    if (red) LAUNCH_RED(1, (k_precond<T, true>), stream_grid(n, sizeof(T)), s, *red, dinv, v, out, n, *red);
+    else k_precond<T, false><<<stream_grid(n, sizeof(T)), BS, 0, s>>>(dinv, v, out, n, noreduce());
 }
 
 template <typename T>
 void launch_pcg_dir(T *d, const T *z, const double *S, int first, size_t n, hipStream_t s)
 {
-    k_pcg_dir<T><<<grid_for(n), BS, 0, s>>>(d, z, S, first, n);
+    k_pcg_dir<T><<<stream_grid(n, sizeof(T)), BS, 0, s>>>(d, z, S, first, n);
 }
 
 template <typename T>
 void launch_pcg_update(T *x, T *r, const T *d, T *z, const T *dinv, const Reduce *red, size_t n, hipStream_t s)
 {
-    k_pcg_update<T><<<grid_for(n), BS, 0, s>>>(x, r, d, z, dinv, n, *red);
+    LAUNCH_RED(1, k_pcg_update<T>, stream_grid(n, sizeof(T)), s, *red, x, r, d, z, dinv, n, *red);
 }
 
 template <typename T>
 void launch_sub_precond(T *r, const T *q, const T *dinv, T *z, const Reduce *red, size_t n, hipStream_t s)
 {
-    k_sub_precond<T><<<grid_for(n), BS, 0, s>>>(r, q, dinv, z, n, *red);
+    LAUNCH_RED(1, k_sub_precond<T>, stream_grid(n, sizeof(T)), s, *red, r, q, dinv, z, n, *red);
 }
 
 template <typename T>
 void launch_minres_dir(T *d, T *q, const T *rt, const T *sv, const T *dinv, const double *S, int first,
                        const Reduce *red, size_t n, hipStream_t s)
 {
-    k_minres_dir<T><<<grid_for(n), BS, 0, s>>>(d, q, rt, sv, dinv, S, first, n, *red);
+    LAUNCH_RED(1, k_minres_dir<T>, stream_grid(n, sizeof(T)), s, *red, d, q, rt, sv, dinv, S, first, n, *red);
 }
 
 template <typename T>
 void launch_minres_update(T *x, T *rt, const T *d, const T *q, const T *dinv, const Reduce *red, size_t n,
                           hipStream_t s)
 {
-    k_minres_update<T><<<grid_for(n), BS, 0, s>>>(x, rt, d, q, dinv, n, *red);
+    LAUNCH_RED(1, k_minres_update<T>, stream_grid(n, sizeof(T)), s, *red, x, rt, d, q, dinv, n, *red);
 }
 
 void launch_add_correction(double *x, const float *e, size_t n, hipStream_t s)
@@ -881,7 +944,7 @@ void launch_add_correction(double *x, const float *e, size_t n, hipStream_t s)
 template <typename T>
 void launch_dot(const T *a, const T *b, size_t n, const Reduce *red, hipStream_t s)
 {
-    k_dot<T><<<grid_for(n), BS, 0, s>>>(a, b, n, *red);
+    LAUNCH_RED(1, k_dot<T>, stream_grid(n, sizeof(T)), s, *red, a, b, n, *red);
 }
 
 void launch_rhs(const OpDesc &op, const double *K64, const LoadCase &lc, int dir, double *b, hipStream_t s)
@@ -900,15 +963,15 @@ void launch_updateC(const OpDesc &op, const double *CB64, const double *seamCB, 
                     const Reduce *red, hipStream_t s)
 {
     if (op.physics == PHY_THERMAL)
-        k_updateC<1><<<grid_for(op.N), BS, 0, s>>>(x, op.keys, CB64, seamCB, op.ebe, op.ny, op.nx, op.nz, op.N, lc, *red);
+        LAUNCH_RED(6, k_updateC<1>, grid_for(op.N), s, *red, x, op.keys, CB64, seamCB, op.ebe, op.ny, op.nx, op.nz, op.N, lc, *red);
     else
-        k_updateC<3><<<grid_for(op.N), BS, 0, s>>>(x, op.keys, CB64, seamCB, op.ebe, op.ny, op.nx, op.nz, op.N, lc, *red);
+        LAUNCH_RED(6, k_updateC<3>, grid_for(op.N), s, *red, x, op.keys, CB64, seamCB, op.ebe, op.ny, op.nx, op.nz, op.N, lc, *red);
 }
 
 template <typename T>
 void launch_fluid_velocity_sum(const OpDesc &op, const T *x, const Reduce *red, hipStream_t s)
 {
-    k_fluid_vsum<T><<<grid_for(op.nvel ? op.nvel : 1), BS, 0, s>>>(x, op.nvel, *red);
+    LAUNCH_RED(3, k_fluid_vsum<T>, grid_for(op.nvel ? op.nvel : 1), s, *red, x, op.nvel, *red);
 }
 
 void launch_interpl2(const double *coarse, double *fine, int rows, int cols, int layers, int stride, hipStream_t s)

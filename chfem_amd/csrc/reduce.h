@@ -1,6 +1,7 @@
 // reduce.h -- deterministic FP64 reductions shared by the kernel files:
-// per-block tree sums, then the last block to finish sums the block partials
-// in a fixed order and publishes the total (kernels.h: Reduce, Finalize).
+// per-block tree sums written as block partials, summed in a fixed order by a
+// separate single-block kernel (kernels.h: Reduce, Finalize,
+// launch_reduce_finish).
 #pragma once
 #include <hip/hip_runtime.h>
 #include "kernels.h"
@@ -40,17 +41,14 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
     __syncthreads();
 }
 
-__device__ __forceinline__ double load_coherent(const double *p)
-{
-    unsigned long long u = __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)u);
-}
-
+// Each block writes its NV partial sums to red.partials[k * nb + block]; the
+// launcher then enqueues the single-block finishing kernel, which adds the
+// partials in a fixed order.  No atomics or device-scope fences: on a
+// multi-XCD part a per-block release fence writes back the whole L2 of the
+// XCD, and same-address atomics from thousands of blocks serialise.
 template <int NV>
-__device__ __forceinline__ void reduce_finalize(const Reduce &red, double (&v)[NV])
+__device__ __forceinline__ void store_partials(const Reduce &red, double (&v)[NV])
 {
-    __shared__ int amLast;
     // linear block id / count: the launching grid may be 1-D, 2-D or 3-D
     const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
     const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
@@ -58,34 +56,6 @@ __device__ __forceinline__ void reduce_finalize(const Reduce &red, double (&v)[N
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < NV; k++) red.partials[(size_t)k * nb + bid] = v[k];
-        __threadfence();
-        unsigned prev = atomicAdd(red.counter, 1u);
-        amLast = (prev == nb - 1);
-    }
-    __syncthreads();
-    if (amLast) {
-        __threadfence();
-        double s[NV];
-#pragma unroll
-        for (int k = 0; k < NV; k++) {
-            s[k] = 0;
-            for (unsigned b = threadIdx.x; b < nb; b += blockDim.x) s[k] += load_coherent(red.partials + (size_t)k * nb + b);
-        }
-        block_sum<NV>(s);
-        if (threadIdx.x == 0) {
-            double *S = red.S;
-            if (red.fin == FIN_ALPHA) {
-                S[S_DQ] = s[0];
-                S[S_ALPHA] = S[S_DELTA] / s[0];
-            } else if (red.fin == FIN_DELTA) {
-                S[S_DELTA_OLD] = S[S_DELTA];
-                S[S_DELTA] = s[0];
-            } else {
-#pragma unroll
-                for (int k = 0; k < NV; k++) S[red.slot + k] = s[k];
-            }
-            __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
     }
 }
 

@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(TR *TC) k_elastic_walsh(const T *__restrict__ 
     }
     if (DOT) {
         double vv[1] = {dsum};
-        reduce_finalize<1>(red, vv);
+        store_partials<1>(red, vv);
     }
 }
 
@@ -286,7 +286,7 @@ template <typename T>
 void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduce *red, WalshTiles t,
                           hipStream_t s)
 {
-    Reduce rr = {nullptr, nullptr, nullptr, 0, 0};
+    Reduce rr = {nullptr, nullptr, 0, 0};
     if (red) rr = *red;
     int LZ = t.LZ > 0 ? t.LZ : 16;
     {
@@ -298,9 +298,11 @@ void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, con
 #define WALSH_LAUNCH(TRv, TCv)                                                                          \
     {                                                                                                   \
         dim3 grid((op.ny + TRv - 2) / (TRv - 1), (op.nx + TCv - 2) / (TCv - 1), (op.nz + LZ - 1) / LZ);             \
-        if (red) k_elastic_walsh<T, TRv, TCv, true><<<grid, TRv * TCv, 0, s>>>(v, q, op.keys, lamu, op.ebe,  \
-                                                                               op.ny, op.nx, op.nz, LZ, rr);   \
-        else k_elastic_walsh<T, TRv, TCv, false><<<grid, TRv * TCv, 0, s>>>(v, q, op.keys, lamu, op.ebe,  \
+        if (red) {                                                                                      \
+            k_elastic_walsh<T, TRv, TCv, true><<<grid, TRv * TCv, 0, s>>>(v, q, op.keys, lamu, op.ebe,     \
+                                                                          op.ny, op.nx, op.nz, LZ, rr);    \
+            launch_reduce_finish(rr, grid.x * grid.y * grid.z, 1, s);                                   \
+        } else k_elastic_walsh<T, TRv, TCv, false><<<grid, TRv * TCv, 0, s>>>(v, q, op.keys, lamu, op.ebe,  \
                                                                             op.ny, op.nx, op.nz, LZ, rr);      \
         return;                                                                                         \
     }
