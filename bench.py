@@ -127,6 +127,7 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg (tuning sweeps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X only; --gpus {args.gpus} is not supported (run with --gpus 1).",
@@ -171,8 +172,10 @@ def main():
     bytes_per_launch = 25.0 * N  # FP32 elastic apply: 1 B image + 12 B read + 12 B written per node
     achieved = bytes_per_launch / (op_avg_ms * 1e-3) / 1e9
     traffic = read_traffic(f"elastic_walsh_fp32_{n}")
-    log(f"[bench] cpu baseline (oracle)")
-    cpu = cpu_baseline(img, n, plain["iterations"])
+    cpu = None
+    if not args.skip_cpu:
+        log(f"[bench] cpu baseline (oracle)")
+        cpu = cpu_baseline(img, n, plain["iterations"])
     out = {
         "metric": METRIC,
         "value": fast["seconds"],

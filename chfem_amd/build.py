@@ -21,7 +21,7 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
-SOURCES = ["kernels.hip", "walsh.hip", "cudapcg.cpp", "femhmg.cpp"]
+SOURCES = ["kernels.hip", "walsh.hip", "guess.hip", "cudapcg.cpp", "femhmg.cpp", "guess.cpp"]
 
 
 def _run(cmd, verbose):
@@ -39,7 +39,7 @@ def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     for s in SOURCES:
-        o = os.path.join(BUILD, os.path.splitext(s)[0] + ".o")
+        o = os.path.join(BUILD, s.replace(".", "_") + ".o")
         extra = ["-fopenmp"] if s.endswith(".cpp") else []
         _run([HIPCC, "-x", "hip", "-c", os.path.join(SRC, s), "-o", o] + COMMON + extra, verbose)
         objs.append(o)

@@ -38,6 +38,9 @@ struct Solver {
     // results
     double residual = 1.0;
     unsigned iteration = 0, sweeps = 0;
+    // initial guess of the last solve: relative residual, kept or replaced
+    double x0_residual = 0;
+    unsigned x0_used = 0;
     double total_time = 0, mean_time = 0, solve_seconds = 0;
     cudapcgFlag_t found = CUDAPCG_FALSE;
     // device state
@@ -49,6 +52,8 @@ struct Solver {
     size_t partials_n = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Jacobi diagonal table (FP64) for the key-Jacobi vector kernels
+    double *kdiag64 = nullptr;
     // isotropic elastic materials for the Walsh-basis operator
     double *lamu64 = nullptr;
     float *lamu32 = nullptr;
@@ -220,6 +225,12 @@ static int make_ops()
     if (const char *t = getenv("CHFEM_WALSH_TILE")) sscanf(t, "%dx%dx%d", &o.tile_tr, &o.tile_tc, &o.tile_lz);
     const char *opsel = getenv("CHFEM_OPERATOR");
     const bool use_walsh = g->walsh_ok && !(opsel && !strcmp(opsel, "generic"));
+    // key-Jacobi vector kernels: NBN thermal/elastic with the Jacobi
+    // preconditioner on (CHFEM_VECTOR=stored keeps the stored-dinv kernels)
+    const char *vsel = getenv("CHFEM_VECTOR");
+    const bool kj = g->kdiag64 && g->precond && !o.ebe && o.physics != PHY_FLUID && !(vsel && !strcmp(vsel, "stored"));
+    o.kdiag = kj ? g->kdiag64 : nullptr;
+    o.nmat = (int)g->dm.nmat;
     g->dm.op64 = o;
     g->dm.op64.K = g->dm.K64;
     g->dm.op64.lamu = use_walsh ? g->lamu64 : nullptr;
@@ -231,16 +242,35 @@ static int make_ops()
 
 static bool above(double delta, double delta0, double tol) { return fabs(delta) > tol * tol * fabs(delta0); }
 
+// Records the relative residual of an initial guess; the guess is kept unless
+// its residual is larger than the zero guess's (CHFEM_X0_FALLBACK=0 keeps it
+// always, as the reference does).
+static void note_x0(double delta, double delta0)
+{
+    static int fallback = -1;
+    if (fallback < 0) {
+        const char *e = getenv("CHFEM_X0_FALLBACK");
+        fallback = e ? atoi(e) != 0 : 1;
+    }
+    g->x0_residual = sqrt(fabs(delta) / fabs(delta0));
+    g->x0_used = !(fallback && fabs(delta) > fabs(delta0));
+}
+
 // Jacobi-PCG in precision T (pcg.cu:28-243).  r holds b on entry; returns 1 if converged.
+// With op.kdiag (thermal/elastic NBN) the Jacobi diagonal comes from the keys
+// and z = M^-1 r is folded into the direction update (d = M^-1 r + beta d);
+// otherwise z is stored in q and the diagonal read from dinv.
 template <typename T>
 static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool x0set, double tol, unsigned maxit,
                    unsigned &it, double &residual, double *delta0_out)
 {
     const size_t n = op.ndof;
     hipStream_t st = g->stream;
+    const bool kj = op.kdiag != nullptr;
     if (!x0set) launch_fill<T>(x, 0.0, n, st);
     Reduce r0 = make_reduce(FIN_STORE, S_DELTA0);
-    launch_precond<T>(dinv, r, d, n, &r0, st);
+    if (kj) launch_kj_precond<T>(op, r, d, &r0, st);
+    else launch_precond<T>(dinv, r, d, n, &r0, st);
     if (!sync_scalars()) return -1;
     const double delta0 = g->Sh[S_DELTA0];
     if (delta0_out) *delta0_out = delta0;
@@ -254,26 +284,42 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
     if (x0set) {
         launch_apply<T>(op, x, q, nullptr, st);
         Reduce rd = make_reduce(FIN_STORE, S_DELTA);
-        launch_sub_precond<T>(r, q, dinv, d, &rd, n, st);
+        if (kj) launch_kj_sub_precond<T>(op, r, q, d, &rd, st);
+        else launch_sub_precond<T>(r, q, dinv, d, &rd, n, st);
         if (!sync_scalars()) return -1;
         delta = g->Sh[S_DELTA];
-        if (!above(delta, delta0, tol)) {
+        note_x0(delta, delta0);
+        if (!g->x0_used) {
+            // the guess is worse than zero: back to x = 0, r = b, d = M^-1 b
+            launch_add<T>(r, q, n, st);
+            launch_fill<T>(x, 0.0, n, st);
+            Reduce r0b = make_reduce(FIN_STORE, S_DELTA);
+            if (kj) launch_kj_precond<T>(op, r, d, &r0b, st);
+            else launch_precond<T>(dinv, r, d, n, &r0b, st);
+            if (!sync_scalars()) return -1;
+            delta = g->Sh[S_DELTA];
+        } else if (!above(delta, delta0, tol)) {
             residual = sqrt(fabs(delta) / fabs(delta0));
             return 1;
         }
     } else if (!set_scalar(S_DELTA, delta0)) return -1;
     Reduce ra = make_reduce(FIN_ALPHA, 0), rdl = make_reduce(FIN_DELTA, 0);
+    auto update = [&]() {
+        if (kj) launch_kj_pcg_update<T>(op, x, r, d, q, &rdl, st);
+        else launch_pcg_update<T>(x, r, d, q, dinv, &rdl, n, st);
+    };
     it = 1;
     timed_apply<T>(op, d, q, &ra);
-    launch_pcg_update<T>(x, r, d, q, dinv, &rdl, n, st);
+    update();
     if (!sync_loop()) return -1;
     delta = g->Sh[S_DELTA];
     bool cont = above(delta, delta0, tol);
     while (cont && it < maxit) {
         it++;
-        launch_pcg_dir<T>(d, q, g->S, 0, n, st);
+        if (kj) launch_kj_pcg_dir<T>(op, d, r, g->S, st);
+        else launch_pcg_dir<T>(d, q, g->S, 0, n, st);
         timed_apply<T>(op, d, q, &ra);
-        launch_pcg_update<T>(x, r, d, q, dinv, &rdl, n, st);
+        update();
         if (!sync_loop()) return -1;
         delta = g->Sh[S_DELTA];
         cont = above(delta, delta0, tol);
@@ -362,9 +408,106 @@ static double inner_tol_floor()
     return e ? atof(e) : 1e-5;
 }
 
+static double reliable_drop()
+{
+    const char *e = getenv("CHFEM_RELIABLE_DROP");
+    return e ? atof(e) : 1e-2;
+}
+
+// Mixed-precision Jacobi-PCG with reliable updates: one FP32 CG iteration
+// sequence (FP32 operator and vectors, FP64 dots/alpha/beta) whose solution
+// is flushed into the FP64 x and whose residual is replaced by the FP64
+// residual b - A x whenever the iterated r.z has dropped by CHFEM_RELIABLE_DROP
+// (default 1e-2) since the last such update, or looks converged.  The search
+// direction is kept across updates, so the Krylov history is not lost as it
+// is with restarted refinement.  Convergence is judged on the FP64 residual
+// with the same rule as the FP64 solver (delta <= tol^2 delta0).
+static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, double &residual)
+{
+    const OpDesc &o64 = g->dm.op64, &o32 = g->dm.op32;
+    const size_t n = o64.ndof;
+    hipStream_t st = g->stream;
+    const double tol = g->num_tol;
+    const bool kj = o32.kdiag != nullptr;
+    float *x = g->x32, *r = g->r32, *d = g->d32, *q = g->q32;
+    const float *dinv = g->dinv32;
+    Reduce r0 = make_reduce(FIN_STORE, S_DELTA0), rd = make_reduce(FIN_STORE, S_DELTA);
+    launch_precond<double>(g->dinv64, g->r64, nullptr, n, &r0, st);
+    if (!sync_scalars()) return -1;
+    const double delta0 = g->Sh[S_DELTA0];
+    it_total = 0;
+    sweeps = 0;
+    residual = 0;
+    if (fabs(delta0) < 1e-12) {
+        launch_fill<double>(g->x64, 0.0, n, st);
+        return 1;
+    }
+    double delta = delta0;
+    if (x0set) {
+        launch_apply<double>(o64, g->x64, g->q64, nullptr, st);
+        launch_sub_precond<double>(g->r64, g->q64, g->dinv64, g->d64, &rd, n, st);
+        if (!sync_scalars()) return -1;
+        delta = g->Sh[S_DELTA];
+        note_x0(delta, delta0);
+        if (!g->x0_used) {
+            // the guess is worse than zero: back to x = 0, r = b
+            launch_add<double>(g->r64, g->q64, n, st);
+            launch_fill<double>(g->x64, 0.0, n, st);
+            delta = delta0;
+        } else if (!above(delta, delta0, tol)) {
+            residual = sqrt(fabs(delta) / fabs(delta0));
+            return 1;
+        }
+    } else launch_fill<double>(g->x64, 0.0, n, st);
+    const double drop = reliable_drop();
+    double delta_rel = delta;
+    // FP32 start: r = (float) r64, x = 0, d = M^-1 r, S[delta] = r.d
+    launch_convert<double, float>(g->r64, r, n, st);
+    launch_fill<float>(x, 0.0, n, st);
+    if (kj) launch_kj_precond<float>(o32, r, d, &rd, st);
+    else launch_precond<float>(dinv, r, d, n, &rd, st);
+    Reduce ra = make_reduce(FIN_ALPHA, 0), rdl = make_reduce(FIN_DELTA, 0);
+    bool converged = false;
+    while (it_total < g->max_iterations) {
+        if (it_total > 0) {
+            if (kj) launch_kj_pcg_dir<float>(o32, d, r, g->S, st);
+            else launch_pcg_dir<float>(d, q, g->S, 0, n, st);
+        }
+        timed_apply<float>(o32, d, q, &ra);
+        if (kj) launch_kj_pcg_update<float>(o32, x, r, d, q, &rdl, st);
+        else launch_pcg_update<float>(x, r, d, q, dinv, &rdl, n, st);
+        it_total++;
+        if (!sync_loop()) return -1;
+        delta = g->Sh[S_DELTA];
+        const bool last = it_total >= g->max_iterations;
+        if (!last && above(delta, delta0, tol) && fabs(delta) > drop * fabs(delta_rel)) continue;
+        // reliable update: x64 += x ; r64 -= A64 x ; x = 0 ; r = (float) r64
+        launch_add_correction(g->x64, x, n, st);
+        launch_convert<float, double>(x, g->d64, n, st);
+        launch_fill<float>(x, 0.0, n, st);
+        launch_apply<double>(o64, g->d64, g->q64, nullptr, st);
+        launch_sub_precond<double>(g->r64, g->q64, g->dinv64, g->d64, &rd, n, st);  // S[delta] = FP64 r.z
+        if (!sync_scalars()) return -1;
+        sweeps++;
+        const double prev = delta_rel;
+        delta = g->Sh[S_DELTA];
+        if (!above(delta, delta0, tol)) {
+            converged = true;
+            break;
+        }
+        if (last || !(fabs(delta) < fabs(prev))) break;  // no progress left in FP32
+        delta_rel = delta;
+        launch_convert<double, float>(g->r64, r, n, st);
+        if (!kj) launch_precond<float>(dinv, r, q, n, nullptr, st);  // stored-dinv direction reads z from q
+    }
+    residual = sqrt(fabs(delta) / fabs(delta0));
+    return converged ? 1 : 0;
+}
+
 // FP32 inner Krylov + FP64 outer iterative refinement.
 static int run_mixed(bool x0set, unsigned &it_total, unsigned &sweeps, double &residual)
 {
+    if (!is_minres()) return run_mixed_reliable(x0set, it_total, sweeps, residual);
     const OpDesc &o64 = g->dm.op64, &o32 = g->dm.op32;
     const size_t n = o64.ndof;
     hipStream_t st = g->stream;
@@ -431,6 +574,8 @@ int solve_device(const double *x0_dev)
         x0set = true;
     }
     g->count++;
+    g->x0_residual = 0;
+    g->x0_used = 0;
     hipEventRecord(g->ev0, g->stream);
     unsigned it = 0, sweeps = 1;
     double res = 0;
@@ -519,6 +664,8 @@ cudapcgFlag_t cudapcgGetSolveStats(chfemSolveStats_t *s)
     s->sweeps = g->sweeps;
     s->solve_seconds = g->solve_seconds;
     s->residual = g->residual;
+    s->x0_residual = g->x0_residual;
+    s->x0_used = g->x0_used;
     return CUDAPCG_TRUE;
 }
 
@@ -574,6 +721,7 @@ cudapcgFlag_t cudapcgEnd(void)
     dfree(g->dm.K32);
     dfree(g->lamu64);
     dfree(g->lamu32);
+    dfree(g->kdiag64);
     if (g->model && g->model->freeAllowed_flag) {
         free_model_device(g->model);
         free(g->model);
@@ -813,6 +961,23 @@ cudapcgFlag_t cudapcgSetLclMtxs(cudapcgVar_t *L)
         }
     }
     for (size_t i = 0; i < ksz * nk; i++) Kf[i] = (float)K[i];
+    // Jacobi diagonal table for the key-Jacobi vector kernels (2-bit keys:
+    // at most 4 materials)
+    dfree(g->kdiag64);
+    if (ph != PHY_FLUID && nk <= 4) {
+        const int nd = ph == PHY_ELASTIC ? 3 : 1, ke = 8 * nd;
+        double kd[4 * 8 * 3];
+        for (unsigned m = 0; m < nk; m++)
+            for (int e = 0; e < 8; e++)
+                for (int a = 0; a < nd; a++) kd[(m * 8 + e) * nd + a] = K[m * ksz + (size_t)(nd * e + a) * ke + nd * e + a];
+        if (hipMalloc(&g->kdiag64, sizeof(double) * nk * 8 * nd) != hipSuccess ||
+            hipMemcpy(g->kdiag64, kd, sizeof(double) * nk * 8 * nd, hipMemcpyHostToDevice) != hipSuccess) {
+            fprintf(stderr, "chfem: failed to upload the Jacobi table\n");
+            free(K);
+            free(Kf);
+            return CUDAPCG_FALSE;
+        }
+    }
     dfree(g->dm.K64);
     dfree(g->dm.K32);
     hipError_t e1 = hipMalloc(&g->dm.K64, ksz * nk * sizeof(double));

@@ -19,6 +19,7 @@
 #include <string>
 #include <vector>
 
+#include "guess.h"
 #include "solver.h"
 #include "voxel.h"
 #include "../../include/femhmg.h"
@@ -769,6 +770,17 @@ logical hmgSolveResident(var *Cout)
     directions(m, dirs);
     std::vector<double> xfile;
     bool ok = true;
+    // learned initial guess: one prediction for all load cases
+    const float *gy = nullptr;
+    int gC = 0;
+    double *x0dev = nullptr;
+    if (guess_active && !m.using_x0 && (m.analysis == HMG_THERMAL || m.analysis == HMG_ELASTIC)) {
+        gy = guess_predict_device(guess_active, m.elem_map.data(), m.props, m.nmat, m.ex, m.ey, m.ez, &gC);
+        if (gy && (hipStreamSynchronize(guess_stream(guess_active)) != hipSuccess ||
+                   hipMalloc(&x0dev, m.ndof * sizeof(double)) != hipSuccess))
+            gy = nullptr;
+        if (!gy) appendf("    | learned initial guess unavailable for this grid: zero guess\n");
+    }
     for (unsigned d : dirs) {
         appendf("    ------------------------------------------------------\n    Analysis on %s:\n", sim_str[d]);
         m.hmg_flag = d;
@@ -793,7 +805,14 @@ logical hmgSolveResident(var *Cout)
                 }
             }
         }
-        const int rc = solve_device(nullptr);
+        int rc;
+        if (gy && x0dev) {
+            nn::guess_to_x0(gy, gC, (int)d, m.analysis == HMG_ELASTIC ? 3 : 1,
+                            nn::Grid3{(int)m.ey, (int)m.ex, (int)m.ez}, 1.0f, x0dev, active_stream());
+            rc = solve_device(x0dev);
+        } else {
+            rc = solve_device(nullptr);
+        }
         chfemSolveStats_t st;
         cudapcgGetSolveStats(&st);
         m.stats.push_back(st);
@@ -823,11 +842,44 @@ logical hmgSolveResident(var *Cout)
             for (unsigned jj = 0; jj < 3; jj++) printf("        %.8e\n", m.C[jj * 3 + d]);
         }
     }
+    if (x0dev) (void)hipFree(x0dev);
     cudapcgGetKernelTiming(&g_op_ms, &g_op_count);
     if (m.save_fields) printf("WARNING: field export is not provided by this library.\n");
     appendf("    ------------------------------------------------------\n");
     if (Cout) memcpy(Cout, m.C, m.C_dim * sizeof(double));
     return ok ? HMG_TRUE : HMG_FALSE;
+}
+
+logical hmgSolveLoadCase(unsigned int dir, const var *x0, var *x, var *Ccol)
+{
+    if (!M || !g_prepared || dir >= (M->analysis == HMG_ELASTIC ? 6u : 3u)) return HMG_FALSE;
+    Model &m = *M;
+    m.hmg_flag = dir;
+    device_rhs(m, g_aux, dir);
+    int rc;
+    if (x0) {
+        double *xd = nullptr;
+        if (hipMalloc(&xd, m.ndof * sizeof(double)) != hipSuccess) return HMG_FALSE;
+        if (hipMemcpy(xd, x0, m.ndof * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(xd);
+            return HMG_FALSE;
+        }
+        rc = solve_device(xd);
+        (void)hipFree(xd);
+    } else {
+        cudapcgSetX0(nullptr, CUDAPCG_FALSE);
+        rc = solve_device(nullptr);
+    }
+    if (rc < 0) return HMG_FALSE;
+    if (x && hipMemcpy(x, solution_buffer(), m.ndof * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return HMG_FALSE;
+    if (Ccol) {
+        for (unsigned i = 0; i < m.C_dim; i++) m.C[i] = 0.0;
+        if (!device_updateC(m, g_aux, dir, solution_buffer())) return HMG_FALSE;
+        const unsigned n = m.analysis == HMG_ELASTIC ? 6 : 3;
+        for (unsigned i = 0; i < n; i++) Ccol[i] = m.C[i * n + dir];
+    }
+    return rc == 1 ? HMG_TRUE : HMG_FALSE;
 }
 
 cudapcgFlag_t hmgGetKernelTiming(double *op_ms_total, unsigned int *op_launches)

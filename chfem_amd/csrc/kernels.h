@@ -52,6 +52,10 @@ struct OpDesc {
     // non-null selects the Walsh-basis operator (walsh.hip)
     const void *lamu;
     int tile_tr, tile_tc, tile_lz;
+    // Jacobi diagonal table kdiag[(m*8 + e)*nd + a] (FP64, nmat <= 4): non-null
+    // selects the key-Jacobi vector kernels (thermal/elastic NBN)
+    const double *kdiag;
+    int nmat;
 };
 
 template <typename T> void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s);
@@ -86,6 +90,21 @@ template <typename T> void launch_minres_update(T *x, T *rt, const T *d, const T
                                                 size_t n, hipStream_t s);
 // FP64 outer refinement: x64 += e32
 void launch_add_correction(double *x, const float *e, size_t n, hipStream_t s);
+// y += x
+template <typename T> void launch_add(T *y, const T *x, size_t n, hipStream_t s);
+
+// Key-Jacobi PCG vector kernels (op.kdiag non-null; thermal/elastic NBN):
+// dinv is rebuilt from the keys and z = dinv r is never stored.
+//   kj_precond:    out = dinv v ; red <- v.out
+//   kj_pcg_dir:    d = dinv r + (S[delta]/S[delta_old]) d
+//   kj_pcg_update: x += alpha d ; r -= alpha q ; red <- r.(dinv r)
+//   kj_sub_precond: r -= q ; z = dinv r ; red <- r.z
+template <typename T> void launch_kj_precond(const OpDesc &op, const T *v, T *out, const Reduce *red, hipStream_t s);
+template <typename T> void launch_kj_pcg_dir(const OpDesc &op, T *d, const T *r, const double *S, hipStream_t s);
+template <typename T> void launch_kj_pcg_update(const OpDesc &op, T *x, T *r, const T *d, const T *q,
+                                                const Reduce *red, hipStream_t s);
+template <typename T> void launch_kj_sub_precond(const OpDesc &op, T *r, const T *q, T *z, const Reduce *red,
+                                                 hipStream_t s);
 // dot reduction of two vectors
 template <typename T> void launch_dot(const T *a, const T *b, size_t n, const Reduce *red, hipStream_t s);
 

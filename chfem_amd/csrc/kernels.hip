@@ -570,6 +570,243 @@ __global__ void __launch_bounds__(BS) k_minres_update(T *__restrict__ x, T *__re
     store_partials<1>(red, vv);
 }
 
+// ---------------------------------------------------------------------------
+// Key-Jacobi vector kernels (thermal/elastic NBN): the Jacobi diagonal is
+// rebuilt from the node's packed key instead of being read from a stored
+// vector, as the reference's preconditioner does (matrix_free.cu:167-188,
+// 388-408), and the preconditioned residual z = dinv r is never stored.
+// Work is done in groups of 4 nodes (4*ND entries = whole 16-byte vectors).
+// kdiag[(m*8 + e)*ND + a] is the diagonal entry of material m's local matrix
+// at local node e, component a; sums run over e = 0..7 in FP64 and the
+// inverse is rounded to T, exactly as k_jacobi_keys computes dinv.
+// ---------------------------------------------------------------------------
+template <typename T, int ND>
+struct Group {
+    static constexpr int E = 4 * ND;
+    static constexpr int NV = E * (int)sizeof(T) / 16;
+    T v[E];
+};
+template <typename T, int ND>
+__device__ __forceinline__ void gload(const T *p, Group<T, ND> &g)
+{
+    uint4 t[Group<T, ND>::NV];
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+#pragma unroll
+    for (int i = 0; i < Group<T, ND>::NV; i++) t[i] = q[i];
+    __builtin_memcpy(g.v, t, sizeof(t));
+}
+template <typename T, int ND>
+__device__ __forceinline__ void gstore(T *p, const Group<T, ND> &g)
+{
+    uint4 t[Group<T, ND>::NV];
+    __builtin_memcpy(t, g.v, sizeof(t));
+    uint4 *q = reinterpret_cast<uint4 *>(p);
+#pragma unroll
+    for (int i = 0; i < Group<T, ND>::NV; i++) q[i] = t[i];
+}
+
+template <int ND>
+__device__ __forceinline__ double *kj_table(const double *kdiag, int nmat)
+{
+    __shared__ double sd[4 * 8 * ND];
+    for (int t = threadIdx.x; t < 4 * 8 * ND; t += blockDim.x) sd[t] = t < nmat * 8 * ND ? kdiag[t] : 1.0;
+    __syncthreads();
+    return sd;
+}
+
+template <typename T, int ND>
+__device__ __forceinline__ void node_dinv(const double *sd, unsigned key, T *dv)
+{
+#pragma unroll
+    for (int a = 0; a < ND; a++) {
+        double s = 0;
+#pragma unroll
+        for (int e = 0; e < 8; e++) s += sd[(((key >> (2 * e)) & 3u) * 8 + e) * ND + a];
+        dv[a] = (T)(1.0 / s);
+    }
+}
+
+template <typename T, int ND>
+__device__ __forceinline__ void group_dinv(const double *sd, const uint16_t *keys, size_t g, Group<T, ND> &dv)
+{
+    const uint2 k = *reinterpret_cast<const uint2 *>(keys + 4 * g);
+    node_dinv<T, ND>(sd, k.x & 0xffffu, dv.v);
+    node_dinv<T, ND>(sd, k.x >> 16, dv.v + ND);
+    node_dinv<T, ND>(sd, k.y & 0xffffu, dv.v + 2 * ND);
+    node_dinv<T, ND>(sd, k.y >> 16, dv.v + 3 * ND);
+}
+
+// block b owns groups [gb_, ge_); the last block also takes the N % 4 tail nodes
+#define GROUP_RANGE(N)                                                                                \
+    const size_t ng_ = (N) / 4;                                                                       \
+    const size_t per_ = (ng_ + gridDim.x - 1) / gridDim.x;                                            \
+    const size_t gb_ = (size_t)blockIdx.x * per_;                                                     \
+    const size_t ge_ = gb_ + per_ < ng_ ? gb_ + per_ : ng_;
+#define FOR_GROUPS for (size_t g_ = gb_ + threadIdx.x; g_ < ge_; g_ += BS)
+#define FOR_TAIL_NODES(N)                                                                             \
+    if (blockIdx.x == gridDim.x - 1)                                                                  \
+        for (size_t i = ng_ * 4 + threadIdx.x; i < (N); i += BS)
+
+// out = dinv v ; reduce v.out
+template <typename T, int ND>
+__global__ void __launch_bounds__(BS) k_kj_precond(const uint16_t *__restrict__ keys, const double *__restrict__ kdiag,
+                                                   int nmat, const T *__restrict__ v, T *__restrict__ out,
+                                                   size_t N, Reduce red)
+{
+    const double *sd = kj_table<ND>(kdiag, nmat);
+    double acc = 0;
+    GROUP_RANGE(N)
+    FOR_GROUPS
+    {
+        Group<T, ND> a, m, o;
+        gload<T, ND>(v + g_ * 4 * ND, a);
+        group_dinv<T, ND>(sd, keys, g_, m);
+#pragma unroll
+        for (int k = 0; k < 4 * ND; k++) {
+            o.v[k] = m.v[k] * a.v[k];
+            acc += (double)a.v[k] * (double)o.v[k];
+        }
+        gstore<T, ND>(out + g_ * 4 * ND, o);
+    }
+    FOR_TAIL_NODES(N)
+    {
+        T dv[ND];
+        node_dinv<T, ND>(sd, keys[i], dv);
+#pragma unroll
+        for (int a = 0; a < ND; a++) {
+            const T o = dv[a] * v[ND * i + a];
+            out[ND * i + a] = o;
+            acc += (double)v[ND * i + a] * (double)o;
+        }
+    }
+    double vv[1] = {acc};
+    store_partials<1>(red, vv);
+}
+
+// d = dinv r + beta d
+template <typename T, int ND>
+__global__ void __launch_bounds__(BS) k_kj_pcg_dir(T *__restrict__ d, const T *__restrict__ r,
+                                                   const uint16_t *__restrict__ keys, const double *__restrict__ kdiag,
+                                                   int nmat, const double *__restrict__ S, size_t N)
+{
+    const double *sd = kj_table<ND>(kdiag, nmat);
+    const T beta = (T)(S[S_DELTA] / S[S_DELTA_OLD]);
+    GROUP_RANGE(N)
+    FOR_GROUPS
+    {
+        Group<T, ND> rv, dv, m;
+        gload<T, ND>(r + g_ * 4 * ND, rv);
+        gload<T, ND>(d + g_ * 4 * ND, dv);
+        group_dinv<T, ND>(sd, keys, g_, m);
+#pragma unroll
+        for (int k = 0; k < 4 * ND; k++) dv.v[k] = m.v[k] * rv.v[k] + beta * dv.v[k];
+        gstore<T, ND>(d + g_ * 4 * ND, dv);
+    }
+    FOR_TAIL_NODES(N)
+    {
+        T dvn[ND];
+        node_dinv<T, ND>(sd, keys[i], dvn);
+#pragma unroll
+        for (int a = 0; a < ND; a++) d[ND * i + a] = dvn[a] * r[ND * i + a] + beta * d[ND * i + a];
+    }
+}
+
+// x += a d ; r -= a q ; reduce r.(dinv r)
+template <typename T, int ND>
+__global__ void __launch_bounds__(BS) k_kj_pcg_update(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ d,
+                                                      const T *__restrict__ q, const uint16_t *__restrict__ keys,
+                                                      const double *__restrict__ kdiag, int nmat, size_t N,
+                                                      Reduce red)
+{
+    const double *sd = kj_table<ND>(kdiag, nmat);
+    const T a = (T)red.S[S_ALPHA];
+    double acc = 0;
+    GROUP_RANGE(N)
+    FOR_GROUPS
+    {
+        const size_t o = g_ * 4 * ND;
+        Group<T, ND> xv, rv, dv, qv, m;
+        gload<T, ND>(x + o, xv);
+        gload<T, ND>(r + o, rv);
+        gload<T, ND>(d + o, dv);
+        gload<T, ND>(q + o, qv);
+        group_dinv<T, ND>(sd, keys, g_, m);
+#pragma unroll
+        for (int k = 0; k < 4 * ND; k++) {
+            xv.v[k] += a * dv.v[k];
+            rv.v[k] -= a * qv.v[k];
+            acc += (double)rv.v[k] * (double)(m.v[k] * rv.v[k]);
+        }
+        gstore<T, ND>(x + o, xv);
+        gstore<T, ND>(r + o, rv);
+    }
+    FOR_TAIL_NODES(N)
+    {
+        T dvn[ND];
+        node_dinv<T, ND>(sd, keys[i], dvn);
+#pragma unroll
+        for (int c = 0; c < ND; c++) {
+            const size_t j = ND * i + c;
+            x[j] += a * d[j];
+            const T rn = r[j] - a * q[j];
+            r[j] = rn;
+            acc += (double)rn * (double)(dvn[c] * rn);
+        }
+    }
+    double vv[1] = {acc};
+    store_partials<1>(red, vv);
+}
+
+// r -= q ; z = dinv r ; reduce r.z
+template <typename T, int ND>
+__global__ void __launch_bounds__(BS) k_kj_sub_precond(T *__restrict__ r, const T *__restrict__ q, T *__restrict__ z,
+                                                       const uint16_t *__restrict__ keys,
+                                                       const double *__restrict__ kdiag, int nmat, size_t N,
+                                                       Reduce red)
+{
+    const double *sd = kj_table<ND>(kdiag, nmat);
+    double acc = 0;
+    GROUP_RANGE(N)
+    FOR_GROUPS
+    {
+        const size_t o = g_ * 4 * ND;
+        Group<T, ND> rv, qv, m, zv;
+        gload<T, ND>(r + o, rv);
+        gload<T, ND>(q + o, qv);
+        group_dinv<T, ND>(sd, keys, g_, m);
+#pragma unroll
+        for (int k = 0; k < 4 * ND; k++) {
+            rv.v[k] -= qv.v[k];
+            zv.v[k] = m.v[k] * rv.v[k];
+            acc += (double)rv.v[k] * (double)zv.v[k];
+        }
+        gstore<T, ND>(r + o, rv);
+        gstore<T, ND>(z + o, zv);
+    }
+    FOR_TAIL_NODES(N)
+    {
+        T dvn[ND];
+        node_dinv<T, ND>(sd, keys[i], dvn);
+#pragma unroll
+        for (int c = 0; c < ND; c++) {
+            const size_t j = ND * i + c;
+            const T rn = r[j] - q[j];
+            r[j] = rn;
+            const T zn = dvn[c] * rn;
+            z[j] = zn;
+            acc += (double)rn * (double)zn;
+        }
+    }
+    double vv[1] = {acc};
+    store_partials<1>(red, vv);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BS) k_add(T *y, const T *__restrict__ x, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) y[i] += x[i];
+}
+
 __global__ void __launch_bounds__(BS) k_add_correction(double *x, const float *__restrict__ e, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) x[i] += (double)e[i];
@@ -936,6 +1173,55 @@ void launch_minres_update(T *x, T *rt, const T *d, const T *q, const T *dinv, co
     LAUNCH_RED(1, k_minres_update<T>, stream_grid(n, sizeof(T)), s, *red, x, rt, d, q, dinv, n, *red);
 }
 
+// Key-Jacobi launchers: one block per contiguous range of 4-node groups
+static inline int group_grid(const OpDesc &op, size_t)
+{
+    size_t b = ((size_t)op.N / 4 + BS - 1) / BS;
+    if (b > (size_t)stream_cap()) b = stream_cap();
+    return b < 1 ? 1 : (int)b;
+}
+#define KJ_DISPATCH(KERNEL, GRIDEXPR, ...)                                                           \
+    do {                                                                                             \
+        if (op.physics == PHY_ELASTIC) KERNEL<T, 3><<<(GRIDEXPR), BS, 0, s>>>(__VA_ARGS__);          \
+        else KERNEL<T, 1><<<(GRIDEXPR), BS, 0, s>>>(__VA_ARGS__);                                    \
+    } while (0)
+
+template <typename T>
+void launch_kj_precond(const OpDesc &op, const T *v, T *out, const Reduce *red, hipStream_t s)
+{
+    const int gr = group_grid(op, sizeof(T));
+    KJ_DISPATCH(k_kj_precond, gr, op.keys, op.kdiag, op.nmat, v, out, (size_t)op.N, *red);
+    launch_reduce_finish(*red, gr, 1, s);
+}
+
+template <typename T>
+void launch_kj_pcg_dir(const OpDesc &op, T *d, const T *r, const double *S, hipStream_t s)
+{
+    KJ_DISPATCH(k_kj_pcg_dir, group_grid(op, sizeof(T)), d, r, op.keys, op.kdiag, op.nmat, S, (size_t)op.N);
+}
+
+template <typename T>
+void launch_kj_pcg_update(const OpDesc &op, T *x, T *r, const T *d, const T *q, const Reduce *red, hipStream_t s)
+{
+    const int gr = group_grid(op, sizeof(T));
+    KJ_DISPATCH(k_kj_pcg_update, gr, x, r, d, q, op.keys, op.kdiag, op.nmat, (size_t)op.N, *red);
+    launch_reduce_finish(*red, gr, 1, s);
+}
+
+template <typename T>
+void launch_kj_sub_precond(const OpDesc &op, T *r, const T *q, T *z, const Reduce *red, hipStream_t s)
+{
+    const int gr = group_grid(op, sizeof(T));
+    KJ_DISPATCH(k_kj_sub_precond, gr, r, q, z, op.keys, op.kdiag, op.nmat, (size_t)op.N, *red);
+    launch_reduce_finish(*red, gr, 1, s);
+}
+
+template <typename T>
+void launch_add(T *y, const T *x, size_t n, hipStream_t s)
+{
+    k_add<T><<<grid_for(n), BS, 0, s>>>(y, x, n);
+}
+
 void launch_add_correction(double *x, const float *e, size_t n, hipStream_t s)
 {
     k_add_correction<<<grid_for(n), BS, 0, s>>>(x, e, n);
@@ -991,6 +1277,7 @@ void launch_interpl2(const double *coarse, double *fine, int rows, int cols, int
     template void launch_apply<T>(const OpDesc &, const T *, T *, const Reduce *, hipStream_t);               \
     template void launch_jacobi<T>(const OpDesc &, T *, hipStream_t);                                         \
     template void launch_fill<T>(T *, double, size_t, hipStream_t);                                            \
+    template void launch_add<T>(T *, const T *, size_t, hipStream_t);                                         \
     template void launch_precond<T>(const T *, const T *, T *, size_t, const Reduce *, hipStream_t);          \
     template void launch_pcg_dir<T>(T *, const T *, const double *, int, size_t, hipStream_t);                \
     template void launch_pcg_update<T>(T *, T *, const T *, T *, const T *, const Reduce *, size_t, hipStream_t); \
@@ -1000,7 +1287,12 @@ void launch_interpl2(const double *coarse, double *fine, int rows, int cols, int
     template void launch_minres_update<T>(T *, T *, const T *, const T *, const T *, const Reduce *, size_t,  \
                                           hipStream_t);                                                       \
     template void launch_dot<T>(const T *, const T *, size_t, const Reduce *, hipStream_t);                   \
-    template void launch_fluid_velocity_sum<T>(const OpDesc &, const T *, const Reduce *, hipStream_t);
+    template void launch_fluid_velocity_sum<T>(const OpDesc &, const T *, const Reduce *, hipStream_t);       \
+    template void launch_kj_precond<T>(const OpDesc &, const T *, T *, const Reduce *, hipStream_t);          \
+    template void launch_kj_pcg_dir<T>(const OpDesc &, T *, const T *, const double *, hipStream_t);          \
+    template void launch_kj_pcg_update<T>(const OpDesc &, T *, T *, const T *, const T *, const Reduce *,     \
+                                          hipStream_t);                                                       \
+    template void launch_kj_sub_precond<T>(const OpDesc &, T *, const T *, T *, const Reduce *, hipStream_t);
 INST(float)
 INST(double)
 template void launch_convert<double, float>(const double *, float *, size_t, hipStream_t);

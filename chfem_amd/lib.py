@@ -20,7 +20,7 @@ cp = C.c_char_p
 
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_uint), ("sweeps", C.c_uint), ("solve_seconds", C.c_double),
-                ("residual", C.c_double)]
+                ("residual", C.c_double), ("x0_residual", C.c_double), ("x0_used", C.c_uint)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/*.h
@@ -110,6 +110,20 @@ SIGNATURES = [
     ("hmgGetKernelTiming", u8, [C.POINTER(dbl), C.POINTER(uint)]),
     ("chfemCopyKernelBench", dbl, [C.c_size_t, uint]),
     ("cudapcgOperatorBench", dbl, [u8, uint]),
+    ("hmgSolveLoadCase", u8, [uint, vp, vp, vp]),
+    ("chfemGuessCreate", vp, [C.c_int, uint]),
+    ("chfemGuessDestroy", None, [vp]),
+    ("chfemGuessNumParams", C.c_size_t, [vp]),
+    ("chfemGuessNumOutputs", uint, [vp]),
+    ("chfemGuessGetParams", C.c_int, [vp, vp]),
+    ("chfemGuessSetParams", C.c_int, [vp, vp]),
+    ("chfemGuessGetGradients", C.c_int, [vp, vp]),
+    ("chfemGuessZeroGradients", C.c_int, [vp]),
+    ("chfemGuessPredict", C.c_int, [vp, vp, vp, uint, uint, uint, uint, vp]),
+    ("chfemGuessAccumulate", C.c_int, [vp, vp, vp, uint, uint, uint, uint, vp, vp]),
+    ("chfemGuessStep", C.c_int, [vp, C.c_float, C.c_float]),
+    ("hmgSetInitialGuessNetwork", C.c_int, [vp]),
+    ("chfemGuessMfmaCheck", C.c_int, [vp, vp, vp]),
 ]
 
 

@@ -36,14 +36,19 @@ int chfem_run(const uint8_t *img, size_t img_size, const char *nf_file, int anal
 cudapcgFlag_t cudapcgSetPrecision(cudapcgFlag_t flag);
 void hmgSetPrecisionFlag(cudapcgFlag_t flag);
 
-/* Statistics of the last cudapcgSolve: total Krylov iterations (inner
- * iterations summed over refinement sweeps), refinement sweeps, device time in
- * seconds of the solve. */
+/* Statistics of the last cudapcgSolve: total Krylov iterations, refinement
+ * sweeps (mixed precision: FP64 residual replacements), device time in seconds
+ * of the solve, final relative residual sqrt(delta / delta0); for a solve
+ * given an initial guess, the guess's relative residual sqrt(delta(x0) /
+ * delta0) and whether it was kept (1) or replaced by the zero guess because
+ * its residual was larger (0). */
 typedef struct {
     unsigned int iterations;
     unsigned int sweeps;
     double solve_seconds;
     double residual;
+    double x0_residual;
+    unsigned int x0_used;
 } chfemSolveStats_t;
 cudapcgFlag_t cudapcgGetSolveStats(chfemSolveStats_t *stats);
 
@@ -85,6 +90,48 @@ void hmgSetMaxIterationsOverride(unsigned int maxit);
 logical hmgPrepare(void);
 logical hmgSolveResident(var *C);
 void hmgRelease(void);
+/* One load case against the resident model: device RHS of direction dir,
+ * solve from x0 (host, NULL: zero guess), solution copied to x (host, ndof
+ * entries, nullable) and the effective-tensor column to Ccol (3 or 6 entries,
+ * nullable).  Returns 1 when the solver converged. */
+logical hmgSolveLoadCase(unsigned int dir, const var *x0, var *x, var *Ccol);
+
+/* Learned initial guess (no reference counterpart; the reference seeds PCG
+ * with zeros, a coarse-grid solution or a guess read from file,
+ * chfem/src/femhmg/femhmg.c:1053-1084).  A periodic 3D convolutional
+ * encoder-decoder (about 332k parameters, BF16 matrix-core kernels, FP32
+ * master weights) maps the material properties of a voxel image to the
+ * fluctuation field of every load case (elastic 6 x 3 outputs, thermal 3).
+ * Grids must be multiples of 8 in every direction.
+ *   chfemGuessCreate(physics 0 thermal / 1 elastic, seed) -> network or NULL
+ *   chfemGuessPredict: out[v * nout + k], v = row + col*ny + layer*nx*ny,
+ *     k = load * ndof_per_node + component (nout = 18 elastic, 3 thermal)
+ *   chfemGuessAccumulate: forward + backward of the mean squared error to
+ *     target (same layout), gradients summed into the network; *loss = MSE
+ *   chfemGuessStep: Adam step (lr, gradients scaled by grad_scale), then
+ *     zeroes the gradients
+ *   parameters: FP32, per layer W[Co][taps][Ci] then b[Co]
+ *   hmgSetInitialGuessNetwork: subsequent thermal/elastic homogenizations
+ *     seed every load case with ramp + prediction (NULL: zero guess).  A
+ *     guess whose initial residual exceeds the zero guess's is replaced by
+ *     the zero guess (chfemSolveStats_t.x0_used = 0). */
+void *chfemGuessCreate(int physics, unsigned int seed);
+void chfemGuessDestroy(void *net);
+size_t chfemGuessNumParams(void *net);
+unsigned int chfemGuessNumOutputs(void *net);
+int chfemGuessGetParams(void *net, float *params);
+int chfemGuessSetParams(void *net, const float *params);
+int chfemGuessGetGradients(void *net, float *grad);
+int chfemGuessZeroGradients(void *net);
+int chfemGuessPredict(void *net, const uint16_t *elem, const double *props, unsigned int nmat, unsigned int nx,
+                      unsigned int ny, unsigned int nz, float *out);
+int chfemGuessAccumulate(void *net, const uint16_t *elem, const double *props, unsigned int nmat, unsigned int nx,
+                         unsigned int ny, unsigned int nz, const float *target, double *loss);
+int chfemGuessStep(void *net, float lr, float grad_scale);
+int hmgSetInitialGuessNetwork(void *net);
+/* BF16 matrix-core lane-layout check: D = A * B for 16x16 row-major BF16
+ * A and B, FP32 D. */
+int chfemGuessMfmaCheck(const uint16_t *A, const uint16_t *B, float *D);
 
 /* Operator-launch timing inside the Krylov loops: with timing on, every
  * operator launch of the inner solve is bracketed with HIP events on the

@@ -192,3 +192,32 @@ def test_walsh_solve_multiblock(precision, monkeypatch):
             assert abs(int(st.iterations) - itr) <= 2
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("vector", ["keys", "stored"])
+@pytest.mark.parametrize("which", ["thermal", "elastic"])
+def test_pcg_vector_kernels(which, vector, monkeypatch):
+    """Both PCG vector-kernel forms (Jacobi diagonal rebuilt from the keys, or
+    a stored dinv vector) on a grid whose node count is not a multiple of 4
+    (tail nodes), in FP64 and mixed precision, against the oracle."""
+    if vector == "stored":
+        monkeypatch.setenv("CHFEM_VECTOR", "stored")
+    img = three_phase((5, 7, 9), 41)
+    if which == "thermal":
+        m = H.model(H.THERMAL, img, [1.0, 10.0, 3.5], colors=(0, 100, 255))
+    else:
+        m = H.model(H.ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25], colors=(0, 100, 255))
+    assert m.N % 4 == 3
+    d = m.cdim - 1
+    xr, okr, itr, _ = m.solve(d, tol=1e-10)
+    for precision in (0, 1):
+        s = H.GpuSolver(m, precision=precision, tol=1e-10 if precision == 0 else 1e-9)
+        try:
+            x, ok, st = s.solve(m.rhs(d))
+            assert ok and okr
+            assert rel(m.update_c(d, x), m.update_c(d, xr)) < 1e-7
+            if precision == 0:
+                assert rel(x, xr) < 1e-7
+                assert abs(int(st.iterations) - itr) <= 2
+        finally:
+            s.close()
