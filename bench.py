@@ -7,12 +7,17 @@ hexahedra, periodic boundaries, all 6 load cases.  A step is one full
 homogenization (6 x [RHS on the device -> Jacobi-preconditioned Krylov solve ->
 C column on the device]) with the model already resident in HBM.
 
-Methods timed in the same run:
-  plain : FP64 Jacobi-PCG (the reference method) at --tol
-  mixed : FP32 operator + inner PCG, FP64 dots/alpha/beta, FP64 outer residual
-          and iterative refinement, at --tol   (reported as `value`)
-Both are checked against C_ref, FP64 PCG at tolerance 1e-9 on the same
-image (untimed).  The operator kernel of the timed method is timed with HIP
+Methods timed in the same run, all at --tol and zero initial guess unless said:
+  plain   : FP64 Jacobi-PCG (the reference method)
+  mixed   : FP32 operator and CG vectors, FP64 dots/alpha/beta, FP64 residual
+            replacement (reliable updates)
+  learned : mixed, seeded by the learned initial guess (ramp + network
+            prediction of every load case's fluctuation field; the network
+            is trained before the timed region on --train-samples seeded
+            64^3 images of the same recipe, solved by the library)
+            (reported as `value`)
+Each is checked against C_ref, FP64 PCG at tolerance 1e-9 on the same image
+(untimed).  The operator kernel of the timed method is timed with HIP
 events on the solver's stream (roofline); the CPU FP64 oracle is timed on a
 bounded sample of the same workload (cpu_baseline).
 """
@@ -49,9 +54,10 @@ def hip_sync():
         pass
 
 
-def run_method(L, precision, tol, steps, warmup, label):
+def run_method(L, precision, tol, steps, warmup, label, net=None):
     """Prepares the resident solver and times `steps` homogenizations."""
     from chfem_amd.lib import SolveStats
+    L.hmgSetInitialGuessNetwork(net.h if net is not None else None)
     L.hmgSetPrecisionFlag(precision)
     L.hmgSetSolverTolerance(tol)
     if L.hmgPrepare() != 1:
@@ -76,13 +82,38 @@ def run_method(L, precision, tol, steps, warmup, label):
         L.hmgGetSolveStats(st, C.byref(cnt))
         its = [st[i].iterations for i in range(cnt.value)]
         sweeps = [st[i].sweeps for i in range(cnt.value)]
+        tile = np.zeros(3, dtype=np.int32)
+        L.cudapcgGetOperatorTile(precision, tile.ctypes.data_as(C.c_void_p))
+        x0res = [st[i].x0_residual for i in range(cnt.value)]
+        x0used = [st[i].x0_used for i in range(cnt.value)]
         op_ms = C.c_double(0)
         op_n = C.c_uint(0)
         L.hmgGetKernelTiming(C.byref(op_ms), C.byref(op_n))
     finally:
         L.hmgRelease()
+        L.hmgSetInitialGuessNetwork(None)
     return {"seconds": float(np.mean(times)) if times else None, "times": times, "C": Cm.reshape(6, 6).copy(),
-            "iterations": its, "sweeps": sweeps, "op_ms_total": op_ms.value, "op_launches": op_n.value}
+            "iterations": its, "sweeps": sweeps, "op_ms_total": op_ms.value, "op_launches": op_n.value,
+            "x0_residual": x0res, "x0_used": x0used, "tile": "x".join(str(int(t)) for t in tile)}
+
+
+def train_guess(n_samples, size, epochs, seed):
+    """Learned initial guess trained on library solutions of seeded images of
+    the headline recipe (untimed)."""
+    from chfem_amd import guess
+    mp = {0: (1.0, 0.3), 255: (10.0, 0.2)}
+    t0 = time.perf_counter()
+    samples = []
+    for k in range(n_samples):
+        img = build_image(size, 1000 + seed + k)
+        samples.append(guess.solve_fluctuations(img, mp, "elastic", tol=1e-6))
+    t1 = time.perf_counter()
+    net = guess.LearnedGuess("elastic", seed=seed)
+    hist = net.train(samples, epochs=epochs, lr=2e-3, batch=4, log=lambda m: log(f"[bench] guess {m}"))
+    t2 = time.perf_counter()
+    info = {"samples": n_samples, "size": size, "epochs": epochs, "params": net.num_params,
+            "data_seconds": t1 - t0, "train_seconds": t2 - t1, "loss_first": hist[0], "loss_last": hist[-1]}
+    return net, info
 
 
 def cpu_baseline(img, n, gpu_iterations, iters_sample=3):
@@ -128,6 +159,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg (tuning sweeps)")
+    ap.add_argument("--train-samples", type=int, default=24)
+    ap.add_argument("--train-size", type=int, default=64)
+    ap.add_argument("--train-epochs", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X only; --gpus {args.gpus} is not supported (run with --gpus 1).",
@@ -141,6 +175,8 @@ def main():
     from chfem_amd import io as cio
     from chfem_amd.lib import lib, ptr
     L = lib()
+    log(f"[bench] training the learned initial guess ({args.train_samples} x {args.train_size}^3)")
+    net, net_info = train_guess(args.train_samples, args.train_size, args.train_epochs, args.seed)
     log(f"[bench] generating {n}^3 sphere image (seed {args.seed})")
     img = build_image(n, args.seed)
     tmpd = tempfile.mkdtemp(prefix="chfem_bench_")
@@ -160,7 +196,8 @@ def main():
         ref = run_method(L, 0, 1e-9, 1, 0, "reference")
         Cref = ref["C"]
         plain = run_method(L, 0, args.tol, args.steps, args.warmup, "plain-fp64")
-        fast = run_method(L, 1, args.tol, args.steps, args.warmup, "mixed")
+        mixed = run_method(L, 1, args.tol, args.steps, args.warmup, "mixed")
+        fast = run_method(L, 1, args.tol, args.steps, args.warmup, "learned+mixed", net=net)
     finally:
         L.hmgEnd()
 
@@ -190,18 +227,24 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"elastic_{n}^3_spheres_r6-12_f0.3_seed{args.seed}_E1-10_nu0.3-0.2_6loads",
-                   "method": "mixed (FP32 inner PCG, FP64 reductions + iterative refinement), Jacobi, NBN",
+                   "method": "learned initial guess + mixed (FP32 CG, FP64 reductions and reliable updates), "
+                             "Jacobi, NBN",
                    "solver_tolerance": args.tol, "reference": "FP64 PCG tol 1e-9"},
         "err_vs_fp64": err(fast["C"]),
         "plain_fp64": {"seconds": plain["seconds"], "err_vs_fp64": err(plain["C"]),
                        "iterations": plain["iterations"],
                        "op_avg_ms": plain["op_ms_total"] / max(plain["op_launches"], 1),
                        "op_GBps": 49.0 * N / (plain["op_ms_total"] / max(plain["op_launches"], 1) * 1e-3) / 1e9},
-        "mixed": {"seconds": fast["seconds"], "iterations": fast["iterations"], "sweeps": fast["sweeps"]},
+        "mixed": {"seconds": mixed["seconds"], "err_vs_fp64": err(mixed["C"]), "iterations": mixed["iterations"],
+                  "sweeps": mixed["sweeps"]},
+        "learned_mixed": {"seconds": fast["seconds"], "err_vs_fp64": err(fast["C"]), "iterations": fast["iterations"],
+                          "sweeps": fast["sweeps"], "x0_residual": fast["x0_residual"], "x0_used": fast["x0_used"],
+                          "network": net_info},
         "speedup_vs_plain": plain["seconds"] / fast["seconds"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_elastic_walsh<float,32,8,true> (FP32 elastic NBN apply + fused d.Ad)",
+                     "kernel": "k_elastic_walsh<float,%s,true> (FP32 elastic NBN apply + fused d.Ad), LZ %s"
+                               % (",".join(fast["tile"].split("x")[:2]), fast["tile"].split("x")[2]),
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": op_avg_ms,
                      "launches": fast["op_launches"]},
         "cpu_baseline": cpu,

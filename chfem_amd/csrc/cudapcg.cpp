@@ -15,6 +15,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <type_traits>
+
 #include "solver.h"
 #include "voxel.h"
 
@@ -54,6 +56,9 @@ struct Solver {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // Jacobi diagonal table (FP64) for the key-Jacobi vector kernels
     double *kdiag64 = nullptr;
+    // operator tiles chosen by the autotuner (rows, columns, layers per block)
+    bool autotune = false, tuned = false;
+    int tile64[3] = {32, 8, 16}, tile32[3] = {32, 8, 16};
     // isotropic elastic materials for the Walsh-basis operator
     double *lamu64 = nullptr;
     float *lamu32 = nullptr;
@@ -222,7 +227,9 @@ static int make_ops()
     o.tile_tr = 32;
     o.tile_tc = 8;
     o.tile_lz = 16;
-    if (const char *t = getenv("CHFEM_WALSH_TILE")) sscanf(t, "%dx%dx%d", &o.tile_tr, &o.tile_tc, &o.tile_lz);
+    const char *tile_env = getenv("CHFEM_WALSH_TILE");
+    const bool fixed_tile = tile_env && strcmp(tile_env, "auto") != 0;
+    if (fixed_tile) sscanf(tile_env, "%dx%dx%d", &o.tile_tr, &o.tile_tc, &o.tile_lz);
     const char *opsel = getenv("CHFEM_OPERATOR");
     const bool use_walsh = g->walsh_ok && !(opsel && !strcmp(opsel, "generic"));
     // key-Jacobi vector kernels: NBN thermal/elastic with the Jacobi
@@ -231,13 +238,61 @@ static int make_ops()
     const bool kj = g->kdiag64 && g->precond && !o.ebe && o.physics != PHY_FLUID && !(vsel && !strcmp(vsel, "stored"));
     o.kdiag = kj ? g->kdiag64 : nullptr;
     o.nmat = (int)g->dm.nmat;
+    const char *esel = getenv("CHFEM_EBE");
+    o.ebe_atomic = (esel && !strcmp(esel, "atomic")) || o.physics == PHY_FLUID;
     g->dm.op64 = o;
     g->dm.op64.K = g->dm.K64;
     g->dm.op64.lamu = use_walsh ? g->lamu64 : nullptr;
     g->dm.op32 = o;
     g->dm.op32.K = g->dm.K32;
     g->dm.op32.lamu = use_walsh ? g->lamu32 : nullptr;
+    if (!fixed_tile && g->tuned) {
+        // tiles picked by the autotuner for this model
+        g->dm.op64.tile_tr = g->tile64[0]; g->dm.op64.tile_tc = g->tile64[1]; g->dm.op64.tile_lz = g->tile64[2];
+        g->dm.op32.tile_tr = g->tile32[0]; g->dm.op32.tile_tc = g->tile32[1]; g->dm.op32.tile_lz = g->tile32[2];
+    }
+    g->autotune = !fixed_tile && use_walsh && !o.ebe;
     return 1;
+}
+
+// Operator tile autotuner: times every Walsh tile shape (rows x columns per
+// block, node layers per block) on the model's own grid in each precision the
+// solve uses and keeps the fastest.  Runs once per model, before its first
+// solve; CHFEM_WALSH_TILE=TRxTCxLZ fixes the tile instead.
+static void autotune_tiles()
+{
+    if (!g->autotune || g->tuned) return;
+    static const int cand[][3] = {{32, 8, 16}, {32, 8, 32}, {16, 16, 16}, {32, 16, 16}, {32, 16, 32}, {64, 4, 16}};
+    const int nc = (int)(sizeof(cand) / sizeof(cand[0]));
+    const size_t n = nvars();
+    auto tune = [&](OpDesc &op, auto *d, auto *q, int *best) {
+        using T = typename std::remove_pointer<decltype(d)>::type;
+        float best_ms = 1e30f;
+        for (int c = 0; c < nc; c++) {
+            op.tile_tr = cand[c][0]; op.tile_tc = cand[c][1]; op.tile_lz = cand[c][2];
+            launch_apply<T>(op, d, q, nullptr, g->stream);
+            hipEventRecord(g->ev0, g->stream);
+            for (int r = 0; r < 3; r++) launch_apply<T>(op, d, q, nullptr, g->stream);
+            hipEventRecord(g->ev1, g->stream);
+            if (hipEventSynchronize(g->ev1) != hipSuccess) return;
+            float ms = 0;
+            hipEventElapsedTime(&ms, g->ev0, g->ev1);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best[0] = cand[c][0]; best[1] = cand[c][1]; best[2] = cand[c][2];
+            }
+        }
+        op.tile_tr = best[0]; op.tile_tc = best[1]; op.tile_lz = best[2];
+    };
+    launch_fill<double>(g->d64, 1.0, n, g->stream);
+    tune(g->dm.op64, g->d64, g->q64, g->tile64);
+    if (g->precision == CHFEM_PREC_MIXED && g->d32) {
+        launch_fill<float>(g->d32, 1.0, n, g->stream);
+        tune(g->dm.op32, g->d32, g->q32, g->tile32);
+    } else {
+        memcpy(g->tile32, g->tile64, sizeof(g->tile32));
+    }
+    g->tuned = true;
 }
 
 static bool above(double delta, double delta0, double tol) { return fabs(delta) > tol * tol * fabs(delta0); }
@@ -566,6 +621,7 @@ int solve_device(const double *x0_dev)
     if (!g || !g->model) return -1;
     if (!g->arrays && alloc_solver_arrays() != CUDAPCG_TRUE) return -1;
     make_ops();
+    autotune_tiles();
     ensure_dinv();
     const size_t n = nvars();
     bool x0set = g->x0_set;
@@ -813,6 +869,7 @@ cudapcgFlag_t cudapcgSetModel(cudapcgModel_t *model)
         free(g->model);
     }
     g->model = model;
+    g->tuned = false;
     return alloc_model_device();
 }
 
@@ -997,6 +1054,7 @@ cudapcgFlag_t cudapcgSetLclMtxs(cudapcgVar_t *L)
     g->dinv_valid = false;
     // isotropic elastic materials: the operator can use the Walsh-basis kernel
     g->walsh_ok = false;
+    g->tuned = false;
     if (ph == PHY_ELASTIC && nk <= 4) {
         double lm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         float lmf[8];
@@ -1067,7 +1125,7 @@ cudapcgFlag_t cudapcgApplyPreconditioner(const cudapcgVar_t *v, cudapcgVar_t *z)
     return CUDAPCG_TRUE;
 }
 
-double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps)
+double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps, cudapcgFlag_t fused_dot)
 {
     if (!g || !g->model) return -1.0;
     if (!g->arrays && alloc_solver_arrays() != CUDAPCG_TRUE) return -1.0;
@@ -1077,20 +1135,39 @@ double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps)
     }
     make_ops();
     const size_t n = nvars();
+    // two inputs (d and x) so that consecutive launches do not re-read one
+    // cache-resident vector
     launch_fill<double>(g->d64, 1.0, n, g->stream);
-    if (precision == CHFEM_PREC_MIXED) launch_convert<double, float>(g->d64, g->d32, n, g->stream);
-    auto once = [&]() {
-        if (precision == CHFEM_PREC_MIXED) launch_apply<float>(g->dm.op32, g->d32, g->q32, nullptr, g->stream);
-        else launch_apply<double>(g->dm.op64, g->d64, g->q64, nullptr, g->stream);
+    launch_fill<double>(g->x64, 0.5, n, g->stream);
+    if (precision == CHFEM_PREC_MIXED) {
+        launch_convert<double, float>(g->d64, g->d32, n, g->stream);
+        launch_convert<double, float>(g->x64, g->x32, n, g->stream);
+    }
+    Reduce red = make_reduce(FIN_STORE, S_TMP0);
+    const Reduce *rp = fused_dot ? &red : nullptr;
+    auto once = [&](unsigned k) {
+        if (precision == CHFEM_PREC_MIXED) launch_apply<float>(g->dm.op32, k & 1 ? g->x32 : g->d32, g->q32, rp, g->stream);
+        else launch_apply<double>(g->dm.op64, k & 1 ? g->x64 : g->d64, g->q64, rp, g->stream);
     };
-    once();
+    once(0);
+    once(1);
     hipEventRecord(g->ev0, g->stream);
-    for (unsigned r = 0; r < reps; r++) once();
+    for (unsigned r = 0; r < reps; r++) once(r);
     hipEventRecord(g->ev1, g->stream);
     if (hipEventSynchronize(g->ev1) != hipSuccess) return -1.0;
     float ms = 0;
     hipEventElapsedTime(&ms, g->ev0, g->ev1);
     return reps ? ms / reps : 0.0;
+}
+
+cudapcgFlag_t cudapcgGetOperatorTile(cudapcgFlag_t precision, int *tile)
+{
+    if (!g || !g->model || !tile) return CUDAPCG_FALSE;
+    const OpDesc &o = precision == CHFEM_PREC_MIXED ? g->dm.op32 : g->dm.op64;
+    tile[0] = o.tile_tr;
+    tile[1] = o.tile_tc;
+    tile[2] = o.tile_lz;
+    return CUDAPCG_TRUE;
 }
 
 cudapcgFlag_t cudapcgSetHeaderString(char *header)

@@ -1,0 +1,308 @@
+// ebe_mfma.hip -- element-by-element operator q = A v for thermal (8x8) and
+// elastic (24x24) voxel elements on the matrix cores, for any dense local
+// matrices of up to 4 materials (chfem/src/femhmg/cudapcg/kernels/
+// matrix_free.cu:1097-1335 computes the same products one element per
+// thread and scatters them with atomics).
+//
+// Per element layer a block of TR x TC work-items (4 wavefronts for 32x8)
+// owns TR x TC elements.  Each wavefront multiplies the stacked local
+// matrices [K_0 | K_1 | ...] (KE x KE*nmat) by the material-masked element
+// vectors of its 64 elements (KE*nmat x 64) with v_mfma_{f32,f64}_16x16x4:
+// row tiles of 16 local DOFs, column tiles of 16 elements, 4 stacked DOFs per
+// step; material blocks absent from the wavefront's elements are skipped.
+// Element forces land in LDS and every interior node sums its 8 element
+// contributions there, as in the Walsh-basis kernel (walsh.hip): tiles
+// overlap by one node row/column, nothing is scattered with atomics, and the
+// result is deterministic.
+#include <hip/hip_runtime.h>
+#include "kernels.h"
+#include "reduce.h"
+
+namespace chfem {
+
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+template <typename T> struct MfmaAcc;
+template <> struct MfmaAcc<float> {
+    typedef f4_t type;
+    static __device__ __forceinline__ type mac(float a, float b, type c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct MfmaAcc<double> {
+    typedef d4_t type;
+    static __device__ __forceinline__ type mac(double a, double b, type c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+#define EDR(k) ((k) == 0 || (k) == 1 || (k) == 4 || (k) == 5 ? 1 : 0)
+#define EDC(k) ((k) == 1 || (k) == 2 || (k) == 5 || (k) == 6 ? 1 : 0)
+#define EDL(k) ((k) >= 4 ? 1 : 0)
+
+__device__ __forceinline__ int ewrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+__device__ __forceinline__ int edr(int k) { return (0x33 >> k) & 1; }
+__device__ __forceinline__ int edc(int k) { return (0x66 >> k) & 1; }
+__device__ __forceinline__ int edl(int k) { return k >> 2; }
+
+template <typename T, int ND, int TR, int TC, bool DOT>
+__global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T *__restrict__ q,
+                                                      const uint16_t *__restrict__ keys, const T *__restrict__ K,
+                                                      int nmat, int ebe, int ny, int nx, int nz, int LZ, Reduce red)
+{
+    static_assert(TR * TC == 256, "four wavefronts per block");
+    constexpr int KE = 8 * ND;                 // local DOFs
+    constexpr int MT = (KE + 15) / 16;         // row tiles
+    constexpr int NT = TR * TC, NR = TR + 1, NC = TC + 1, NV = NR * NC * ND;
+    constexpr int SLOTS = (NV + NT - 1) / NT;
+    constexpr int SF = KE + 1;                 // padded element stride of the force buffer
+    __shared__ T su[2][NV];
+    __shared__ T sf[NT * SF];
+    __shared__ T sK[4 * KE * KE];              // K[m][row][col]
+    __shared__ int smat[NT];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6, li = lane & 15, kg = lane >> 4;
+    const int tr = tid % TR, tc = tid / TR;
+    const int r0 = blockIdx.x * (TR - 1), c0 = blockIdx.y * (TC - 1), l0 = blockIdx.z * LZ;
+    const int vr = min(TR - 1, ny - r0), vc = min(TC - 1, nx - c0), vl = min(LZ, nz - l0);
+    const size_t nxy = (size_t)nx * ny;
+    for (int i = tid; i < nmat * KE * KE; i += NT) sK[i] = K[i];
+
+    uint32_t goff[SLOTS];
+    bool gval[SLOTS];
+#pragma unroll
+    for (int k = 0; k < SLOTS; k++) {
+        const int idx = tid + k * NT;
+        const int comp = idx % ND, t2 = idx / ND, nr = t2 % NR, nc = t2 / NR;
+        gval[k] = idx < NV && nr <= vr + 1 && nc <= vc + 1;
+        const int gr = ewrap(r0 - 1 + nr, ny), gc = ewrap(c0 - 1 + nc, nx);
+        goff[k] = gval[k] ? (uint32_t)ND * ((uint32_t)gc * (uint32_t)ny + (uint32_t)gr) + (uint32_t)comp : 0u;
+    }
+    T pre[SLOTS];
+    auto fetch = [&](int lg) {
+        const T *base = v + (size_t)ND * (size_t)ewrap(lg, nz) * nxy;
+#pragma unroll
+        for (int k = 0; k < SLOTS; k++) pre[k] = gval[k] ? base[goff[k]] : T(0);
+    };
+    auto store = [&](T *dst) {
+#pragma unroll
+        for (int k = 0; k < SLOTS; k++)
+            if (gval[k]) dst[tid + k * NT] = pre[k];
+    };
+    const bool elem_ok = tr <= vr && tc <= vc;
+    const uint32_t eoff = (uint32_t)ewrap(c0 - 1 + tc, nx) * (uint32_t)ny + (uint32_t)ewrap(r0 - 1 + tr, ny);
+    // material of this work-item's element in layer le; -1: outside the patch
+    auto material = [&](int le) -> int {
+        if (!elem_ok) return -1;
+        const unsigned kv = keys[(size_t)ewrap(le, nz) * nxy + eoff];
+        return ebe ? (int)(kv & 3u) : (int)((kv >> 6) & 3u);
+    };
+
+    // element forces of the wavefront's 64 elements (block elements 64*wave ..)
+    auto elements = [&](const T *cur, const T *nxt) {
+        // materials present in this wavefront
+        const int me = smat[tid];
+        unsigned present = 0;
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+            if (__ballot(me == m)) present |= 1u << m;
+        typename MfmaAcc<T>::type acc[MT][4];
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) acc[mt][nt] = typename MfmaAcc<T>::type{0, 0, 0, 0};
+        // the 4 elements this lane feeds as B columns: block element 64*wave + 16*nt + li
+        int ebase[4], emat[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            const int e = 64 * wave + 16 * nt + li;
+            ebase[nt] = ((e / TR) * NR + (e % TR)) * ND;
+            emat[nt] = smat[e];
+        }
+        for (int m = 0; m < 4; m++) {
+            if (!((present >> m) & 1u)) continue;
+            const T *Km = sK + m * KE * KE;
+#pragma unroll
+            for (int s = 0; s < KE / 4; s++) {
+                const int d = 4 * s + kg;           // stacked DOF of this lane in the step
+                const int kn = d / ND, comp = d % ND;
+                const T *pl = edl(kn) ? nxt : cur;
+                const int doff = (edc(kn) * NR + edr(kn)) * ND + comp;
+                T b[4];
+#pragma unroll
+                for (int nt = 0; nt < 4; nt++) b[nt] = emat[nt] == m ? pl[ebase[nt] + doff] : T(0);
+#pragma unroll
+                for (int mt = 0; mt < MT; mt++) {
+                    const int row = 16 * mt + li;
+                    const T a = row < KE ? Km[row * KE + d] : T(0);
+#pragma unroll
+                    for (int nt = 0; nt < 4; nt++) acc[mt][nt] = MfmaAcc<T>::mac(a, b[nt], acc[mt][nt]);
+                }
+            }
+        }
+        // D[row][col]: this lane holds column li (element), rows 4*kg + i of each row tile
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) {
+                T *dst = sf + (64 * wave + 16 * nt + li) * SF;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int row = 16 * mt + 4 * kg + i;
+                    if (row < KE) dst[row] = acc[mt][nt][i];
+                }
+            }
+    };
+    const bool own = tr >= 1 && tc >= 1 && tr - 1 < vr && tc - 1 < vc;
+    auto fslot = [&](int k) { return ((tc - EDC(k)) * TR + (tr - EDR(k))) * SF + ND * k; };
+
+    T *cur = su[0], *nxt = su[1];
+    fetch(l0 - 1);
+    store(cur);
+    fetch(l0);
+    store(nxt);
+    smat[tid] = material(l0 - 1);
+    __syncthreads();
+    fetch(l0 + 1);
+    elements(cur, nxt);
+    const int mnext = material(l0);
+    __syncthreads();
+    smat[tid] = mnext;
+    T acc[ND];
+#pragma unroll
+    for (int a = 0; a < ND; a++) acc[a] = 0;
+    if (own) {
+#pragma unroll
+        for (int k = 4; k < 8; k++) {
+            const T *sp = sf + fslot(k);
+#pragma unroll
+            for (int a = 0; a < ND; a++) acc[a] += sp[a];
+        }
+    }
+    store(cur);
+    {
+        T *t = cur; cur = nxt; nxt = t;
+    }
+    __syncthreads();
+    double dsum = 0.0;
+    for (int s = 0; s < vl; s++) {
+        const int l = l0 + s;
+        const bool more = s + 1 < vl;
+        if (more) fetch(l + 2);
+        T own_u[ND];
+#pragma unroll
+        for (int a = 0; a < ND; a++) own_u[a] = 0;
+        if (DOT && own) {
+            const T *uu = cur + (tc * NR + tr) * ND;
+#pragma unroll
+            for (int a = 0; a < ND; a++) own_u[a] = uu[a];
+        }
+        elements(cur, nxt);
+        const int mn = more ? material(l + 1) : -1;
+        __syncthreads();
+        if (own) {
+            T o[ND];
+#pragma unroll
+            for (int a = 0; a < ND; a++) o[a] = acc[a];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const T *sp = sf + fslot(k);
+#pragma unroll
+                for (int a = 0; a < ND; a++) o[a] += sp[a];
+            }
+#pragma unroll
+            for (int a = 0; a < ND; a++) acc[a] = 0;
+#pragma unroll
+            for (int k = 4; k < 8; k++) {
+                const T *sp = sf + fslot(k);
+#pragma unroll
+                for (int a = 0; a < ND; a++) acc[a] += sp[a];
+            }
+            T *qo = q + ND * ((size_t)l * nxy + (size_t)(c0 + tc - 1) * ny + (r0 + tr - 1));
+#pragma unroll
+            for (int a = 0; a < ND; a++) {
+                qo[a] = o[a];
+                if (DOT) dsum += (double)own_u[a] * (double)o[a];
+            }
+        }
+        if (more) {
+            store(cur);
+            T *t = cur; cur = nxt; nxt = t;
+            smat[tid] = mn;
+        }
+        __syncthreads();
+    }
+    if (DOT) {
+        double vv[1] = {dsum};
+        store_partials<1>(red, vv);
+    }
+}
+
+template <typename T>
+void launch_ebe_mfma(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s)
+{
+    constexpr int TR = 32, TC = 8;
+    Reduce rr = {nullptr, nullptr, 0, 0};
+    if (red) rr = *red;
+    int LZ = 16;
+    const long cols = (long)((op.ny + TR - 2) / (TR - 1)) * ((op.nx + TC - 2) / (TC - 1));
+    while (cols * ((op.nz + LZ - 1) / LZ) > MAX_PARTIAL_BLOCKS) LZ *= 2;
+    dim3 grid((op.ny + TR - 2) / (TR - 1), (op.nx + TC - 2) / (TC - 1), (op.nz + LZ - 1) / LZ);
+    const T *K = (const T *)op.K;
+    if (op.physics == PHY_ELASTIC) {
+        if (red) k_ebe_mfma<T, 3, TR, TC, true><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr);
+        else k_ebe_mfma<T, 3, TR, TC, false><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr);
+    } else {
+        if (red) k_ebe_mfma<T, 1, TR, TC, true><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr);
+        else k_ebe_mfma<T, 1, TR, TC, false><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr);
+    }
+    if (red) launch_reduce_finish(rr, grid.x * grid.y * grid.z, 1, s);
+}
+
+template void launch_ebe_mfma<float>(const OpDesc &, const float *, float *, const Reduce *, hipStream_t);
+template void launch_ebe_mfma<double>(const OpDesc &, const double *, double *, const Reduce *, hipStream_t);
+
+// lane-layout check of the FP32 / FP64 16x16x4 matrix-core products used
+// above: D = A (16x4) * B (4x16), row-major inputs and output
+__global__ void k_mfma_check_f(const float *A, const float *B, float *D, const double *A2, const double *B2, double *D2)
+{
+    const int l = threadIdx.x, li = l & 15, kg = l >> 4;
+    f4_t d = __builtin_amdgcn_mfma_f32_16x16x4f32(A[li * 4 + kg], B[kg * 16 + li], f4_t{0, 0, 0, 0}, 0, 0, 0);
+    d4_t d2 = __builtin_amdgcn_mfma_f64_16x16x4f64(A2[li * 4 + kg], B2[kg * 16 + li], d4_t{0, 0, 0, 0}, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        D[(4 * kg + i) * 16 + li] = d[i];
+        D2[(4 * kg + i) * 16 + li] = d2[i];
+    }
+}
+
+}  // namespace chfem
+
+extern "C" int chfemMfmaCheckF(const float *A, const float *B, float *D, const double *A2, const double *B2,
+                               double *D2)
+{
+    float *dA = nullptr, *dB = nullptr, *dD = nullptr;
+    double *dA2 = nullptr, *dB2 = nullptr, *dD2 = nullptr;
+    int ok = hipMalloc(&dA, 64 * 4) == hipSuccess && hipMalloc(&dB, 64 * 4) == hipSuccess &&
+             hipMalloc(&dD, 256 * 4) == hipSuccess && hipMalloc(&dA2, 64 * 8) == hipSuccess &&
+             hipMalloc(&dB2, 64 * 8) == hipSuccess && hipMalloc(&dD2, 256 * 8) == hipSuccess;
+    ok = ok && hipMemcpy(dA, A, 64 * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(dB, B, 64 * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(dA2, A2, 64 * 8, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(dB2, B2, 64 * 8, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        chfem::k_mfma_check_f<<<1, 64>>>(dA, dB, dD, dA2, dB2, dD2);
+        ok = hipDeviceSynchronize() == hipSuccess && hipMemcpy(D, dD, 256 * 4, hipMemcpyDeviceToHost) == hipSuccess &&
+             hipMemcpy(D2, dD2, 256 * 8, hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (dA) (void)hipFree(dA);
+    if (dB) (void)hipFree(dB);
+    if (dD) (void)hipFree(dD);
+    if (dA2) (void)hipFree(dA2);
+    if (dB2) (void)hipFree(dB2);
+    if (dD2) (void)hipFree(dD2);
+    return ok;
+}

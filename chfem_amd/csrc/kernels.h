@@ -56,9 +56,15 @@ struct OpDesc {
     // selects the key-Jacobi vector kernels (thermal/elastic NBN)
     const double *kdiag;
     int nmat;
+    // EBE: 1 selects the reference-style element kernel with atomics instead
+    // of the matrix-core kernel (ebe_mfma.hip; up to 4 materials)
+    int ebe_atomic;
 };
 
 template <typename T> void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s);
+
+// EBE operator on the matrix cores (ebe_mfma.hip), thermal/elastic, nmat <= 4
+template <typename T> void launch_ebe_mfma(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s);
 
 // Sums the nv x nb partials a reducing launch of nb blocks left in
 // red.partials (fixed order) and publishes them as red.fin says.  Every

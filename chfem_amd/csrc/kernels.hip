@@ -1021,14 +1021,14 @@ __global__ void k_interp(double *v, uint32_t nx, uint32_t ny, uint32_t nz, int s
 static inline int grid_for(size_t n) { return reduce_blocks(n); }
 
 // Streaming vector kernels: one contiguous range of 16-byte vectors per
-// block, at most stream_cap() blocks (env CHFEM_STREAM_GRID, default 2048 =
-// 8 blocks of 4 wavefronts per CU on 256 CUs).
+// block, at most stream_cap() blocks (env CHFEM_STREAM_GRID, default 8192:
+// measured 3 % faster than 2048 on the 256^3 elastic solve).
 static int stream_cap()
 {
     static int cap = 0;
     if (!cap) {
         const char *e = getenv("CHFEM_STREAM_GRID");
-        cap = e ? atoi(e) : 2048;
+        cap = e ? atoi(e) : 8192;
         if (cap < 1) cap = 1;
         if (cap > MAX_RED_GRID) cap = MAX_RED_GRID;
     }
@@ -1093,6 +1093,10 @@ void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStre
         const uint32_t nt = op.nvel + op.nborder;
         if (red) LAUNCH_RED(1, (k_fluid<T, true>), grid_for(nt), s, rr, v, q, op.dof2node, op.node2dof, op.bkeys, K, op.ny, op.nx, op.nz, op.nvel, op.nborder, rr);
         else k_fluid<T, false><<<grid_for(nt), BS, 0, s>>>(v, q, op.dof2node, op.node2dof, op.bkeys, K, op.ny, op.nx, op.nz, op.nvel, op.nborder, rr);
+        return;
+    }
+    if (op.ebe && op.nmat <= 4 && !op.ebe_atomic) {
+        launch_ebe_mfma<T>(op, v, q, red, s);
         return;
     }
     if (op.physics == PHY_ELASTIC && op.lamu) {

@@ -109,7 +109,7 @@ SIGNATURES = [
     ("cudapcgGetKernelTiming", u8, [C.POINTER(dbl), C.POINTER(uint)]),
     ("hmgGetKernelTiming", u8, [C.POINTER(dbl), C.POINTER(uint)]),
     ("chfemCopyKernelBench", dbl, [C.c_size_t, uint]),
-    ("cudapcgOperatorBench", dbl, [u8, uint]),
+    ("cudapcgOperatorBench", dbl, [u8, uint, u8]),
     ("hmgSolveLoadCase", u8, [uint, vp, vp, vp]),
     ("chfemGuessCreate", vp, [C.c_int, uint]),
     ("chfemGuessDestroy", None, [vp]),
@@ -124,6 +124,8 @@ SIGNATURES = [
     ("chfemGuessStep", C.c_int, [vp, C.c_float, C.c_float]),
     ("hmgSetInitialGuessNetwork", C.c_int, [vp]),
     ("chfemGuessMfmaCheck", C.c_int, [vp, vp, vp]),
+    ("chfemMfmaCheckF", C.c_int, [vp, vp, vp, vp, vp, vp]),
+    ("cudapcgGetOperatorTile", u8, [u8, vp]),
 ]
 
 

@@ -26,6 +26,7 @@ def main():
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--strategy", type=int, default=0)
     ap.add_argument("--copy", action="store_true", help="also time the calibration copy kernel")
+    ap.add_argument("--dot", action="store_true", help="with the fused v.Av reduction, as inside the solver")
     a = ap.parse_args()
     n = a.size
     img = synthetic.spheres((n, n, n), rmin=6, rmax=12, fraction=0.3, seed=3)
@@ -39,9 +40,9 @@ def main():
     L.hmgSetParallelStrategyFlag(a.strategy)
     L.hmgSetPrecisionFlag(prec)
     assert L.hmgPrepare() == 1
-    ms = L.cudapcgOperatorBench(prec, a.reps)
+    ms = L.cudapcgOperatorBench(prec, a.reps, 1 if a.dot else 0)
     N = n ** 3
-    out = {"physics": a.physics, "size": n, "precision": a.precision, "strategy": a.strategy,
+    out = {"physics": a.physics, "size": n, "precision": a.precision, "strategy": a.strategy, "fused_dot": a.dot,
            "avg_launch_ms": ms, "algorithmic_bytes": BYTES[(a.physics, prec)] * N,
            "GBps": BYTES[(a.physics, prec)] * N / (ms * 1e-3) / 1e9}
     L.hmgRelease()

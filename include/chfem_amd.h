@@ -132,20 +132,29 @@ int hmgSetInitialGuessNetwork(void *net);
 /* BF16 matrix-core lane-layout check: D = A * B for 16x16 row-major BF16
  * A and B, FP32 D. */
 int chfemGuessMfmaCheck(const uint16_t *A, const uint16_t *B, float *D);
+/* FP32 / FP64 16x16x4 matrix-core lane-layout check used by the EBE
+ * operator: D = A (16x4) * B (4x16), row-major. */
+int chfemMfmaCheckF(const float *A, const float *B, float *D, const double *A2, const double *B2, double *D2);
 
 /* Operator-launch timing inside the Krylov loops: with timing on, every
  * operator launch of the inner solve is bracketed with HIP events on the
  * solver's stream and the durations are summed. */
 cudapcgFlag_t cudapcgSetKernelTiming(cudapcgFlag_t on);
 cudapcgFlag_t cudapcgGetKernelTiming(double *op_ms_total, unsigned int *op_launches);
+/* Elastic operator tile in use for a precision (rows, columns and node
+ * layers per block), as fixed by CHFEM_WALSH_TILE or chosen by the
+ * autotuner before the first solve. */
+cudapcgFlag_t cudapcgGetOperatorTile(cudapcgFlag_t precision, int *tile);
 /* Same counters for the last hmgSolveResident / hmgSolveHomogenization. */
 cudapcgFlag_t hmgGetKernelTiming(double *op_ms_total, unsigned int *op_launches);
 
 /* Operator micro-benchmark on the current solver (after cudapcgAllocateArrays
- * or hmgPrepare): q = A d, reps launches, precision CHFEM_PREC_FP64 or
- * CHFEM_PREC_MIXED (FP32 operator); returns the mean launch time in ms from
- * HIP events on the solver's stream, or a negative value on failure. */
-double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps);
+ * or hmgPrepare): q = A v for reps launches alternating between two input
+ * vectors, precision CHFEM_PREC_FP64 or CHFEM_PREC_MIXED (FP32 operator),
+ * with the solver's fused v.Av reduction when fused_dot is set; returns the
+ * mean launch time in ms from HIP events on the solver's stream, or a
+ * negative value on failure. */
+double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps, cudapcgFlag_t fused_dot);
 
 /* Calibration kernel for hardware-counter traffic: a grid-stride FP64 copy of
  * n entries, run reps times on device buffers it allocates; returns the mean

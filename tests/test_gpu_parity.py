@@ -221,3 +221,48 @@ def test_pcg_vector_kernels(which, vector, monkeypatch):
                 assert abs(int(st.iterations) - itr) <= 2
         finally:
             s.close()
+
+
+def test_mfma_f32_f64_layout():
+    """Lane layout of the 16x16x4 FP32/FP64 matrix-core products the EBE
+    operator is built on."""
+    from chfem_amd.lib import lib, ptr
+    rng = np.random.default_rng(1)
+    A, B = rng.standard_normal((16, 4)), rng.standard_normal((4, 16))
+    Af, Bf = A.astype(np.float32), B.astype(np.float32)
+    D = np.zeros((16, 16), np.float32)
+    D2 = np.zeros((16, 16))
+    assert lib().chfemMfmaCheckF(ptr(Af), ptr(Bf), ptr(D), ptr(A), ptr(B), ptr(D2)) == 1
+    np.testing.assert_allclose(D, Af.astype(np.float64) @ Bf.astype(np.float64), rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(D2, A @ B, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("kernel", ["mfma", "atomic"])
+@pytest.mark.parametrize("which", ["thermal", "elastic"])
+@pytest.mark.parametrize("nphase", [1, 4])
+def test_ebe_kernels(kernel, which, nphase, monkeypatch):
+    """EBE operator on the matrix cores (default) and the reference-style
+    element kernel with atomics, with 1 and 4 materials on a grid of several
+    blocks in every direction, FP64 and FP32 against the oracle."""
+    if kernel == "atomic":
+        monkeypatch.setenv("CHFEM_EBE", "atomic")
+    rng = np.random.default_rng(51 + nphase)
+    colors = (0, 60, 120, 255)[:nphase]
+    img = rng.choice(np.array(colors, dtype=np.uint8), size=(20, 40, 35))
+    if which == "thermal":
+        m = H.model(H.THERMAL, img, [1.0, 10.0, 3.5, 0.5][:nphase], colors=colors)
+    else:
+        m = H.model(H.ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25, 2.0, 0.35][:2 * nphase], colors=colors)
+    v = rng.standard_normal(m.ndof)
+    q_ref = m.apply(v, strategy=1)
+    for precision, tol in ((0, 1e-12), (1, 2e-5)):
+        s = H.GpuSolver(m, strategy=1, precision=precision)
+        try:
+            if precision == 0:
+                assert rel(s.apply(v), q_ref) < tol
+            else:
+                b = m.rhs(0)
+                x, ok, st = s.solve(b)
+                assert ok
+        finally:
+            s.close()
