@@ -24,12 +24,17 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
 template <typename T> struct MfmaAcc;
+// Operand layout of both products: lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16].  Result item i of lane l holds D[row(l / 16, i)][l % 16]
+// with row = 4 (l/16) + i for FP32 and (l/16) + 4 i for FP64 (checked by
+// chfemMfmaCheckF).
 template <> struct MfmaAcc<float> {
     typedef f4_t type;
     static __device__ __forceinline__ type mac(float a, float b, type c)
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int kg, int i) { return 4 * kg + i; }
 };
 template <> struct MfmaAcc<double> {
     typedef d4_t type;
@@ -37,6 +42,7 @@ template <> struct MfmaAcc<double> {
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int kg, int i) { return kg + 4 * i; }
 };
 
 #define EDR(k) ((k) == 0 || (k) == 1 || (k) == 4 || (k) == 5 ? 1 : 0)
@@ -143,7 +149,7 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
                 }
             }
         }
-        // D[row][col]: this lane holds column li (element), rows 4*kg + i of each row tile
+        // D[row][col]: this lane holds column li (element) and 4 rows of each row tile
 #pragma unroll
         for (int mt = 0; mt < MT; mt++)
 #pragma unroll
@@ -151,7 +157,7 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
                 T *dst = sf + (64 * wave + 16 * nt + li) * SF;
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    const int row = 16 * mt + 4 * kg + i;
+                    const int row = 16 * mt + MfmaAcc<T>::row(kg, i);
                     if (row < KE) dst[row] = acc[mt][nt][i];
                 }
             }
@@ -274,8 +280,8 @@ __global__ void k_mfma_check_f(const float *A, const float *B, float *D, const d
     d4_t d2 = __builtin_amdgcn_mfma_f64_16x16x4f64(A2[li * 4 + kg], B2[kg * 16 + li], d4_t{0, 0, 0, 0}, 0, 0, 0);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        D[(4 * kg + i) * 16 + li] = d[i];
-        D2[(4 * kg + i) * 16 + li] = d2[i];
+        D[MfmaAcc<float>::row(kg, i) * 16 + li] = d[i];
+        D2[MfmaAcc<double>::row(kg, i) * 16 + li] = d2[i];
     }
 }
 
