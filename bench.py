@@ -162,6 +162,7 @@ def main():
     ap.add_argument("--train-samples", type=int, default=24)
     ap.add_argument("--train-size", type=int, default=64)
     ap.add_argument("--train-epochs", type=int, default=20)
+    ap.add_argument("--only-learned", action="store_true", help="time only the learned method (tuning sweeps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X only; --gpus {args.gpus} is not supported (run with --gpus 1).",
@@ -195,8 +196,10 @@ def main():
         log("[bench] reference C: FP64 PCG, tol 1e-9")
         ref = run_method(L, 0, 1e-9, 1, 0, "reference")
         Cref = ref["C"]
-        plain = run_method(L, 0, args.tol, args.steps, args.warmup, "plain-fp64")
-        mixed = run_method(L, 1, args.tol, args.steps, args.warmup, "mixed")
+        plain = mixed = None
+        if not args.only_learned:
+            plain = run_method(L, 0, args.tol, args.steps, args.warmup, "plain-fp64")
+            mixed = run_method(L, 1, args.tol, args.steps, args.warmup, "mixed")
         fast = run_method(L, 1, args.tol, args.steps, args.warmup, "learned+mixed", net=net)
     finally:
         L.hmgEnd()
@@ -210,7 +213,7 @@ def main():
     achieved = bytes_per_launch / (op_avg_ms * 1e-3) / 1e9
     traffic = read_traffic(f"elastic_walsh_fp32_{n}")
     cpu = None
-    if not args.skip_cpu:
+    if not args.skip_cpu and plain is not None:
         log(f"[bench] cpu baseline (oracle)")
         cpu = cpu_baseline(img, n, plain["iterations"])
     out = {
@@ -231,16 +234,17 @@ def main():
                              "Jacobi, NBN",
                    "solver_tolerance": args.tol, "reference": "FP64 PCG tol 1e-9"},
         "err_vs_fp64": err(fast["C"]),
-        "plain_fp64": {"seconds": plain["seconds"], "err_vs_fp64": err(plain["C"]),
-                       "iterations": plain["iterations"],
-                       "op_avg_ms": plain["op_ms_total"] / max(plain["op_launches"], 1),
-                       "op_GBps": 49.0 * N / (plain["op_ms_total"] / max(plain["op_launches"], 1) * 1e-3) / 1e9},
-        "mixed": {"seconds": mixed["seconds"], "err_vs_fp64": err(mixed["C"]), "iterations": mixed["iterations"],
-                  "sweeps": mixed["sweeps"]},
+        "plain_fp64": None if plain is None else {
+            "seconds": plain["seconds"], "err_vs_fp64": err(plain["C"]), "iterations": plain["iterations"],
+            "op_avg_ms": plain["op_ms_total"] / max(plain["op_launches"], 1),
+            "op_GBps": 49.0 * N / (plain["op_ms_total"] / max(plain["op_launches"], 1) * 1e-3) / 1e9},
+        "mixed": None if mixed is None else {
+            "seconds": mixed["seconds"], "err_vs_fp64": err(mixed["C"]), "iterations": mixed["iterations"],
+            "sweeps": mixed["sweeps"]},
         "learned_mixed": {"seconds": fast["seconds"], "err_vs_fp64": err(fast["C"]), "iterations": fast["iterations"],
                           "sweeps": fast["sweeps"], "x0_residual": fast["x0_residual"], "x0_used": fast["x0_used"],
                           "network": net_info},
-        "speedup_vs_plain": plain["seconds"] / fast["seconds"],
+        "speedup_vs_plain": None if plain is None else plain["seconds"] / fast["seconds"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_elastic_walsh<float,%s,true> (FP32 elastic NBN apply + fused d.Ad), LZ %s"

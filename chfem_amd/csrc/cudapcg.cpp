@@ -15,7 +15,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <type_traits>
+#include <vector>
 
 #include "solver.h"
 #include "voxel.h"
@@ -54,8 +56,10 @@ struct Solver {
     size_t partials_n = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // Jacobi diagonal table (FP64) for the key-Jacobi vector kernels
-    double *kdiag64 = nullptr;
+    // inverse Jacobi diagonal per packed key (65536 x nd) for the key-Jacobi
+    // vector kernels, FP64 and FP32
+    double *kinv64 = nullptr;
+    float *kinv32 = nullptr;
     // operator tiles chosen by the autotuner (rows, columns, layers per block)
     bool autotune = false, tuned = false;
     int tile64[3] = {32, 8, 16}, tile32[3] = {32, 8, 16};
@@ -100,7 +104,41 @@ Reduce make_reduce(int fin, int slot)
     r.S = g->S;
     r.fin = fin;
     r.slot = slot;
+    r.guard = 0;
     return r;
+}
+
+static Reduce make_guarded(int fin, int slot)
+{
+    Reduce r = make_reduce(fin, slot);
+    r.guard = 1;
+    return r;
+}
+
+// iterations queued between host synchronisations in the CG loops
+// (CHFEM_CG_BATCH, default 16; 1 synchronises every iteration)
+static unsigned cg_batch()
+{
+    static unsigned b = 0;
+    if (!b) {
+        const char *e = getenv("CHFEM_CG_BATCH");
+        const int v = e ? atoi(e) : 16;
+        b = v < 1 ? 1u : (unsigned)v;
+    }
+    return b;
+}
+
+// arms the device-side stop test of a guarded loop: S_DONE = 0,
+// S_ITER = 0, S_THRESH = thresh
+static int arm_guard(double thresh)
+{
+    g->Sh[S_DONE] = 0.0;
+    g->Sh[S_ITER] = 0.0;
+    g->Sh[S_THRESH] = thresh;
+    if (hipMemcpyAsync(g->S + S_DONE, g->Sh + S_DONE, 3 * sizeof(double), hipMemcpyHostToDevice, g->stream) !=
+        hipSuccess)
+        return 0;
+    return hipStreamSynchronize(g->stream) == hipSuccess;
 }
 
 int sync_scalars()
@@ -234,18 +272,33 @@ static int make_ops()
     const bool use_walsh = g->walsh_ok && !(opsel && !strcmp(opsel, "generic"));
     // key-Jacobi vector kernels: NBN thermal/elastic with the Jacobi
     // preconditioner on (CHFEM_VECTOR=stored keeps the stored-dinv kernels)
+    // scalar density field: element-by-element operator only (the reference
+    // forces EBE for it, femhmg.c:196-203)
+    o.sdf.d = nullptr;
+    o.sdf.fmin = o.sdf.fmax = 0.0;
+    if (m->parametric_density_field_flag && m->parametric_density_field && o.physics != PHY_FLUID) {
+        if (!o.ebe) {
+            printf("ERROR: the scalar density field needs the element-by-element strategy.\n");
+            return 0;
+        }
+        o.sdf.d = m->parametric_density_field;
+        o.sdf.fmin = m->limits_density_field[0];
+        o.sdf.fmax = m->limits_density_field[1];
+    }
     const char *vsel = getenv("CHFEM_VECTOR");
-    const bool kj = g->kdiag64 && g->precond && !o.ebe && o.physics != PHY_FLUID && !(vsel && !strcmp(vsel, "stored"));
-    o.kdiag = kj ? g->kdiag64 : nullptr;
+    const bool kj = g->kinv64 && g->precond && !o.ebe && o.physics != PHY_FLUID && !(vsel && !strcmp(vsel, "stored"));
+    o.kinv = nullptr;
     o.nmat = (int)g->dm.nmat;
     const char *esel = getenv("CHFEM_EBE");
     o.ebe_atomic = (esel && !strcmp(esel, "atomic")) || o.physics == PHY_FLUID;
     g->dm.op64 = o;
     g->dm.op64.K = g->dm.K64;
     g->dm.op64.lamu = use_walsh ? g->lamu64 : nullptr;
+    g->dm.op64.kinv = kj ? g->kinv64 : nullptr;
     g->dm.op32 = o;
     g->dm.op32.K = g->dm.K32;
     g->dm.op32.lamu = use_walsh ? g->lamu32 : nullptr;
+    g->dm.op32.kinv = kj ? g->kinv32 : nullptr;
     if (!fixed_tile && g->tuned) {
         // tiles picked by the autotuner for this model
         g->dm.op64.tile_tr = g->tile64[0]; g->dm.op64.tile_tc = g->tile64[1]; g->dm.op64.tile_lz = g->tile64[2];
@@ -312,7 +365,7 @@ static void note_x0(double delta, double delta0)
 }
 
 // Jacobi-PCG in precision T (pcg.cu:28-243).  r holds b on entry; returns 1 if converged.
-// With op.kdiag (thermal/elastic NBN) the Jacobi diagonal comes from the keys
+// With op.kinv (thermal/elastic NBN) the Jacobi diagonal comes from the keys
 // and z = M^-1 r is folded into the direction update (d = M^-1 r + beta d);
 // otherwise z is stored in q and the diagonal read from dinv.
 template <typename T>
@@ -321,7 +374,7 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
 {
     const size_t n = op.ndof;
     hipStream_t st = g->stream;
-    const bool kj = op.kdiag != nullptr;
+    const bool kj = op.kinv != nullptr;
     if (!x0set) launch_fill<T>(x, 0.0, n, st);
     Reduce r0 = make_reduce(FIN_STORE, S_DELTA0);
     if (kj) launch_kj_precond<T>(op, r, d, &r0, st);
@@ -358,27 +411,32 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
             return 1;
         }
     } else if (!set_scalar(S_DELTA, delta0)) return -1;
-    Reduce ra = make_reduce(FIN_ALPHA, 0), rdl = make_reduce(FIN_DELTA, 0);
-    auto update = [&]() {
-        if (kj) launch_kj_pcg_update<T>(op, x, r, d, q, &rdl, st);
-        else launch_pcg_update<T>(x, r, d, q, dinv, &rdl, n, st);
-    };
-    it = 1;
-    timed_apply<T>(op, d, q, &ra);
-    update();
-    if (!sync_loop()) return -1;
-    delta = g->Sh[S_DELTA];
-    bool cont = above(delta, delta0, tol);
-    while (cont && it < maxit) {
-        it++;
-        if (kj) launch_kj_pcg_dir<T>(op, d, r, g->S, st);
-        else launch_pcg_dir<T>(d, q, g->S, 0, n, st);
-        timed_apply<T>(op, d, q, &ra);
-        update();
+    // Iterations are queued cg_batch() at a time: the finishing kernel of
+    // each update counts the iteration and raises S_DONE once delta <=
+    // tol^2 delta0, after which every queued launch returns at once, so the
+    // iterate and the iteration count are those of a loop that synchronises
+    // every iteration.
+    Reduce ra = make_guarded(FIN_ALPHA, 0), rdl = make_guarded(FIN_DELTA, 0);
+    if (!arm_guard(tol * tol * fabs(delta0))) return -1;
+    it = 0;
+    bool done = false;
+    while (!done && it < maxit) {
+        const unsigned nb = std::min(cg_batch(), maxit - it);
+        for (unsigned b = 0; b < nb; b++) {
+            if (it + b > 0) {
+                if (kj) launch_kj_pcg_dir<T>(op, d, r, g->S, st, 1);
+                else launch_pcg_dir<T>(d, q, g->S, 0, n, st, 1);
+            }
+            if (b == 0) timed_apply<T>(op, d, q, &ra);
+            else launch_apply<T>(op, d, q, &ra, st);
+            if (kj) launch_kj_pcg_update<T>(op, x, r, d, q, &rdl, st);
+            else launch_pcg_update<T>(x, r, d, q, dinv, &rdl, n, st);
+        }
         if (!sync_loop()) return -1;
-        delta = g->Sh[S_DELTA];
-        cont = above(delta, delta0, tol);
+        it = (unsigned)g->Sh[S_ITER];
+        done = g->Sh[S_DONE] != 0.0;
     }
+    delta = g->Sh[S_DELTA];
     residual = sqrt(fabs(delta) / fabs(delta0));
     return !above(delta, delta0, tol);
 }
@@ -483,7 +541,7 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
     const size_t n = o64.ndof;
     hipStream_t st = g->stream;
     const double tol = g->num_tol;
-    const bool kj = o32.kdiag != nullptr;
+    const bool kj = o32.kinv != nullptr;
     float *x = g->x32, *r = g->r32, *d = g->d32, *q = g->q32;
     const float *dinv = g->dinv32;
     Reduce r0 = make_reduce(FIN_STORE, S_DELTA0), rd = make_reduce(FIN_STORE, S_DELTA);
@@ -521,21 +579,36 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
     launch_fill<float>(x, 0.0, n, st);
     if (kj) launch_kj_precond<float>(o32, r, d, &rd, st);
     else launch_precond<float>(dinv, r, d, n, &rd, st);
-    Reduce ra = make_reduce(FIN_ALPHA, 0), rdl = make_reduce(FIN_DELTA, 0);
+    // FP32 iterations are queued cg_batch() at a time; the device stops the
+    // queue once r.z falls to the next replacement threshold
+    // max(tol^2 delta0, drop * delta at the last replacement).
+    Reduce ra = make_guarded(FIN_ALPHA, 0), rdl = make_guarded(FIN_DELTA, 0);
+    const unsigned maxit = g->max_iterations;
+    auto thresh = [&]() { return std::max(tol * tol * fabs(delta0), drop * fabs(delta_rel)); };
+    if (!arm_guard(thresh())) return -1;
     bool converged = false;
-    while (it_total < g->max_iterations) {
-        if (it_total > 0) {
-            if (kj) launch_kj_pcg_dir<float>(o32, d, r, g->S, st);
-            else launch_pcg_dir<float>(d, q, g->S, 0, n, st);
+    unsigned base = 0;  // iterations before the current guarded segment
+    while (true) {
+        unsigned seg = 0;
+        bool done = false;
+        while (!done && base + seg < maxit) {
+            const unsigned nb = std::min(cg_batch(), maxit - base - seg);
+            for (unsigned b = 0; b < nb; b++) {
+                if (base + seg + b > 0) {
+                    if (kj) launch_kj_pcg_dir<float>(o32, d, r, g->S, st, 1);
+                    else launch_pcg_dir<float>(d, q, g->S, 0, n, st, 1);
+                }
+                if (b == 0) timed_apply<float>(o32, d, q, &ra);
+                else launch_apply<float>(o32, d, q, &ra, st);
+                if (kj) launch_kj_pcg_update<float>(o32, x, r, d, q, &rdl, st);
+                else launch_pcg_update<float>(x, r, d, q, dinv, &rdl, n, st);
+            }
+            if (!sync_loop()) return -1;
+            seg = (unsigned)g->Sh[S_ITER];
+            done = g->Sh[S_DONE] != 0.0;
         }
-        timed_apply<float>(o32, d, q, &ra);
-        if (kj) launch_kj_pcg_update<float>(o32, x, r, d, q, &rdl, st);
-        else launch_pcg_update<float>(x, r, d, q, dinv, &rdl, n, st);
-        it_total++;
-        if (!sync_loop()) return -1;
-        delta = g->Sh[S_DELTA];
-        const bool last = it_total >= g->max_iterations;
-        if (!last && above(delta, delta0, tol) && fabs(delta) > drop * fabs(delta_rel)) continue;
+        it_total = base + seg;
+        const bool last = it_total >= maxit;
         // reliable update: x64 += x ; r64 -= A64 x ; x = 0 ; r = (float) r64
         launch_add_correction(g->x64, x, n, st);
         launch_convert<float, double>(x, g->d64, n, st);
@@ -554,6 +627,8 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
         delta_rel = delta;
         launch_convert<double, float>(g->r64, r, n, st);
         if (!kj) launch_precond<float>(dinv, r, q, n, nullptr, st);  // stored-dinv direction reads z from q
+        base = it_total;
+        if (!arm_guard(thresh())) return -1;
     }
     residual = sqrt(fabs(delta) / fabs(delta0));
     return converged ? 1 : 0;
@@ -620,7 +695,7 @@ int solve_device(const double *x0_dev)
 {
     if (!g || !g->model) return -1;
     if (!g->arrays && alloc_solver_arrays() != CUDAPCG_TRUE) return -1;
-    make_ops();
+    if (!make_ops()) return -1;
     autotune_tiles();
     ensure_dinv();
     const size_t n = nvars();
@@ -777,7 +852,8 @@ cudapcgFlag_t cudapcgEnd(void)
     dfree(g->dm.K32);
     dfree(g->lamu64);
     dfree(g->lamu32);
-    dfree(g->kdiag64);
+    dfree(g->kinv64);
+    dfree(g->kinv32);
     if (g->model && g->model->freeAllowed_flag) {
         free_model_device(g->model);
         free(g->model);
@@ -952,11 +1028,27 @@ cudapcgFlag_t cudapcgSetImage(cudapcgMap_t *img)
     return CUDAPCG_TRUE;
 }
 
+// Scalar density field (chfem/src/femhmg/cudapcg/cudapcg.cu:339-352): uint16
+// per element, element scale field * (fmax - fmin) / 65535 + fmin applied to
+// the local matrices of the element-by-element operator.
 cudapcgFlag_t cudapcgSetParametricDensityField(parametricScalarField_t *field, double fmin, double fmax)
 {
-    (void)field; (void)fmin; (void)fmax;
-    printf("ERROR: scalar density field input is not provided by this library.\n");
-    return CUDAPCG_FALSE;
+    if (!g || !g->model || !field) return CUDAPCG_FALSE;
+    cudapcgModel_t *m = g->model;
+    if (g->analysis_flag == CUDAPCG_FLUID_3D) {
+        printf("ERROR: scalar density field input is not supported for permeability analysis.\n");
+        return CUDAPCG_FALSE;
+    }
+    if (!m->parametric_density_field)
+        CHECK_HIP(hipMalloc(&m->parametric_density_field, sizeof(parametricScalarField_t) * m->nelem));
+    CHECK_HIP(hipMemcpyAsync(m->parametric_density_field, field, sizeof(parametricScalarField_t) * m->nelem,
+                             hipMemcpyHostToDevice, g->stream));
+    CHECK_HIP(hipStreamSynchronize(g->stream));
+    m->limits_density_field[0] = fmin;
+    m->limits_density_field[1] = fmax;
+    m->parametric_density_field_flag = CUDAPCG_TRUE;
+    g->dinv_valid = false;
+    return CUDAPCG_TRUE;
 }
 
 cudapcgFlag_t cudapcgSetPoreMap(cudapcgFlag_t *pores)
@@ -1018,17 +1110,31 @@ cudapcgFlag_t cudapcgSetLclMtxs(cudapcgVar_t *L)
         }
     }
     for (size_t i = 0; i < ksz * nk; i++) Kf[i] = (float)K[i];
-    // Jacobi diagonal table for the key-Jacobi vector kernels (2-bit keys:
-    // at most 4 materials)
-    dfree(g->kdiag64);
+    // inverse Jacobi diagonal of every 16-bit key (2 bits per local node: at
+    // most 4 materials) for the key-Jacobi vector kernels; sums in the order
+    // k_jacobi_keys uses, so the values equal the stored dinv
+    dfree(g->kinv64);
+    dfree(g->kinv32);
     if (ph != PHY_FLUID && nk <= 4) {
         const int nd = ph == PHY_ELASTIC ? 3 : 1, ke = 8 * nd;
-        double kd[4 * 8 * 3];
-        for (unsigned m = 0; m < nk; m++)
-            for (int e = 0; e < 8; e++)
-                for (int a = 0; a < nd; a++) kd[(m * 8 + e) * nd + a] = K[m * ksz + (size_t)(nd * e + a) * ke + nd * e + a];
-        if (hipMalloc(&g->kdiag64, sizeof(double) * nk * 8 * nd) != hipSuccess ||
-            hipMemcpy(g->kdiag64, kd, sizeof(double) * nk * 8 * nd, hipMemcpyHostToDevice) != hipSuccess) {
+        std::vector<double> t64((size_t)65536 * nd, 0.0);
+        std::vector<float> t32((size_t)65536 * nd, 0.f);
+        for (unsigned key = 0; key < 65536; key++) {
+            bool valid = true;
+            for (int e = 0; e < 8; e++) valid = valid && ((key >> (2 * e)) & 3u) < nk;
+            if (!valid) continue;
+            for (int a = 0; a < nd; a++) {
+                double sum = 0;
+                for (int e = 0; e < 8; e++)
+                    sum += K[(size_t)((key >> (2 * e)) & 3u) * ksz + (size_t)(nd * e + a) * ke + nd * e + a];
+                t64[(size_t)key * nd + a] = 1.0 / sum;
+                t32[(size_t)key * nd + a] = (float)(1.0 / sum);
+            }
+        }
+        if (hipMalloc(&g->kinv64, t64.size() * sizeof(double)) != hipSuccess ||
+            hipMalloc(&g->kinv32, t32.size() * sizeof(float)) != hipSuccess ||
+            hipMemcpy(g->kinv64, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(g->kinv32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
             fprintf(stderr, "chfem: failed to upload the Jacobi table\n");
             free(K);
             free(Kf);

@@ -57,9 +57,11 @@ __device__ __forceinline__ int edl(int k) { return k >> 2; }
 template <typename T, int ND, int TR, int TC, bool DOT>
 __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T *__restrict__ q,
                                                       const uint16_t *__restrict__ keys, const T *__restrict__ K,
-                                                      int nmat, int ebe, int ny, int nx, int nz, int LZ, Reduce red)
+                                                      int nmat, int ebe, int ny, int nx, int nz, int LZ, Reduce red,
+                                                      Sdf sdf)
 {
     static_assert(TR * TC == 256, "four wavefronts per block");
+    if (DOT && stopped(red)) return;
     constexpr int KE = 8 * ND;                 // local DOFs
     constexpr int MT = (KE + 15) / 16;         // row tiles
     constexpr int NT = TR * TC, NR = TR + 1, NC = TC + 1, NV = NR * NC * ND;
@@ -69,6 +71,7 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
     __shared__ T sf[NT * SF];
     __shared__ T sK[4 * KE * KE];              // K[m][row][col]
     __shared__ int smat[NT];
+    __shared__ T sesc[NT];  // scalar density field: element scale
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6, li = lane & 15, kg = lane >> 4;
     const int tr = tid % TR, tc = tid / TR;
@@ -106,6 +109,9 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
         const unsigned kv = keys[(size_t)ewrap(le, nz) * nxy + eoff];
         return ebe ? (int)(kv & 3u) : (int)((kv >> 6) & 3u);
     };
+    auto escale = [&](int le) -> T {
+        return (sdf.d && elem_ok) ? (T)sdf_scale(sdf, (size_t)ewrap(le, nz) * nxy + eoff) : T(1);
+    };
 
     // element forces of the wavefront's 64 elements (block elements 64*wave ..)
     auto elements = [&](const T *cur, const T *nxt) {
@@ -122,11 +128,13 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
             for (int nt = 0; nt < 4; nt++) acc[mt][nt] = typename MfmaAcc<T>::type{0, 0, 0, 0};
         // the 4 elements this lane feeds as B columns: block element 64*wave + 16*nt + li
         int ebase[4], emat[4];
+        T esc[4];
 #pragma unroll
         for (int nt = 0; nt < 4; nt++) {
             const int e = 64 * wave + 16 * nt + li;
             ebase[nt] = ((e / TR) * NR + (e % TR)) * ND;
             emat[nt] = smat[e];
+            esc[nt] = sesc[e];
         }
         for (int m = 0; m < 4; m++) {
             if (!((present >> m) & 1u)) continue;
@@ -139,7 +147,7 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
                 const int doff = (edc(kn) * NR + edr(kn)) * ND + comp;
                 T b[4];
 #pragma unroll
-                for (int nt = 0; nt < 4; nt++) b[nt] = emat[nt] == m ? pl[ebase[nt] + doff] : T(0);
+                for (int nt = 0; nt < 4; nt++) b[nt] = emat[nt] == m ? esc[nt] * pl[ebase[nt] + doff] : T(0);
 #pragma unroll
                 for (int mt = 0; mt < MT; mt++) {
                     const int row = 16 * mt + li;
@@ -171,12 +179,15 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
     fetch(l0);
     store(nxt);
     smat[tid] = material(l0 - 1);
+    sesc[tid] = escale(l0 - 1);
     __syncthreads();
     fetch(l0 + 1);
     elements(cur, nxt);
     const int mnext = material(l0);
+    const T enext = escale(l0);
     __syncthreads();
     smat[tid] = mnext;
+    sesc[tid] = enext;
     T acc[ND];
 #pragma unroll
     for (int a = 0; a < ND; a++) acc[a] = 0;
@@ -208,6 +219,7 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
         }
         elements(cur, nxt);
         const int mn = more ? material(l + 1) : -1;
+        const T en = more ? escale(l + 1) : T(1);
         __syncthreads();
         if (own) {
             T o[ND];
@@ -238,6 +250,7 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
             store(cur);
             T *t = cur; cur = nxt; nxt = t;
             smat[tid] = mn;
+            sesc[tid] = en;
         }
         __syncthreads();
     }
@@ -251,7 +264,7 @@ template <typename T>
 void launch_ebe_mfma(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s)
 {
     constexpr int TR = 32, TC = 8;
-    Reduce rr = {nullptr, nullptr, 0, 0};
+    Reduce rr = {nullptr, nullptr, 0, 0, 0};
     if (red) rr = *red;
     int LZ = 16;
     const long cols = (long)((op.ny + TR - 2) / (TR - 1)) * ((op.nx + TC - 2) / (TC - 1));
@@ -259,11 +272,11 @@ void launch_ebe_mfma(const OpDesc &op, const T *v, T *q, const Reduce *red, hipS
     dim3 grid((op.ny + TR - 2) / (TR - 1), (op.nx + TC - 2) / (TC - 1), (op.nz + LZ - 1) / LZ);
     const T *K = (const T *)op.K;
     if (op.physics == PHY_ELASTIC) {
-        if (red) k_ebe_mfma<T, 3, TR, TC, true><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr);
-        else k_ebe_mfma<T, 3, TR, TC, false><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr);
+        if (red) k_ebe_mfma<T, 3, TR, TC, true><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr, op.sdf);
+        else k_ebe_mfma<T, 3, TR, TC, false><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr, op.sdf);
     } else {
-        if (red) k_ebe_mfma<T, 1, TR, TC, true><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr);
-        else k_ebe_mfma<T, 1, TR, TC, false><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr);
+        if (red) k_ebe_mfma<T, 1, TR, TC, true><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr, op.sdf);
+        else k_ebe_mfma<T, 1, TR, TC, false><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr, op.sdf);
     }
     if (red) launch_reduce_finish(rr, grid.x * grid.y * grid.z, 1, s);
 }

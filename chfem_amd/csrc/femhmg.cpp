@@ -16,6 +16,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -36,7 +37,10 @@ const int DL[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 
 struct Model {
     std::string nf, nf_noext, image_file, x0_file;
-    bool have_sdf = false;
+    bool have_sdf = false;            // scalar density field read from a .bin file
+    bool sdf_f64 = false;             // %data_type float64 (else float32)
+    std::vector<uint16_t> density;    // quantised field per (refined) element
+    double dmin = 0, dmax = 0;
     int analysis = HMG_THERMAL;
     int dim = HMG_3D;
     unsigned hmg_flag = HOMOGENIZE_ALL;
@@ -316,6 +320,7 @@ bool read_data(Model &m, const char *filename)
             }
         } else if (!strcmp(str, "%data_type")) {
             if (fscanf(f, "%255s", str) != 1) break;
+            m.sdf_f64 = !strcmp(str, "float64");
             count++;
         } else if (!strcmp(str, "%xreduce_scale_factor")) {
             if (fscanf(f, "%lf", &g_xreduce_stab) != 1) break;
@@ -361,6 +366,82 @@ bool read_image(Model &m, const uint8_t *img)
     return true;
 }
 
+// strategy of the solve: a scalar density field needs element-by-element
+// products (femhmg.c:196-203)
+cudapcgFlag_t strategy(const Model &m) { return m.have_sdf ? CUDAPCG_EBE : g_strategy; }
+
+// Scalar density field from a .bin file of float32/float64 values in image
+// file order (chfem/src/femhmg/femhmg.c:1760-1835): the element value is the
+// file value times the first property of the element's material, quantised
+// to 16 bits between the extreme values found (the extremes pair file-order
+// values with the element-order material map, as the reference does), and
+// replicated over refined elements.  Properties are then normalised: thermal
+// runs with one unit material, elastic keeps each material's Poisson ratio
+// with E = 1.
+bool read_sdf(Model &m, const char *filename)
+{
+    const unsigned ref = m.refinement;
+    const size_t rows = m.ey / ref, cols = m.ex / ref, slices = m.ez / ref;
+    const size_t rows_ref = m.ey, cols_ref = m.ex, n0 = rows * cols * slices;
+    const size_t sz = m.sdf_f64 ? sizeof(double) : sizeof(float);
+    std::vector<unsigned char> raw(sz * m.nelem);
+    printf("    Getting density field from .bin...");
+    FILE *f = fopen(filename, "rb");
+    if (!f) {
+        printf(" ERROR: Failed to open %s.\n", filename);
+        return false;
+    }
+    const size_t got = fread(raw.data(), sz, m.nelem, f);
+    fclose(f);
+    if (got < m.nelem) {
+        printf(" ERROR: Failed to read all %e MB from %s.\n", (float)(sz * m.nelem) / 1024.0 / 1024.0, filename);
+        return false;
+    }
+    printf("Done.\n");
+    auto val = [&](size_t i) -> double {
+        if (m.sdf_f64) {
+            double d;
+            memcpy(&d, raw.data() + i * sizeof(double), sizeof(double));
+            return d;
+        }
+        float v;
+        memcpy(&v, raw.data() + i * sizeof(float), sizeof(float));
+        return (double)v;
+    };
+    const unsigned stride = m.analysis == HMG_ELASTIC ? 2 : 1;
+    double Emax = val(0) * m.props[m.elem_map[0] * stride], Emin = Emax;
+    for (size_t i = 1; i < n0; i++) {
+        const double e = val(i) * m.props[m.elem_map[i] * stride];
+        Emax = e > Emax ? e : Emax;
+        Emin = e < Emin ? e : Emin;
+    }
+    if (Emax == Emin) Emin = Emax - 1.0;
+    const double dE = Emax - Emin;
+    m.dmax = Emax;
+    m.dmin = Emin;
+    m.density.assign(m.nelem, 0);
+    for (size_t k = 0; k < slices; k++)
+        for (size_t i = 0; i < rows; i++)
+            for (size_t j = 0; j < cols; j++) {
+                const double b = val(j + i * cols + k * rows * cols) *
+                                 m.props[m.elem_map[(i + j * rows_ref + k * rows_ref * cols_ref) * ref] * stride];
+                const uint16_t d = (uint16_t)(65535 * (b - Emin) / dE);
+                for (size_t kk = ref * k; kk < ref * (k + 1); kk++)
+                    for (size_t ii = ref * i; ii < ref * (i + 1); ii++)
+                        for (size_t jj = ref * j; jj < ref * (j + 1); jj++)
+                            m.density[ii + jj * rows_ref + kk * rows_ref * cols_ref] = d;
+            }
+    if (m.analysis == HMG_THERMAL) {
+        m.nmat = 1;
+        m.props[0] = 1.0;
+        std::fill(m.elem_map.begin(), m.elem_map.end(), (uint16_t)0);
+    } else {
+        for (unsigned j = 0; j < m.nmat; j++) m.props[2 * j] = 1.0;
+    }
+    m.have_sdf = true;
+    return true;
+}
+
 bool read_raw(Model &m, const char *filename)
 {
     const size_t n = (size_t)(m.ex / m.refinement) * (m.ey / m.refinement) * (m.ez / m.refinement);
@@ -388,9 +469,9 @@ cudapcgFlag_t model_ctor(cudapcgModel_t **ptr, const void *data)
     cudapcgModel_t *p = *ptr;
     p->name = (char *)malloc(m->nf_noext.size() + 1);
     strcpy(p->name, m->nf_noext.c_str());
-    p->parStrategy_flag = g_strategy;
+    p->parStrategy_flag = strategy(*m);
     p->poremap_flag = m->poremap;
-    p->parametric_density_field_flag = CUDAPCG_FALSE;
+    p->parametric_density_field_flag = m->have_sdf ? CUDAPCG_TRUE : CUDAPCG_FALSE;
     p->nrows = m->ey;
     p->ncols = m->ex;
     p->nlayers = m->ez;
@@ -518,7 +599,7 @@ void directions(const Model &m, std::vector<unsigned> &dirs)
 // set up the solver for model m (femhmg.c:914-1002)
 bool solver_setup(Model &m, DeviceAux &aux)
 {
-    if (!cudapcgInit(m.pcg_flag, m.analysis == HMG_FLUID ? CUDAPCG_NBN : g_strategy)) return false;
+    if (!cudapcgInit(m.pcg_flag, m.analysis == HMG_FLUID ? CUDAPCG_NBN : strategy(m))) return false;
     cudapcgSetXReduceMode(g_xreduce);
     cudapcgSetXReduceScale(m.elem_size * m.elem_size / (double)m.nelem);
     cudapcgSetReduceStabFactor(g_xreduce_stab);
@@ -533,8 +614,12 @@ bool solver_setup(Model &m, DeviceAux &aux)
     cudapcgSetNumTol(m.tol);
     cudapcgSetMaxIterations(m.maxit);
     if (m.analysis != HMG_FLUID) {
-        if (g_strategy == CUDAPCG_NBN) cudapcgSetImage(m.keys.data());
+        if (strategy(m) == CUDAPCG_NBN) cudapcgSetImage(m.keys.data());
         else cudapcgSetImage(m.elem_map.data());
+        if (m.have_sdf && !cudapcgSetParametricDensityField(m.density.data(), m.dmin, m.dmax)) {
+            cudapcgEnd();
+            return false;
+        }
     } else {
         cudapcgSetPeriodic2DOFMap(m.dof_id.data());
         cudapcgSetPoreMap(m.bkeys.data());
@@ -589,10 +674,9 @@ logical hmgInit(char *data_filename, char *elem_filename, char *sdf_filename, ui
     if (elem_filename) m.image_file = elem_filename;
     m.poremap = g_poremap_default;
     if (!read_data(m, data_filename)) { delete M; M = nullptr; return HMG_FALSE; }
-    if (sdf_filename) {
-        printf("ERROR: scalar density field input (.bin) is not provided by this library.\n");
-        delete M; M = nullptr;
-        return HMG_FALSE;
+    if (sdf_filename && m.analysis == HMG_FLUID) {
+        printf("WARNING: Scalar density field input is not supported for permeability analysis.\n");
+        sdf_filename = nullptr;
     }
     if (!init_sizes(m)) { delete M; M = nullptr; return HMG_FALSE; }
     bool ok;
@@ -603,6 +687,7 @@ logical hmgInit(char *data_filename, char *elem_filename, char *sdf_filename, ui
         printf("ERROR: file %s is not in a valid format.\n", elem_filename ? elem_filename : "(null)");
         ok = false;
     }
+    if (ok && sdf_filename) ok = read_sdf(m, sdf_filename);
     if (!ok) { delete M; M = nullptr; return HMG_FALSE; }
     if (m.analysis == HMG_FLUID) build_fluid_maps(m);
     else build_keys(m);
@@ -672,6 +757,10 @@ void hmgImportX(char *file)
 void hmgFindInitialGuesses(unsigned int nlevels)
 {
     if (!nlevels || !M) return;
+    if (M->have_sdf) {
+        printf("WARNING: Initial guess estimation is not available with a scalar density field input.\n");
+        return;
+    }
     if (M->analysis == HMG_FLUID) {
         printf("WARNING: Initial guess estimation is not yet available for permeability analysis.\n");
         printf("         Null initial guess will be considered (x0 = [0]).\n");

@@ -17,7 +17,10 @@ enum Physics { PHY_THERMAL = 0, PHY_ELASTIC = 1, PHY_FLUID = 2 };
 // Device-resident scalars of one Krylov solve (FP64).
 enum ScalarSlot {
     S_DELTA = 0, S_DELTA_OLD, S_DELTA0, S_DQ, S_ALPHA, S_STOP, S_RES0, S_TMP0, S_TMP1,
-    S_TMP2, S_TMP3, S_TMP4, S_TMP5, S_TMP6, S_TMP7, S_COUNT = 16
+    S_TMP2, S_TMP3, S_TMP4, S_TMP5, S_TMP6, S_TMP7,
+    // guarded Krylov loops: done flag, iterations counted on the device,
+    // stop threshold on r.z
+    S_DONE = 16, S_ITER, S_THRESH, S_COUNT = 24
 };
 
 // How the finishing kernel of a fused reduction publishes its total.
@@ -32,6 +35,19 @@ struct Reduce {
     double *S;           // scalar slots
     int fin;
     int slot;
+    // guarded: the launch and its finishing kernel do nothing once S[S_DONE]
+    // is set; a guarded FIN_DELTA finish counts the iteration in S[S_ITER]
+    // and sets S[S_DONE] when |r.z| <= S[S_THRESH].  The host can then queue
+    // several iterations between synchronisations.
+    int guard;
+};
+
+// Scalar density field (EBE only): element e's local matrices are scaled by
+// d[e] * (1/65535) * (fmax - fmin) + fmin (chfem/src/femhmg/femhmg.c:1760-1835);
+// d == nullptr: no field.
+struct Sdf {
+    const uint16_t *d;
+    double fmin, fmax;
 };
 
 // Geometry and material data of one operator, all device pointers.
@@ -52,13 +68,15 @@ struct OpDesc {
     // non-null selects the Walsh-basis operator (walsh.hip)
     const void *lamu;
     int tile_tr, tile_tc, tile_lz;
-    // Jacobi diagonal table kdiag[(m*8 + e)*nd + a] (FP64, nmat <= 4): non-null
-    // selects the key-Jacobi vector kernels (thermal/elastic NBN)
-    const double *kdiag;
+    // inverse Jacobi diagonal per packed key, kinv[key * nd + a] (65536 keys,
+    // precision of K): non-null selects the key-Jacobi vector kernels
+    // (thermal/elastic NBN)
+    const void *kinv;
     int nmat;
     // EBE: 1 selects the reference-style element kernel with atomics instead
     // of the matrix-core kernel (ebe_mfma.hip; up to 4 materials)
     int ebe_atomic;
+    Sdf sdf;
 };
 
 template <typename T> void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s);
@@ -81,7 +99,8 @@ template <typename Tin, typename Tout> void launch_convert(const Tin *in, Tout *
 // out = dinv*v ; optional reduction of v.out
 template <typename T> void launch_precond(const T *dinv, const T *v, T *out, size_t n, const Reduce *red, hipStream_t s);
 // d = z + beta*d, beta = S[DELTA]/S[DELTA_OLD] (first: d = z)
-template <typename T> void launch_pcg_dir(T *d, const T *z, const double *S, int first, size_t n, hipStream_t s);
+template <typename T> void launch_pcg_dir(T *d, const T *z, const double *S, int first, size_t n, hipStream_t s,
+                                          int guard = 0);
 // x += a d ; r -= a q ; z = dinv r ; reduce r.z  (a = S[ALPHA])
 template <typename T> void launch_pcg_update(T *x, T *r, const T *d, T *z, const T *dinv, const Reduce *red,
                                              size_t n, hipStream_t s);
@@ -99,14 +118,15 @@ void launch_add_correction(double *x, const float *e, size_t n, hipStream_t s);
 // y += x
 template <typename T> void launch_add(T *y, const T *x, size_t n, hipStream_t s);
 
-// Key-Jacobi PCG vector kernels (op.kdiag non-null; thermal/elastic NBN):
-// dinv is rebuilt from the keys and z = dinv r is never stored.
+// Key-Jacobi PCG vector kernels (op.kinv non-null; thermal/elastic NBN):
+// dinv is looked up by key and z = dinv r is never stored.
 //   kj_precond:    out = dinv v ; red <- v.out
 //   kj_pcg_dir:    d = dinv r + (S[delta]/S[delta_old]) d
 //   kj_pcg_update: x += alpha d ; r -= alpha q ; red <- r.(dinv r)
 //   kj_sub_precond: r -= q ; z = dinv r ; red <- r.z
 template <typename T> void launch_kj_precond(const OpDesc &op, const T *v, T *out, const Reduce *red, hipStream_t s);
-template <typename T> void launch_kj_pcg_dir(const OpDesc &op, T *d, const T *r, const double *S, hipStream_t s);
+template <typename T> void launch_kj_pcg_dir(const OpDesc &op, T *d, const T *r, const double *S, hipStream_t s,
+                                             int guard = 0);
 template <typename T> void launch_kj_pcg_update(const OpDesc &op, T *x, T *r, const T *d, const T *q,
                                                 const Reduce *red, hipStream_t s);
 template <typename T> void launch_kj_sub_precond(const OpDesc &op, T *r, const T *q, T *z, const Reduce *red,

@@ -106,6 +106,7 @@ __global__ void __launch_bounds__(BS) k_thermal_nbn(const T *__restrict__ v, T *
                                                     const uint16_t *__restrict__ keys, const T *__restrict__ K,
                                                     int ny, int nx, int nz, uint32_t N, Reduce red)
 {
+    if (DOT && stopped(red)) return;
     double acc = 0.0;
     for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS) {
         int r, c, l;
@@ -147,6 +148,7 @@ __global__ void __launch_bounds__(BS) k_elastic_nbn(const T *__restrict__ v, T *
                                                     const uint16_t *__restrict__ keys, const T *__restrict__ K,
                                                     int ny, int nx, int nz, uint32_t N, Reduce red)
 {
+    if (DOT && stopped(red)) return;
     double acc = 0.0;
     for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS) {
         int r, c, l;
@@ -198,7 +200,7 @@ __global__ void __launch_bounds__(BS) k_elastic_nbn(const T *__restrict__ v, T *
 template <typename T, int ND>
 __global__ void __launch_bounds__(BS) k_ebe(const T *__restrict__ v, T *__restrict__ q,
                                             const uint16_t *__restrict__ emap, const T *__restrict__ K,
-                                            int ny, int nx, int nz, uint32_t N)
+                                            int ny, int nx, int nz, uint32_t N, Sdf sdf)
 {
     constexpr int KE = 8 * ND;
     for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS) {
@@ -227,10 +229,11 @@ __global__ void __launch_bounds__(BS) k_ebe(const T *__restrict__ v, T *__restri
                 fe[t] = s;
             }
         });
+        const T sc = (T)sdf_scale(sdf, i);
 #pragma unroll
         for (int k = 0; k < 8; k++)
 #pragma unroll
-            for (int a = 0; a < ND; a++) atomicAdd(q + ND * idx[k] + a, fe[ND * k + a]);
+            for (int a = 0; a < ND; a++) atomicAdd(q + ND * idx[k] + a, sc * fe[ND * k + a]);
     }
 }
 
@@ -246,6 +249,7 @@ __global__ void __launch_bounds__(BS) k_fluid(const T *__restrict__ v, T *__rest
                                               const uint8_t *__restrict__ bkeys, const T *__restrict__ K,
                                               int ny, int nx, int nz, uint32_t nvel, uint32_t nborder, Reduce red)
 {
+    if (DOT && stopped(red)) return;
     const uint32_t ntot = nvel + nborder;
     double acc = 0.0;
     for (uint32_t t = blockIdx.x * BS + threadIdx.x; t < ntot; t += gridDim.x * BS) {
@@ -392,6 +396,7 @@ __global__ void __launch_bounds__(BS) k_precond(const T *__restrict__ dinv, cons
 template <typename T>
 __global__ void __launch_bounds__(BS) k_dot(const T *__restrict__ a, const T *__restrict__ b, size_t n, Reduce red)
 {
+    if (stopped(red)) return;
     double acc = 0;
     STREAM_RANGE(T, n)
     FOR_VECTORS
@@ -407,8 +412,9 @@ __global__ void __launch_bounds__(BS) k_dot(const T *__restrict__ a, const T *__
 
 template <typename T>
 __global__ void __launch_bounds__(BS) k_pcg_dir(T *__restrict__ d, const T *__restrict__ z,
-                                                const double *__restrict__ S, int first, size_t n)
+                                                const double *__restrict__ S, int first, size_t n, int guard)
 {
+    if (guard && S[S_DONE] != 0.0) return;
     const T beta = first ? T(0) : (T)(S[S_DELTA] / S[S_DELTA_OLD]);
     STREAM_RANGE(T, n)
     FOR_VECTORS
@@ -432,6 +438,7 @@ __global__ void __launch_bounds__(BS) k_pcg_update(T *__restrict__ x, T *__restr
                                                    T *__restrict__ z, const T *__restrict__ dinv, size_t n,
                                                    Reduce red)
 {
+    if (stopped(red)) return;
     const T a = (T)red.S[S_ALPHA];
     double acc = 0;
     STREAM_RANGE(T, n)
@@ -572,13 +579,13 @@ __global__ void __launch_bounds__(BS) k_minres_update(T *__restrict__ x, T *__re
 
 // ---------------------------------------------------------------------------
 // Key-Jacobi vector kernels (thermal/elastic NBN): the Jacobi diagonal is
-// rebuilt from the node's packed key instead of being read from a stored
-// vector, as the reference's preconditioner does (matrix_free.cu:167-188,
+// taken from the node's packed key instead of a stored vector, as the
+// reference's preconditioner rebuilds it from the key (matrix_free.cu:167-188,
 // 388-408), and the preconditioned residual z = dinv r is never stored.
+// kinv[key * ND + a] (one entry per possible 16-bit key) holds
+// 1 / sum_e K[m_e][e][a] summed over e = 0..7 in FP64 and rounded to T, the
+// same value k_jacobi_keys computes; a two-phase image touches 256 entries.
 // Work is done in groups of 4 nodes (4*ND entries = whole 16-byte vectors).
-// kdiag[(m*8 + e)*ND + a] is the diagonal entry of material m's local matrix
-// at local node e, component a; sums run over e = 0..7 in FP64 and the
-// inverse is rounded to T, exactly as k_jacobi_keys computes dinv.
 // ---------------------------------------------------------------------------
 template <typename T, int ND>
 struct Group {
@@ -605,35 +612,23 @@ __device__ __forceinline__ void gstore(T *p, const Group<T, ND> &g)
     for (int i = 0; i < Group<T, ND>::NV; i++) q[i] = t[i];
 }
 
-template <int ND>
-__device__ __forceinline__ double *kj_table(const double *kdiag, int nmat)
+// dinv of the ND DOFs of a node from its packed key: kinv[key * ND + a]
+template <typename T, int ND>
+__device__ __forceinline__ void node_dinv(const T *__restrict__ kinv, unsigned key, T *dv)
 {
-    __shared__ double sd[4 * 8 * ND];
-    for (int t = threadIdx.x; t < 4 * 8 * ND; t += blockDim.x) sd[t] = t < nmat * 8 * ND ? kdiag[t] : 1.0;
-    __syncthreads();
-    return sd;
+#pragma unroll
+    for (int a = 0; a < ND; a++) dv[a] = kinv[(size_t)key * ND + a];
 }
 
 template <typename T, int ND>
-__device__ __forceinline__ void node_dinv(const double *sd, unsigned key, T *dv)
-{
-#pragma unroll
-    for (int a = 0; a < ND; a++) {
-        double s = 0;
-#pragma unroll
-        for (int e = 0; e < 8; e++) s += sd[(((key >> (2 * e)) & 3u) * 8 + e) * ND + a];
-        dv[a] = (T)(1.0 / s);
-    }
-}
-
-template <typename T, int ND>
-__device__ __forceinline__ void group_dinv(const double *sd, const uint16_t *keys, size_t g, Group<T, ND> &dv)
+__device__ __forceinline__ void group_dinv(const T *__restrict__ kinv, const uint16_t *keys, size_t g,
+                                           Group<T, ND> &dv)
 {
     const uint2 k = *reinterpret_cast<const uint2 *>(keys + 4 * g);
-    node_dinv<T, ND>(sd, k.x & 0xffffu, dv.v);
-    node_dinv<T, ND>(sd, k.x >> 16, dv.v + ND);
-    node_dinv<T, ND>(sd, k.y & 0xffffu, dv.v + 2 * ND);
-    node_dinv<T, ND>(sd, k.y >> 16, dv.v + 3 * ND);
+    node_dinv<T, ND>(kinv, k.x & 0xffffu, dv.v);
+    node_dinv<T, ND>(kinv, k.x >> 16, dv.v + ND);
+    node_dinv<T, ND>(kinv, k.y & 0xffffu, dv.v + 2 * ND);
+    node_dinv<T, ND>(kinv, k.y >> 16, dv.v + 3 * ND);
 }
 
 // block b owns groups [gb_, ge_); the last block also takes the N % 4 tail nodes
@@ -649,18 +644,16 @@ __device__ __forceinline__ void group_dinv(const double *sd, const uint16_t *key
 
 // out = dinv v ; reduce v.out
 template <typename T, int ND>
-__global__ void __launch_bounds__(BS) k_kj_precond(const uint16_t *__restrict__ keys, const double *__restrict__ kdiag,
-                                                   int nmat, const T *__restrict__ v, T *__restrict__ out,
+__global__ void __launch_bounds__(BS) k_kj_precond(const uint16_t *__restrict__ keys, const T *__restrict__ kinv, const T *__restrict__ v, T *__restrict__ out,
                                                    size_t N, Reduce red)
 {
-    const double *sd = kj_table<ND>(kdiag, nmat);
     double acc = 0;
     GROUP_RANGE(N)
     FOR_GROUPS
     {
         Group<T, ND> a, m, o;
         gload<T, ND>(v + g_ * 4 * ND, a);
-        group_dinv<T, ND>(sd, keys, g_, m);
+        group_dinv<T, ND>(kinv, keys, g_, m);
 #pragma unroll
         for (int k = 0; k < 4 * ND; k++) {
             o.v[k] = m.v[k] * a.v[k];
@@ -671,7 +664,7 @@ __global__ void __launch_bounds__(BS) k_kj_precond(const uint16_t *__restrict__ 
     FOR_TAIL_NODES(N)
     {
         T dv[ND];
-        node_dinv<T, ND>(sd, keys[i], dv);
+        node_dinv<T, ND>(kinv, keys[i], dv);
 #pragma unroll
         for (int a = 0; a < ND; a++) {
             const T o = dv[a] * v[ND * i + a];
@@ -686,10 +679,10 @@ __global__ void __launch_bounds__(BS) k_kj_precond(const uint16_t *__restrict__ 
 // d = dinv r + beta d
 template <typename T, int ND>
 __global__ void __launch_bounds__(BS) k_kj_pcg_dir(T *__restrict__ d, const T *__restrict__ r,
-                                                   const uint16_t *__restrict__ keys, const double *__restrict__ kdiag,
-                                                   int nmat, const double *__restrict__ S, size_t N)
+                                                   const uint16_t *__restrict__ keys, const T *__restrict__ kinv, const double *__restrict__ S, size_t N,
+                                                   int guard)
 {
-    const double *sd = kj_table<ND>(kdiag, nmat);
+    if (guard && S[S_DONE] != 0.0) return;
     const T beta = (T)(S[S_DELTA] / S[S_DELTA_OLD]);
     GROUP_RANGE(N)
     FOR_GROUPS
@@ -697,7 +690,7 @@ __global__ void __launch_bounds__(BS) k_kj_pcg_dir(T *__restrict__ d, const T *_
         Group<T, ND> rv, dv, m;
         gload<T, ND>(r + g_ * 4 * ND, rv);
         gload<T, ND>(d + g_ * 4 * ND, dv);
-        group_dinv<T, ND>(sd, keys, g_, m);
+        group_dinv<T, ND>(kinv, keys, g_, m);
 #pragma unroll
         for (int k = 0; k < 4 * ND; k++) dv.v[k] = m.v[k] * rv.v[k] + beta * dv.v[k];
         gstore<T, ND>(d + g_ * 4 * ND, dv);
@@ -705,7 +698,7 @@ __global__ void __launch_bounds__(BS) k_kj_pcg_dir(T *__restrict__ d, const T *_
     FOR_TAIL_NODES(N)
     {
         T dvn[ND];
-        node_dinv<T, ND>(sd, keys[i], dvn);
+        node_dinv<T, ND>(kinv, keys[i], dvn);
 #pragma unroll
         for (int a = 0; a < ND; a++) d[ND * i + a] = dvn[a] * r[ND * i + a] + beta * d[ND * i + a];
     }
@@ -715,10 +708,10 @@ __global__ void __launch_bounds__(BS) k_kj_pcg_dir(T *__restrict__ d, const T *_
 template <typename T, int ND>
 __global__ void __launch_bounds__(BS) k_kj_pcg_update(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ d,
                                                       const T *__restrict__ q, const uint16_t *__restrict__ keys,
-                                                      const double *__restrict__ kdiag, int nmat, size_t N,
+                                                      const T *__restrict__ kinv, size_t N,
                                                       Reduce red)
 {
-    const double *sd = kj_table<ND>(kdiag, nmat);
+    if (stopped(red)) return;
     const T a = (T)red.S[S_ALPHA];
     double acc = 0;
     GROUP_RANGE(N)
@@ -730,7 +723,7 @@ __global__ void __launch_bounds__(BS) k_kj_pcg_update(T *__restrict__ x, T *__re
         gload<T, ND>(r + o, rv);
         gload<T, ND>(d + o, dv);
         gload<T, ND>(q + o, qv);
-        group_dinv<T, ND>(sd, keys, g_, m);
+        group_dinv<T, ND>(kinv, keys, g_, m);
 #pragma unroll
         for (int k = 0; k < 4 * ND; k++) {
             xv.v[k] += a * dv.v[k];
@@ -743,7 +736,7 @@ __global__ void __launch_bounds__(BS) k_kj_pcg_update(T *__restrict__ x, T *__re
     FOR_TAIL_NODES(N)
     {
         T dvn[ND];
-        node_dinv<T, ND>(sd, keys[i], dvn);
+        node_dinv<T, ND>(kinv, keys[i], dvn);
 #pragma unroll
         for (int c = 0; c < ND; c++) {
             const size_t j = ND * i + c;
@@ -761,10 +754,9 @@ __global__ void __launch_bounds__(BS) k_kj_pcg_update(T *__restrict__ x, T *__re
 template <typename T, int ND>
 __global__ void __launch_bounds__(BS) k_kj_sub_precond(T *__restrict__ r, const T *__restrict__ q, T *__restrict__ z,
                                                        const uint16_t *__restrict__ keys,
-                                                       const double *__restrict__ kdiag, int nmat, size_t N,
+                                                       const T *__restrict__ kinv, size_t N,
                                                        Reduce red)
 {
-    const double *sd = kj_table<ND>(kdiag, nmat);
     double acc = 0;
     GROUP_RANGE(N)
     FOR_GROUPS
@@ -773,7 +765,7 @@ __global__ void __launch_bounds__(BS) k_kj_sub_precond(T *__restrict__ r, const 
         Group<T, ND> rv, qv, m, zv;
         gload<T, ND>(r + o, rv);
         gload<T, ND>(q + o, qv);
-        group_dinv<T, ND>(sd, keys, g_, m);
+        group_dinv<T, ND>(kinv, keys, g_, m);
 #pragma unroll
         for (int k = 0; k < 4 * ND; k++) {
             rv.v[k] -= qv.v[k];
@@ -786,7 +778,7 @@ __global__ void __launch_bounds__(BS) k_kj_sub_precond(T *__restrict__ r, const 
     FOR_TAIL_NODES(N)
     {
         T dvn[ND];
-        node_dinv<T, ND>(sd, keys[i], dvn);
+        node_dinv<T, ND>(kinv, keys[i], dvn);
 #pragma unroll
         for (int c = 0; c < ND; c++) {
             const size_t j = ND * i + c;
@@ -817,7 +809,7 @@ __global__ void __launch_bounds__(BS) k_add_correction(double *x, const float *_
 // ---------------------------------------------------------------------------
 template <typename T, int ND>
 __global__ void __launch_bounds__(BS) k_jacobi_keys(const uint16_t *__restrict__ keys, const T *__restrict__ K,
-                                                    T *dinv, uint32_t N, int ebe, int ny, int nx, int nz)
+                                                    T *dinv, uint32_t N, int ebe, int ny, int nx, int nz, Sdf sdf)
 {
     constexpr int KE = 8 * ND;
     for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS) {
@@ -825,6 +817,7 @@ __global__ void __launch_bounds__(BS) k_jacobi_keys(const uint16_t *__restrict__
 #pragma unroll
         for (int a = 0; a < ND; a++) dsum[a] = 0;
         unsigned key = 0;
+        double esc[8] = {1, 1, 1, 1, 1, 1, 1, 1};
         if (ebe) {
             // EBE keeps the element map: the material of the element with this
             // node as local node e sits at the element origin.
@@ -833,13 +826,18 @@ __global__ void __launch_bounds__(BS) k_jacobi_keys(const uint16_t *__restrict__
             for (int e = 0; e < 8; e++) {
                 int re = r - cDR[e], ce = c - cDC[e], le = l - cDL[e];
                 re += re < 0 ? ny : 0; ce += ce < 0 ? nx : 0; le += le < 0 ? nz : 0;
-                key |= (unsigned)(keys[(size_t)re + (size_t)ce * ny + (size_t)le * nx * ny] & 3u) << (2 * e);
+                const size_t el = (size_t)re + (size_t)ce * ny + (size_t)le * nx * ny;
+                key |= (unsigned)(keys[el] & 3u) << (2 * e);
+                esc[e] = sdf_scale(sdf, el);
             }
         } else key = keys[i];
         for (int e = 0; e < 8; e++) {
             const int m = (key >> (2 * e)) & 3;
 #pragma unroll
-            for (int a = 0; a < ND; a++) dsum[a] += (double)K[m * KE * KE + (ND * e + a) * KE + ND * e + a];
+            for (int a = 0; a < ND; a++) {
+                const double kd = (double)K[m * KE * KE + (ND * e + a) * KE + ND * e + a];
+                dsum[a] += sdf.d ? esc[e] * kd : kd;
+            }
         }
 #pragma unroll
         for (int a = 0; a < ND; a++) dinv[(size_t)ND * i + a] = (T)(1.0 / dsum[a]);
@@ -880,7 +878,7 @@ __global__ void __launch_bounds__(BS) k_jacobi_fluid(const uint8_t *__restrict__
 template <int ND>
 __global__ void __launch_bounds__(BS) k_rhs_seam(double *b, const uint16_t *__restrict__ keys,
                                                  const double *__restrict__ K, int ebe, int ny, int nx, int nz,
-                                                 uint32_t N, LoadCase lc)
+                                                 uint32_t N, LoadCase lc, Sdf sdf)
 {
     constexpr int KE = 8 * ND;
     for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS) {
@@ -898,14 +896,15 @@ __global__ void __launch_bounds__(BS) k_rhs_seam(double *b, const uint16_t *__re
                 re += re < 0 ? ny : 0; ce += ce < 0 ? nx : 0; le += le < 0 ? nz : 0;
                 const bool seam = lc.axis == 0 ? ce == nx - 1 : (lc.axis == 1 ? re == 0 : le == 0);
                 if (!seam) continue;
-                const int m = ebe ? (int)keys[(size_t)re + (size_t)ce * ny + (size_t)le * nx * ny]
-                                  : (int)((key >> (2 * e)) & 3u);
+                const size_t el = (size_t)re + (size_t)ce * ny + (size_t)le * nx * ny;
+                const int m = ebe ? (int)keys[el] : (int)((key >> (2 * e)) & 3u);
                 const double *Km = K + (size_t)m * KE * KE;
+                const double sc = sdf_scale(sdf, el);
 #pragma unroll
                 for (int a = 0; a < ND; a++) {
                     double s = 0;
                     for (int t = 0; t < 4; t++) s += Km[(ND * e + a) * KE + ND * lc.S[t] + lc.comp];
-                    acc[a] -= s * lc.L;
+                    acc[a] -= (sdf.d ? sc * s : s) * lc.L;
                 }
             }
         }
@@ -933,7 +932,8 @@ __global__ void __launch_bounds__(BS) k_rhs_fluid(double *b, uint32_t nvel, uint
 template <int ND>
 __global__ void __launch_bounds__(BS) k_updateC(const double *__restrict__ x, const uint16_t *__restrict__ keys,
                                                 const double *__restrict__ CB, const double *__restrict__ seamCB,
-                                                int ebe, int ny, int nx, int nz, uint32_t N, LoadCase lc, Reduce red)
+                                                int ebe, int ny, int nx, int nz, uint32_t N, LoadCase lc, Reduce red,
+                                                Sdf sdf)
 {
     constexpr int KE = 8 * ND, NR = ND == 3 ? 6 : 3;
     double acc[6] = {0, 0, 0, 0, 0, 0};
@@ -950,6 +950,7 @@ __global__ void __launch_bounds__(BS) k_updateC(const double *__restrict__ x, co
             for (int a = 0; a < ND; a++) ue[ND * k + a] = x[ND * id + a];
         }
         const bool seam = lc.axis == 0 ? c == nx - 1 : (lc.axis == 1 ? r == 0 : l == 0);
+        const double sc = sdf_scale(sdf, i);
         for_each_uniform(m, [&](const int mu) {
             cptr<double> CBm = as_const(CB) + (size_t)mu * NR * KE;
 #pragma unroll
@@ -958,7 +959,7 @@ __global__ void __launch_bounds__(BS) k_updateC(const double *__restrict__ x, co
 #pragma unroll
                 for (int j = 0; j < KE; j++) s = fma(CBm[row * KE + j], ue[j], s);
                 if (seam) s += seamCB[mu * 6 + row];
-                acc[row] += s;
+                acc[row] += sdf.d ? sc * s : s;
             }
         });
     }
@@ -1042,11 +1043,12 @@ static inline int stream_grid(size_t n, size_t elem_bytes)
     return b < 1 ? 1 : (int)b;
 }
 
-static Reduce noreduce() { Reduce r; r.partials = nullptr; r.S = nullptr; r.fin = 0; r.slot = 0; return r; }
+static Reduce noreduce() { Reduce r; r.partials = nullptr; r.S = nullptr; r.fin = 0; r.slot = 0; r.guard = 0; return r; }
 
 template <int NV>
 __global__ void __launch_bounds__(1024) k_reduce_finish(Reduce red, unsigned nb)
 {
+    if (stopped(red)) return;
     double s[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
@@ -1062,6 +1064,10 @@ __global__ void __launch_bounds__(1024) k_reduce_finish(Reduce red, unsigned nb)
         } else if (red.fin == FIN_DELTA) {
             S[S_DELTA_OLD] = S[S_DELTA];
             S[S_DELTA] = s[0];
+            if (red.guard) {
+                S[S_ITER] += 1.0;
+                if (fabs(s[0]) <= S[S_THRESH]) S[S_DONE] = 1.0;
+            }
         } else {
 #pragma unroll
             for (int k = 0; k < NV; k++) S[red.slot + k] = s[k];
@@ -1106,8 +1112,8 @@ void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStre
     }
     if (op.ebe) {
         k_fill<T><<<grid_for(op.ndof), BS, 0, s>>>(q, T(0), op.ndof);
-        if (op.physics == PHY_THERMAL) k_ebe<T, 1><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N);
-        else k_ebe<T, 3><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N);
+        if (op.physics == PHY_THERMAL) k_ebe<T, 1><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N, op.sdf);
+        else k_ebe<T, 3><<<grid_for(op.N), BS, 0, s>>>(v, q, op.keys, K, op.ny, op.nx, op.nz, op.N, op.sdf);
         if (red) LAUNCH_RED(1, k_dot<T>, stream_grid(op.ndof, sizeof(T)), s, rr, v, q, op.ndof, rr);
         return;
     }
@@ -1127,9 +1133,9 @@ void launch_jacobi(const OpDesc &op, T *dinv, hipStream_t s)
     if (op.physics == PHY_FLUID)
         k_jacobi_fluid<T><<<grid_for(op.nvel + op.nborder), BS, 0, s>>>(op.bkeys, K, dinv, op.nvel, op.nborder);
     else if (op.physics == PHY_THERMAL)
-        k_jacobi_keys<T, 1><<<grid_for(op.N), BS, 0, s>>>(op.keys, K, dinv, op.N, op.ebe, op.ny, op.nx, op.nz);
+        k_jacobi_keys<T, 1><<<grid_for(op.N), BS, 0, s>>>(op.keys, K, dinv, op.N, op.ebe, op.ny, op.nx, op.nz, op.sdf);
     else
-        k_jacobi_keys<T, 3><<<grid_for(op.N), BS, 0, s>>>(op.keys, K, dinv, op.N, op.ebe, op.ny, op.nx, op.nz);
+        k_jacobi_keys<T, 3><<<grid_for(op.N), BS, 0, s>>>(op.keys, K, dinv, op.N, op.ebe, op.ny, op.nx, op.nz, op.sdf);
 }
 
 template <typename T>
@@ -1146,9 +1152,9 @@ void launch_precond(const T *dinv, const T *v, T *out, size_t n, const Reduce *r
 }
 
 template <typename T>
-void launch_pcg_dir(T *d, const T *z, const double *S, int first, size_t n, hipStream_t s)
+void launch_pcg_dir(T *d, const T *z, const double *S, int first, size_t n, hipStream_t s, int guard)
 {
-    k_pcg_dir<T><<<stream_grid(n, sizeof(T)), BS, 0, s>>>(d, z, S, first, n);
+    k_pcg_dir<T><<<stream_grid(n, sizeof(T)), BS, 0, s>>>(d, z, S, first, n, guard);
 }
 
 template <typename T>
@@ -1194,21 +1200,21 @@ template <typename T>
 void launch_kj_precond(const OpDesc &op, const T *v, T *out, const Reduce *red, hipStream_t s)
 {
     const int gr = group_grid(op, sizeof(T));
-    KJ_DISPATCH(k_kj_precond, gr, op.keys, op.kdiag, op.nmat, v, out, (size_t)op.N, *red);
+    KJ_DISPATCH(k_kj_precond, gr, op.keys, (const T *)op.kinv, v, out, (size_t)op.N, *red);
     launch_reduce_finish(*red, gr, 1, s);
 }
 
 template <typename T>
-void launch_kj_pcg_dir(const OpDesc &op, T *d, const T *r, const double *S, hipStream_t s)
+void launch_kj_pcg_dir(const OpDesc &op, T *d, const T *r, const double *S, hipStream_t s, int guard)
 {
-    KJ_DISPATCH(k_kj_pcg_dir, group_grid(op, sizeof(T)), d, r, op.keys, op.kdiag, op.nmat, S, (size_t)op.N);
+    KJ_DISPATCH(k_kj_pcg_dir, group_grid(op, sizeof(T)), d, r, op.keys, (const T *)op.kinv, S, (size_t)op.N, guard);
 }
 
 template <typename T>
 void launch_kj_pcg_update(const OpDesc &op, T *x, T *r, const T *d, const T *q, const Reduce *red, hipStream_t s)
 {
     const int gr = group_grid(op, sizeof(T));
-    KJ_DISPATCH(k_kj_pcg_update, gr, x, r, d, q, op.keys, op.kdiag, op.nmat, (size_t)op.N, *red);
+    KJ_DISPATCH(k_kj_pcg_update, gr, x, r, d, q, op.keys, (const T *)op.kinv, (size_t)op.N, *red);
     launch_reduce_finish(*red, gr, 1, s);
 }
 
@@ -1216,7 +1222,7 @@ template <typename T>
 void launch_kj_sub_precond(const OpDesc &op, T *r, const T *q, T *z, const Reduce *red, hipStream_t s)
 {
     const int gr = group_grid(op, sizeof(T));
-    KJ_DISPATCH(k_kj_sub_precond, gr, r, q, z, op.keys, op.kdiag, op.nmat, (size_t)op.N, *red);
+    KJ_DISPATCH(k_kj_sub_precond, gr, r, q, z, op.keys, (const T *)op.kinv, (size_t)op.N, *red);
     launch_reduce_finish(*red, gr, 1, s);
 }
 
@@ -1244,18 +1250,18 @@ void launch_rhs(const OpDesc &op, const double *K64, const LoadCase &lc, int dir
         return;
     }
     if (op.physics == PHY_THERMAL)
-        k_rhs_seam<1><<<grid_for(op.N), BS, 0, s>>>(b, op.keys, K64, op.ebe, op.ny, op.nx, op.nz, op.N, lc);
+        k_rhs_seam<1><<<grid_for(op.N), BS, 0, s>>>(b, op.keys, K64, op.ebe, op.ny, op.nx, op.nz, op.N, lc, op.sdf);
     else
-        k_rhs_seam<3><<<grid_for(op.N), BS, 0, s>>>(b, op.keys, K64, op.ebe, op.ny, op.nx, op.nz, op.N, lc);
+        k_rhs_seam<3><<<grid_for(op.N), BS, 0, s>>>(b, op.keys, K64, op.ebe, op.ny, op.nx, op.nz, op.N, lc, op.sdf);
 }
 
 void launch_updateC(const OpDesc &op, const double *CB64, const double *seamCB, const LoadCase &lc, const double *x,
                     const Reduce *red, hipStream_t s)
 {
     if (op.physics == PHY_THERMAL)
-        LAUNCH_RED(6, k_updateC<1>, grid_for(op.N), s, *red, x, op.keys, CB64, seamCB, op.ebe, op.ny, op.nx, op.nz, op.N, lc, *red);
+        LAUNCH_RED(6, k_updateC<1>, grid_for(op.N), s, *red, x, op.keys, CB64, seamCB, op.ebe, op.ny, op.nx, op.nz, op.N, lc, *red, op.sdf);
     else
-        LAUNCH_RED(6, k_updateC<3>, grid_for(op.N), s, *red, x, op.keys, CB64, seamCB, op.ebe, op.ny, op.nx, op.nz, op.N, lc, *red);
+        LAUNCH_RED(6, k_updateC<3>, grid_for(op.N), s, *red, x, op.keys, CB64, seamCB, op.ebe, op.ny, op.nx, op.nz, op.N, lc, *red, op.sdf);
 }
 
 template <typename T>
@@ -1283,7 +1289,7 @@ void launch_interpl2(const double *coarse, double *fine, int rows, int cols, int
     template void launch_fill<T>(T *, double, size_t, hipStream_t);                                            \
     template void launch_add<T>(T *, const T *, size_t, hipStream_t);                                         \
     template void launch_precond<T>(const T *, const T *, T *, size_t, const Reduce *, hipStream_t);          \
-    template void launch_pcg_dir<T>(T *, const T *, const double *, int, size_t, hipStream_t);                \
+    template void launch_pcg_dir<T>(T *, const T *, const double *, int, size_t, hipStream_t, int);           \
     template void launch_pcg_update<T>(T *, T *, const T *, T *, const T *, const Reduce *, size_t, hipStream_t); \
     template void launch_sub_precond<T>(T *, const T *, const T *, T *, const Reduce *, size_t, hipStream_t); \
     template void launch_minres_dir<T>(T *, T *, const T *, const T *, const T *, const double *, int,        \
@@ -1293,7 +1299,7 @@ void launch_interpl2(const double *coarse, double *fine, int rows, int cols, int
     template void launch_dot<T>(const T *, const T *, size_t, const Reduce *, hipStream_t);                   \
     template void launch_fluid_velocity_sum<T>(const OpDesc &, const T *, const Reduce *, hipStream_t);       \
     template void launch_kj_precond<T>(const OpDesc &, const T *, T *, const Reduce *, hipStream_t);          \
-    template void launch_kj_pcg_dir<T>(const OpDesc &, T *, const T *, const double *, hipStream_t);          \
+    template void launch_kj_pcg_dir<T>(const OpDesc &, T *, const T *, const double *, hipStream_t, int);     \
     template void launch_kj_pcg_update<T>(const OpDesc &, T *, T *, const T *, const T *, const Reduce *,     \
                                           hipStream_t);                                                       \
     template void launch_kj_sub_precond<T>(const OpDesc &, T *, const T *, T *, const Reduce *, hipStream_t);

@@ -41,6 +41,9 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
     __syncthreads();
 }
 
+// a guarded launch does nothing once its Krylov loop has converged
+__device__ __forceinline__ bool stopped(const Reduce &red) { return red.guard && red.S[S_DONE] != 0.0; }
+
 // Each block writes its NV partial sums to red.partials[k * nb + block]; the
 // launcher then enqueues the single-block finishing kernel, which adds the
 // partials in a fixed order.  No atomics or device-scope fences: on a
@@ -57,6 +60,13 @@ __device__ __forceinline__ void store_partials(const Reduce &red, double (&v)[NV
 #pragma unroll
         for (int k = 0; k < NV; k++) red.partials[(size_t)k * nb + bid] = v[k];
     }
+}
+
+// element scale of a scalar density field (1 without one); same operation
+// order as the reference (femhmg_thermal_3D.c:265)
+__device__ __forceinline__ double sdf_scale(const Sdf &f, size_t e)
+{
+    return f.d ? (double)f.d[e] * (1.0 / 65535.0) * (f.fmax - f.fmin) + f.fmin : 1.0;
 }
 
 }  // namespace chfem

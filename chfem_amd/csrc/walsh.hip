@@ -148,6 +148,7 @@ __global__ void __launch_bounds__(TR *TC) k_elastic_walsh(const T *__restrict__ 
                                                            const T *__restrict__ lamu, int ebe, int ny, int nx,
                                                            int nz, int LZ, Reduce red)
 {
+    if (DOT && stopped(red)) return;
     constexpr int NT = TR * TC, NR = TR + 1, NC = TC + 1, NV = NR * NC * 3;
     constexpr int SLOTS = (NV + NT - 1) / NT;
     constexpr int SF = 25;  // padded element stride in the force buffer (fewer LDS bank conflicts)
@@ -286,7 +287,7 @@ template <typename T>
 void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduce *red, WalshTiles t,
                           hipStream_t s)
 {
-    Reduce rr = {nullptr, nullptr, 0, 0};
+    Reduce rr = {nullptr, nullptr, 0, 0, 0};
     if (red) rr = *red;
     int LZ = t.LZ > 0 ? t.LZ : 16;
     {

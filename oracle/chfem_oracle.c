@@ -296,7 +296,24 @@ typedef struct {
     double *K;           /* dense per material: 64 (thermal), 576 (elastic); 1024 (fluid) */
     double *CB;          /* 24 per mat thermal (3x8), 144 per mat elastic, 80 fluid */
     double props[512];
+    double *esc;         /* scalar density field: per-element scale of K and CB (NULL: none) */
 } orc_model;
+
+static inline double esc_of(const orc_model *m, size_t e) { return m->esc ? m->esc[e] : 1.0; }
+
+/* Scalar density field input (chfem/src/femhmg/femhmg.c:1760-1835): every
+ * element's local matrices are those of its material times its own scale
+ * (operator, Jacobi, RHS and C alike: femhmg_thermal_3D.c:265-325,558-583,
+ * femhmg_elastic_3D.c:541-811,1382-1538).  esc = NULL removes it. */
+void orc_set_element_scale(void *h, const double *esc)
+{
+    orc_model *m = (orc_model *)h;
+    free(m->esc);
+    m->esc = NULL;
+    if (!esc) return;
+    m->esc = (double *)malloc(m->N * sizeof(double));
+    memcpy(m->esc, esc, m->N * sizeof(double));
+}
 
 void *orc_create(int analysis, unsigned nx, unsigned ny, unsigned nz, unsigned nmat,
                  const uint16_t *elem, const double *props, double voxel)
@@ -353,7 +370,7 @@ void orc_destroy(void *h)
     orc_model *m = (orc_model *)h;
     if (!m) return;
     free(m->elem); free(m->keys); free(m->fkeys); free(m->dof_id); free(m->dof2node);
-    free(m->bkeys); free(m->K); free(m->CB); free(m);
+    free(m->bkeys); free(m->K); free(m->CB); free(m->esc); free(m);
 }
 
 size_t orc_ndof(void *h) { return ((orc_model *)h)->ndof; }
@@ -446,9 +463,10 @@ static void aprod_ebe(const orc_model *m, const double *v, double *q)
             for (int a = 0; a < nd; a++) ue[nd * j + a] = v[d + a];
         }
         const double *K = m->K + (size_t)ksz * m->elem[e];
+        const double sc = esc_of(m, (size_t)e);
         for (int i = 0; i < ke; i++) {
             double t = 0; for (int j = 0; j < ke; j++) t += K[i * ke + j] * ue[j];
-            fe[(size_t)e * ke + i] = t;
+            fe[(size_t)e * ke + i] = sc * t;
         }
     }
     #pragma omp parallel for schedule(static)
@@ -509,7 +527,7 @@ void orc_apply(void *h, int strategy, const double *v, double *q)
 {
     orc_model *m = (orc_model *)h;
     if (m->analysis == ORC_FLUID) aprod_fluid(m, v, q);
-    else if (strategy == 1) aprod_ebe(m, v, q);
+    else if (strategy == 1 || m->esc) aprod_ebe(m, v, q);
     else if (m->analysis == ORC_THERMAL) aprod_thermal_nbn(m, v, q);
     else aprod_elastic_nbn(m, v, q);
 }
@@ -539,13 +557,18 @@ void orc_jacobi_inv(void *h, double *dinv)
         return;
     }
     const int nd = m->analysis == ORC_ELASTIC ? 3 : 1, ke = 8 * nd, ksz = ke * ke;
+    const long ny = m->ny, nx = m->nx, nz = m->nz;
     #pragma omp parallel for schedule(static)
     for (long i = 0; i < (long)m->N; i++) {
         unsigned key = m->keys[i];
+        long r = i % ny, c = (i / ny) % nx, l = i / (nx * ny);
         for (int a = 0; a < nd; a++) {
             double d = 0;
-            for (int e = 0; e < 8; e++)
-                d += m->K[(size_t)ksz * ((key >> (2 * e)) & 3u) + (nd * e + a) * ke + nd * e + a];
+            for (int e = 0; e < 8; e++) {
+                double kd = m->K[(size_t)ksz * ((key >> (2 * e)) & 3u) + (nd * e + a) * ke + nd * e + a];
+                if (m->esc) kd *= m->esc[pidx(r - DR[e], c - DC[e], l - DL[e], ny, nx, nz)];
+                d += kd;
+            }
             dinv[nd * i + a] = 1.0 / d;
         }
     }
@@ -609,12 +632,13 @@ void orc_rhs(void *h, int dir, double *b)
         size_t e = seam_elem(m, lc.seam_axis, i);
         long r = e % ny, c = (e / ny) % nx, l = e / (nx * ny);
         const double *K = m->K + (size_t)ksz * m->elem[e];
+        const double sc = esc_of(m, e);
         for (int k = 0; k < 8; k++) {
             size_t d = (size_t)nd * pidx(r + DR[k], c + DC[k], l + DL[k], ny, nx, nz);
             for (int a = 0; a < nd; a++) {
                 double s = 0;
                 for (int t = 0; t < 4; t++) s += K[(nd * k + a) * ke + nd * lc.S[t] + comp];
-                b[d + a] -= s * L;
+                b[d + a] -= sc * s * L;
             }
         }
     }
@@ -654,22 +678,26 @@ void orc_updateC(void *h, int dir, const double *x, double *C)
     double acc[6] = {0, 0, 0, 0, 0, 0};
     const size_t ns = seam_count(m, lc.seam_axis);
     for (size_t i = 0; i < ns; i++) {
-        const double *CB = m->CB + (size_t)cbsz * m->elem[seam_elem(m, lc.seam_axis, i)];
+        const size_t se = seam_elem(m, lc.seam_axis, i);
+        const double *CB = m->CB + (size_t)cbsz * m->elem[se];
+        const double sc = esc_of(m, se);
         for (int row = 0; row < nrow; row++)
-            for (int t = 0; t < 4; t++) acc[row] += CB[row * ke + nd * lc.S[t] + comp] * L;
+            for (int t = 0; t < 4; t++) acc[row] += sc * CB[row * ke + nd * lc.S[t] + comp] * L;
     }
     double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0;
     #pragma omp parallel for reduction(+ : a0, a1, a2, a3, a4, a5) schedule(static)
     for (long e = 0; e < (long)m->N; e++) {
         long r = e % ny, c = (e / ny) % nx, l = e / (nx * ny);
         const double *CB = m->CB + (size_t)cbsz * m->elem[e];
+        const double sc = esc_of(m, (size_t)e);
         double loc[6] = {0, 0, 0, 0, 0, 0};
         for (int k = 0; k < 8; k++) {
             size_t d = (size_t)nd * pidx(r + DR[k], c + DC[k], l + DL[k], ny, nx, nz);
             for (int a = 0; a < nd; a++)
                 for (int row = 0; row < nrow; row++) loc[row] += CB[row * ke + nd * k + a] * x[d + a];
         }
-        a0 += loc[0]; a1 += loc[1]; a2 += loc[2]; a3 += loc[3]; a4 += loc[4]; a5 += loc[5];
+        a0 += sc * loc[0]; a1 += sc * loc[1]; a2 += sc * loc[2]; a3 += sc * loc[3]; a4 += sc * loc[4];
+        a5 += sc * loc[5];
     }
     acc[0] += a0; acc[1] += a1; acc[2] += a2; acc[3] += a3; acc[4] += a4; acc[5] += a5;
     for (int row = 0; row < nrow; row++) C[row * nrow + dir] = acc[row] / (double)m->N;

@@ -57,6 +57,7 @@ def lib():
         L.orc_ke_elastic.argtypes = [dbl, dbl, i32, vp, vp]
         L.orc_ke_fluid.argtypes = [vp, vp]
         L.orc_thermal_pack40.argtypes = [vp, dbl, vp]
+        L.orc_set_element_scale.argtypes = [vp, vp]
         L.orc_interpl2.argtypes = [vp, C.c_uint, C.c_uint, C.c_uint, C.c_uint, vp]
         L.orc_coarse_elem_map.argtypes = [vp, C.c_uint, C.c_uint, C.c_uint, vp]
         L.orc_num_threads.restype = i32
@@ -116,6 +117,15 @@ class Model:
     @property
     def nborder(self):
         return lib().orc_nborder(self.h)
+
+    def set_element_scale(self, esc):
+        """Scalar density field: per-element scale of the local matrices (None removes it)."""
+        if esc is None:
+            lib().orc_set_element_scale(self.h, None)
+            return
+        self._esc = np.ascontiguousarray(esc, dtype=np.float64)
+        assert self._esc.size == self.N
+        lib().orc_set_element_scale(self.h, _p(self._esc))
 
     def keys(self):
         out = np.zeros(self.N, dtype=np.uint16)

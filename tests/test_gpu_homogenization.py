@@ -87,3 +87,80 @@ def test_rhs_and_updateC_device_vs_oracle():
             assert np.abs(C.reshape(6, 6)[:, d] - Cr.reshape(6, 6)[:, d]).max() <= 1e-12 * np.abs(Cr).max()
     finally:
         L.hmgEnd()
+
+
+def write_nf(path, analysis, props, dims, tol=1e-6, maxit=1000, data_type="float32"):
+    """.nf text (chfem/io.py:208-335 keyword layout) with a %data_type entry."""
+    lines = ["%type_of_analysis", str(analysis), "", "%type_of_solver", "0", "", "%type_of_rhs", "0", "",
+             "%voxel_size", "1.0", "", "%solver_tolerance", str(tol), "", "%number_of_iterations", str(maxit), "",
+             "%image_dimensions", " ".join(str(d) for d in dims), "", "%refinement", "1", "",
+             "%number_of_materials", str(len(props)), "", "%properties_of_materials"]
+    lines += [" ".join(str(v) for v in p) for p in props]
+    lines += ["", "%data_type", data_type, ""]
+    open(path, "w").write("\n".join(lines))
+
+
+@pytest.mark.parametrize("case", [("thermal", 0, {87: 129.0, 182: 80.4}, [(87, 1.0), (182, 1.0)], 3, 0),
+                                  ("elastic", 1, {87: 39.7, 182: 210.0}, [(87, 1.0, 0.2225), (182, 1.0, 0.3)], 6, 1)],
+                         ids=["thermal", "elastic"])
+def test_golden_sdf(case, tmp_path):
+    """Scalar density field runs of the reference's tests (test_chfem.c:63,66):
+    unit-modulus materials and a float32 field with each voxel's modulus
+    (built here from ggg40_100.raw: the reference's .bin holds exactly these
+    per-phase values), through hmgInit(nf, raw, sdf), against the golden
+    column of the plain run."""
+    from chfem_amd.lib import lib, ptr
+    name, an, field_props, mats, dim, d = case
+    raw = os.path.join(GOLDEN, "ggg40_100.raw")
+    img = np.fromfile(raw, dtype=np.uint8)
+    lut = np.zeros(256, np.float32)
+    for k, v in field_props.items():
+        lut[k] = v
+    binf = str(tmp_path / "field.bin")
+    lut[img].tofile(binf)
+    nf = str(tmp_path / "model.nf")
+    write_nf(nf, an, mats, (100, 100, 100))
+    L = lib()
+    assert L.hmgInit(nf.encode(), raw.encode(), binf.encode(), None) == 1
+    try:
+        L.hmgSetHomogenizationFlag(d)
+        L.hmgSolveHomogenization()
+        import ctypes
+        Cp = ctypes.cast(L.hmgGetConstitutiveMtx(), ctypes.POINTER(ctypes.c_double))
+        C = np.array([Cp[i] for i in range(dim * dim)]).reshape(dim, dim)
+    finally:
+        L.hmgEnd()
+    ref = np.fromfile(os.path.join(GOLDEN, f"{name}_ggg40_100_result.bin")).reshape(dim, dim)
+    assert np.abs(C[:, d] - ref[:, d]).max() / np.abs(ref).max() < 1e-4
+
+
+@pytest.mark.parametrize("kernel", ["mfma", "atomic"])
+@pytest.mark.parametrize("which", ["thermal", "elastic"])
+def test_sdf_vs_oracle(which, kernel, monkeypatch):
+    """Element-by-element operator, Jacobi, RHS and solve with a random 16-bit
+    density field against the oracle with the same element scales."""
+    from chfem_amd.lib import lib, ptr
+    if kernel == "atomic":
+        monkeypatch.setenv("CHFEM_EBE", "atomic")
+    rng = np.random.default_rng(61)
+    img = rng.choice(np.array([0, 255], dtype=np.uint8), size=(9, 17, 12))
+    if which == "thermal":
+        m = H.model(H.THERMAL, img, [1.0, 1.0])
+    else:
+        m = H.model(H.ELASTIC, img, [1.0, 0.3, 1.0, 0.2])
+    dens = rng.integers(0, 65536, size=m.N).astype(np.uint16)
+    fmin, fmax = 0.7, 12.0
+    m.set_element_scale(dens * (1.0 / 65535.0) * (fmax - fmin) + fmin)
+    s = H.GpuSolver(m, strategy=1, tol=1e-9)
+    try:
+        assert lib().cudapcgSetParametricDensityField(ptr(dens), fmin, fmax) == 1
+        v = rng.standard_normal(m.ndof)
+        assert relmax(s.apply(v), m.apply(v, strategy=1)) < 1e-12
+        assert relmax(s.precond(v), m.jacobi_inv() * v) < 1e-13
+        b = m.rhs(m.cdim - 1)
+        x, ok, st = s.solve(b)
+        xr, okr, itr, _ = m.solve(m.cdim - 1, tol=1e-9, strategy=1)
+        assert ok and okr
+        assert relmax(m.update_c(m.cdim - 1, x), m.update_c(m.cdim - 1, xr)) < 1e-7
+    finally:
+        s.close()

@@ -479,3 +479,99 @@ def test_interpl2_linear_fields():
     inner = (np.arange(lays)[:, None, None] < lays - 1) & (np.arange(cols)[None, :, None] < cols - 1) & \
             (np.arange(rows)[None, None, :] < rows - 1)
     np.testing.assert_allclose(fine[..., 1][inner], lin[inner], atol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# scalar density field (chfem/src/femhmg/femhmg.c:1760-1835)
+# ---------------------------------------------------------------------------
+def sdf_scales(field_elem, props_first, elem, quantise=True):
+    """Element scales as the reference builds them: value times the first
+    property of the element's material, quantised to 16 bits between the
+    extremes and mapped back (femhmg.c:1800-1824, femhmg_thermal_3D.c:265)."""
+    E = field_elem * props_first[elem]
+    if not quantise:
+        return E
+    emax, emin = E.max(), E.min()
+    if emax == emin:
+        emin = emax - 1.0
+    d = np.floor(65535 * (E - emin) / (emax - emin)).astype(np.uint16)
+    return d * (1.0 / 65535.0) * (emax - emin) + emin
+
+
+@pytest.mark.parametrize("analysis", [THERMAL, ELASTIC])
+def test_sdf_equals_material_properties(analysis):
+    """A field holding each voxel's modulus, on unit-modulus materials, is
+    the same model as the materials themselves."""
+    img = three_phase((4, 5, 6), 31)
+    if analysis == THERMAL:
+        props = [1.0, 10.0, 3.5]
+        unit = [1.0, 1.0, 1.0]
+        first = np.array(props)
+    else:
+        props = [1.0, 0.3, 10.0, 0.2, 4.0, 0.25]
+        unit = [1.0, 0.3, 1.0, 0.2, 1.0, 0.25]
+        first = np.array(props[0::2])
+    m = make(analysis, img, props, (0, 100, 255))
+    ms = make(analysis, img, unit, (0, 100, 255))
+    ms.set_element_scale(first[ms.elem])
+    v = np.random.default_rng(32).standard_normal(m.ndof)
+    assert rel(ms.apply(v), m.apply(v, strategy=1)) < 1e-13
+    np.testing.assert_allclose(ms.jacobi_inv(), m.jacobi_inv(), rtol=1e-13)
+    for d in range(m.cdim):
+        assert rel(ms.rhs(d), m.rhs(d)) < 1e-13
+        np.testing.assert_allclose(ms.update_c(d, v), m.update_c(d, v), rtol=1e-12, atol=1e-14)
+
+
+def test_sdf_brute_and_scaling():
+    img = three_phase((3, 4, 5), 33)
+    m = make(ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25], (0, 100, 255))
+    esc = np.random.default_rng(34).uniform(0.5, 3.0, m.N)
+    m.set_element_scale(esc)
+    Kl, _ = local(m)
+    # brute force with per-element scales
+    import scipy.sparse as sp
+    conn = B.connectivity(m.nx, m.ny, m.nz)
+    ldof = (3 * conn[:, :, None] + np.arange(3)[None, None, :]).reshape(m.N, 24)
+    vals = (Kl[m.elem] * esc[:, None, None]).reshape(m.N, -1).ravel()
+    A = sp.csr_matrix((vals, (np.repeat(ldof, 24, axis=1).ravel(), np.tile(ldof, (1, 24)).ravel())),
+                      shape=(m.ndof, m.ndof))
+    v = np.random.default_rng(35).standard_normal(m.ndof)
+    assert rel(m.apply(v), A @ v) < 1e-13
+    # a uniform field scales the effective tensor
+    m2 = make(ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25], (0, 100, 255))
+    C1 = homogenize(m2)
+    m2.set_element_scale(np.full(m2.N, 2.5))
+    C2 = homogenize(m2)
+    assert rel(C2, 2.5 * C1) < 1e-9
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("case", [("thermal", 0, {87: 129.0, 182: 80.4}, [1.0], 3),
+                                  ("elastic", 1, {87: 39.7, 182: 210.0}, None, 6)], ids=["thermal", "elastic"])
+def test_oracle_golden_sdf(case):
+    """The reference's SDF runs (test_chfem.c:63,66): unit-modulus materials
+    plus a float32 field with each voxel's modulus (the reference's
+    ggg40_100_{thermal,elastic}.bin holds exactly the per-phase values
+    129.0/80.4 and 39.7/210.0 over ggg40_100.raw), against the same golden
+    column as the plain run."""
+    name, an, field_props, _, dim = case
+    img = np.fromfile(os.path.join(GOLDEN, "ggg40_100.raw"), dtype=np.uint8).reshape(100, 100, 100)
+    lut = np.zeros(256, np.float32)
+    for k, v in field_props.items():
+        lut[k] = v
+    field = lut[img]                                   # file order, float32
+    colors = (87, 182)
+    elem = elem_of(img, colors)
+    field_elem = np.ascontiguousarray(field.transpose(0, 2, 1)).reshape(-1).astype(np.float64)
+    if an == THERMAL:
+        esc = sdf_scales(field_elem, np.array([1.0, 1.0]), elem)
+        m = O.Model(THERMAL, (100, 100, 100), np.zeros_like(elem), [1.0])
+        d = 0
+    else:
+        esc = sdf_scales(field_elem, np.array([1.0, 1.0]), elem)
+        m = O.Model(ELASTIC, (100, 100, 100), elem, [1.0, 0.2225, 1.0, 0.3])
+        d = 1
+    m.set_element_scale(esc)
+    C, it = m.homogenize(directions=[d], tol=1e-6, maxit=1000, strategy=1)
+    ref = np.fromfile(os.path.join(GOLDEN, f"{name}_ggg40_100_result.bin")).reshape(dim, dim)
+    assert np.abs(C[:, d] - ref[:, d]).max() / np.abs(ref).max() < 1e-4
