@@ -1,0 +1,328 @@
+// batch.hip -- kernels of the batched solver: many independent systems
+// (every load case of many samples of different sizes) advanced together,
+// one launch per kernel and iteration for the whole batch.
+//
+// Work is described by tables built on the host: operator tiles
+// (system, tile coordinates) -- Walsh tiles for elastic systems, node ranges
+// for thermal ones -- and vector chunks (system, node range).  Vectors of all
+// systems are concatenated; each system keeps its own FP64 scalars
+// (delta, alpha, stop flags) in a row of Sb, and its block partials sit in a
+// contiguous range of the partials buffer, summed in a fixed order by one
+// finishing block per system.  Systems whose flags say so skip their tiles
+// and chunks, so converged systems cost only the launch.
+#include <hip/hip_runtime.h>
+#include "batch.h"
+#include "nbn.h"
+#include "walsh.h"
+
+namespace chfem {
+namespace batch {
+
+#define BB 256
+
+__device__ __forceinline__ bool flag(const double *Sb, int sys, int slot) { return Sb[sys * SB_STRIDE + slot] != 0.0; }
+
+// block sum of one double; valid in thread 0
+__device__ __forceinline__ double bsum(double v)
+{
+    double a[1] = {v};
+    block_sum<1>(a);
+    return a[0];
+}
+
+// ---------------------------------------------------------------------------
+// operator: q = A v (+ per-block partials of v.q) for the systems selected
+// by `mode`: OP_ITER runs systems with ACTIVE set and DONE clear, OP_REPL
+// those with REPL set.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ bool selected(const double *Sb, int sys, int mode)
+{
+    return mode == OP_ITER ? (flag(Sb, sys, SB_ACTIVE) && !flag(Sb, sys, SB_DONE)) : flag(Sb, sys, SB_REPL);
+}
+
+template <typename T, int TR, int TC>
+__global__ void __launch_bounds__(TR *TC) k_apply_elastic(const Sys *__restrict__ sys, const Tile *__restrict__ tiles,
+                                                           const T *v, T *q, const double *__restrict__ Sb,
+                                                           double *__restrict__ part, int mode)
+{
+    const Tile t = tiles[blockIdx.x];
+    if (!selected<T>(Sb, t.sys, mode)) return;
+    const Sys &S = sys[t.sys];
+    const T *lamu = (const T *)(sizeof(T) == 4 ? S.mat32 : S.mat64);
+    const double d = walsh_tile<T, TR, TC, true>(v + S.off, q + S.off, S.keys, lamu, 0, S.ny, S.nx, S.nz, S.LZ,
+                                                  t.bx, t.by, t.bz);
+    const double s = bsum(d);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BB) k_apply_thermal(const Sys *__restrict__ sys, const Tile *__restrict__ tiles,
+                                                      const T *v, T *q, const double *__restrict__ Sb,
+                                                      double *__restrict__ part, int mode)
+{
+    const Tile t = tiles[blockIdx.x];
+    if (!selected<T>(Sb, t.sys, mode)) return;
+    const Sys &S = sys[t.sys];
+    const T *K = (const T *)(sizeof(T) == 4 ? S.mat32 : S.mat64);
+    const T *vs = v + S.off;
+    T *qs = q + S.off;
+    double acc = 0;
+    const uint32_t i = (uint32_t)t.bx * BB + threadIdx.x;  // node chunk of BB nodes
+    if (i < S.N) {
+        const T res = thermal_node<T>(vs, S.keys, K, S.ny, S.nx, S.nz, i);
+        qs[i] = res;
+        acc = (double)vs[i] * (double)res;
+    }
+    const double s = bsum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------------------
+// finishing kernels: one block per system sums its partial range
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BB) k_finish(const Sys *__restrict__ sys, double *__restrict__ Sb,
+                                               const double *__restrict__ part, int what)
+{
+    const int s = blockIdx.x;
+    double *R = Sb + s * SB_STRIDE;
+    const Sys &S = sys[s];
+    bool run;
+    if (what == FIN_B_ALPHA || what == FIN_B_DELTA) run = R[SB_ACTIVE] != 0.0 && R[SB_DONE] == 0.0;
+    else if (what == FIN_B_TRUE) run = R[SB_REPL] != 0.0;
+    else if (what == FIN_B_START) run = R[SB_START] != 0.0;
+    else run = true;
+    if (!run) return;
+    const bool op = what == FIN_B_ALPHA;
+    const int b0 = op ? S.tb0 : S.vb0, b1 = op ? S.tb1 : S.vb1;
+    double acc = 0;
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += blockDim.x) acc += part[b];
+    const double tot = bsum(acc);
+    if (threadIdx.x != 0) return;
+    if (what == FIN_B_ALPHA) {
+        R[SB_DQ] = tot;
+        R[SB_ALPHA] = R[SB_DELTA] / tot;
+    } else if (what == FIN_B_DELTA) {
+        R[SB_DELTA_OLD] = R[SB_DELTA];
+        R[SB_DELTA] = tot;
+        R[SB_ITER] += 1.0;
+        R[SB_FRESH] = 0.0;
+        if (fabs(tot) <= R[SB_THRESH]) R[SB_DONE] = 1.0;
+    } else if (what == FIN_B_TRUE) {
+        R[SB_DELTA] = tot;
+    } else if (what == FIN_B_DELTA0) {
+        R[SB_DELTA0] = tot;
+    } else {  // FIN_B_START: FP32 r.z of a fresh start
+        R[SB_DELTA] = tot;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels over chunks of BB nodes
+// ---------------------------------------------------------------------------
+template <typename T, int ND>
+__device__ __forceinline__ void node_dinv(const Sys &S, uint32_t i, T *dv)
+{
+    const T *kinv = (const T *)(sizeof(T) == 4 ? S.kinv32 : S.kinv64);
+    const unsigned key = S.keys[i];
+#pragma unroll
+    for (int a = 0; a < ND; a++) dv[a] = kinv[(size_t)key * ND + a];
+}
+
+// d = dinv r + beta d (beta = delta / delta_old; not on a system's first iteration)
+template <typename T, int ND>
+__global__ void __launch_bounds__(BB) k_dir(const Sys *__restrict__ sys, const Chunk *__restrict__ chunks,
+                                            const double *__restrict__ Sb, T *d, const T *__restrict__ r)
+{
+    const Chunk ch = chunks[blockIdx.x];
+    const double *R = Sb + ch.sys * SB_STRIDE;
+    if (R[SB_ACTIVE] == 0.0 || R[SB_DONE] != 0.0 || R[SB_FRESH] != 0.0) return;
+    const Sys &S = sys[ch.sys];
+    const uint32_t i = ch.i0 + threadIdx.x;
+    if (i >= ch.i1) return;
+    const T beta = (T)(R[SB_DELTA] / R[SB_DELTA_OLD]);
+    T dv[ND];
+    node_dinv<T, ND>(S, i, dv);
+    const size_t o = S.off + (size_t)ND * i;
+#pragma unroll
+    for (int a = 0; a < ND; a++) d[o + a] = dv[a] * r[o + a] + beta * d[o + a];
+}
+
+// x += alpha d ; r -= alpha q ; partial r.(dinv r)
+template <typename T, int ND>
+__global__ void __launch_bounds__(BB) k_update(const Sys *__restrict__ sys, const Chunk *__restrict__ chunks,
+                                               const double *__restrict__ Sb, T *x, T *r, const T *__restrict__ d,
+                                               const T *__restrict__ q, double *__restrict__ part)
+{
+    const Chunk ch = chunks[blockIdx.x];
+    const double *R = Sb + ch.sys * SB_STRIDE;
+    if (R[SB_ACTIVE] == 0.0 || R[SB_DONE] != 0.0) return;
+    const Sys &S = sys[ch.sys];
+    const uint32_t i = ch.i0 + threadIdx.x;
+    double acc = 0;
+    if (i < ch.i1) {
+        const T alpha = (T)R[SB_ALPHA];
+        T dv[ND];
+        node_dinv<T, ND>(S, i, dv);
+        const size_t o = S.off + (size_t)ND * i;
+#pragma unroll
+        for (int a = 0; a < ND; a++) {
+            x[o + a] += alpha * d[o + a];
+            const T rn = r[o + a] - alpha * q[o + a];
+            r[o + a] = rn;
+            acc += (double)rn * (double)(dv[a] * rn);
+        }
+    }
+    const double s = bsum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// start of an FP32 segment: r32 = (float) r64, d32 = dinv r32, x32 = 0,
+// partial r32.d32 (systems with `start` set)
+__global__ void __launch_bounds__(BB) k_start(const Sys *__restrict__ sys, const Chunk *__restrict__ chunks,
+                                              const double *__restrict__ Sb, const double *__restrict__ r64,
+                                              float *r32, float *d32, float *x32, double *__restrict__ part, int nd)
+{
+    const Chunk ch = chunks[blockIdx.x];
+    const double *R = Sb + ch.sys * SB_STRIDE;
+    if (R[SB_START] == 0.0) return;
+    const Sys &S = sys[ch.sys];
+    const uint32_t i = ch.i0 + threadIdx.x;
+    double acc = 0;
+    if (i < ch.i1) {
+        const float *kinv = (const float *)S.kinv32;
+        const unsigned key = S.keys[i];
+        for (int a = 0; a < nd; a++) {
+            const size_t o = S.off + (size_t)nd * i + a;
+            const float rv = (float)r64[o];
+            const float z = kinv[(size_t)key * nd + a] * rv;
+            r32[o] = rv;
+            d32[o] = z;
+            x32[o] = 0.f;
+            acc += (double)rv * (double)z;
+        }
+    }
+    const double s = bsum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// FP64 r.(dinv r) of every system (initial delta0), partials per chunk
+__global__ void __launch_bounds__(BB) k_delta0(const Sys *__restrict__ sys, const Chunk *__restrict__ chunks,
+                                               const double *__restrict__ r64, double *__restrict__ part, int nd)
+{
+    const Chunk ch = chunks[blockIdx.x];
+    const Sys &S = sys[ch.sys];
+    const uint32_t i = ch.i0 + threadIdx.x;
+    double acc = 0;
+    if (i < ch.i1) {
+        const double *kinv = (const double *)S.kinv64;
+        const unsigned key = S.keys[i];
+        for (int a = 0; a < nd; a++) {
+            const double rv = r64[S.off + (size_t)nd * i + a];
+            acc += rv * (kinv[(size_t)key * nd + a] * rv);
+        }
+    }
+    const double s = bsum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// residual replacement, part 1 (systems with REPL set): x64 += x32,
+// e64 = x32, x32 = 0
+__global__ void __launch_bounds__(BB) k_flush(const Sys *__restrict__ sys, const Chunk *__restrict__ chunks,
+                                              const double *__restrict__ Sb, double *x64, double *e64, float *x32,
+                                              int nd)
+{
+    const Chunk ch = chunks[blockIdx.x];
+    if (Sb[ch.sys * SB_STRIDE + SB_REPL] == 0.0) return;
+    const Sys &S = sys[ch.sys];
+    const uint32_t i = ch.i0 + threadIdx.x;
+    if (i >= ch.i1) return;
+    for (int a = 0; a < nd; a++) {
+        const size_t o = S.off + (size_t)nd * i + a;
+        const double e = (double)x32[o];
+        x64[o] += e;
+        e64[o] = e;
+        x32[o] = 0.f;
+    }
+}
+
+// part 2: r64 -= q64 ; partial r64.(dinv r64) ; r32 = (float) r64 (the next
+// FP32 segment continues from the FP64 residual)
+__global__ void __launch_bounds__(BB) k_resid(const Sys *__restrict__ sys, const Chunk *__restrict__ chunks,
+                                              const double *__restrict__ Sb, double *r64, const double *__restrict__ q64,
+                                              float *r32, double *__restrict__ part, int nd)
+{
+    const Chunk ch = chunks[blockIdx.x];
+    if (Sb[ch.sys * SB_STRIDE + SB_REPL] == 0.0) return;
+    const Sys &S = sys[ch.sys];
+    const uint32_t i = ch.i0 + threadIdx.x;
+    double acc = 0;
+    if (i < ch.i1) {
+        const double *kinv = (const double *)S.kinv64;
+        const unsigned key = S.keys[i];
+        for (int a = 0; a < nd; a++) {
+            const size_t o = S.off + (size_t)nd * i + a;
+            const double rn = r64[o] - q64[o];
+            r64[o] = rn;
+            r32[o] = (float)rn;
+            acc += rn * (kinv[(size_t)key * nd + a] * rn);
+        }
+    }
+    const double s = bsum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename T>
+void apply(const Batch &b, const T *v, T *q, int mode, hipStream_t s)
+{
+    if (b.ntiles == 0) return;
+    if (b.nd == 3)
+        k_apply_elastic<T, TILE_R, TILE_C><<<b.ntiles, TILE_R * TILE_C, 0, s>>>(b.sys, b.tiles, v, q, b.Sb, b.part, mode);
+    else
+        k_apply_thermal<T><<<b.ntiles, BB, 0, s>>>(b.sys, b.tiles, v, q, b.Sb, b.part, mode);
+}
+template void apply<float>(const Batch &, const float *, float *, int, hipStream_t);
+template void apply<double>(const Batch &, const double *, double *, int, hipStream_t);
+
+void finish(const Batch &b, int what, hipStream_t s)
+{
+    k_finish<<<b.nsys, BB, 0, s>>>(b.sys, b.Sb, b.part, what);
+}
+
+void dir(const Batch &b, float *d, const float *r, hipStream_t s)
+{
+    if (b.nd == 3) k_dir<float, 3><<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, d, r);
+    else k_dir<float, 1><<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, d, r);
+}
+
+void update(const Batch &b, float *x, float *r, const float *d, const float *q, hipStream_t s)
+{
+    if (b.nd == 3) k_update<float, 3><<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, x, r, d, q, b.part);
+    else k_update<float, 1><<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, x, r, d, q, b.part);
+}
+
+void start(const Batch &b, const double *r64, float *r32, float *d32, float *x32, hipStream_t s)
+{
+    k_start<<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, r64, r32, d32, x32, b.part, b.nd);
+}
+
+void delta0(const Batch &b, const double *r64, hipStream_t s)
+{
+    k_delta0<<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, r64, b.part, b.nd);
+}
+
+void flush(const Batch &b, double *x64, double *e64, float *x32, hipStream_t s)
+{
+    k_flush<<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, x64, e64, x32, b.nd);
+}
+
+void resid(const Batch &b, double *r64, const double *q64, float *r32, hipStream_t s)
+{
+    k_resid<<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, r64, q64, r32, b.part, b.nd);
+}
+
+}  // namespace batch
+}  // namespace chfem

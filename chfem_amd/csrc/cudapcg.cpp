@@ -1116,21 +1116,10 @@ cudapcgFlag_t cudapcgSetLclMtxs(cudapcgVar_t *L)
     dfree(g->kinv64);
     dfree(g->kinv32);
     if (ph != PHY_FLUID && nk <= 4) {
-        const int nd = ph == PHY_ELASTIC ? 3 : 1, ke = 8 * nd;
-        std::vector<double> t64((size_t)65536 * nd, 0.0);
-        std::vector<float> t32((size_t)65536 * nd, 0.f);
-        for (unsigned key = 0; key < 65536; key++) {
-            bool valid = true;
-            for (int e = 0; e < 8; e++) valid = valid && ((key >> (2 * e)) & 3u) < nk;
-            if (!valid) continue;
-            for (int a = 0; a < nd; a++) {
-                double sum = 0;
-                for (int e = 0; e < 8; e++)
-                    sum += K[(size_t)((key >> (2 * e)) & 3u) * ksz + (size_t)(nd * e + a) * ke + nd * e + a];
-                t64[(size_t)key * nd + a] = 1.0 / sum;
-                t32[(size_t)key * nd + a] = (float)(1.0 / sum);
-            }
-        }
+        const int nd = ph == PHY_ELASTIC ? 3 : 1;
+        std::vector<double> t64;
+        std::vector<float> t32;
+        voxel::inverse_diagonal_table(K, nk, nd, t64, t32);
         if (hipMalloc(&g->kinv64, t64.size() * sizeof(double)) != hipSuccess ||
             hipMalloc(&g->kinv32, t32.size() * sizeof(float)) != hipSuccess ||
             hipMemcpy(g->kinv64, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||

@@ -20,6 +20,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.h"
 #include "guess.h"
 #include "solver.h"
 #include "voxel.h"
@@ -1180,5 +1181,403 @@ int chfem_run(const uint8_t *img, size_t img_size, const char *nf_file, int anal
     if (alpha_out) *alpha_out = u.num_of_thermal_expansion_coeffs;
     return 0;
 }
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched solver: every load case of many samples (one analysis type, any
+// sizes) advanced together by the batch kernels (batch.hip).  Mixed
+// precision: FP32 CG with FP64 scalars and FP64 residual replacement per
+// system (the scheme of the single-system mixed solver).
+// ===========================================================================
+namespace {
+
+struct BatchSample {
+    Model *m = nullptr;
+    uint16_t *keys = nullptr;
+    void *mat32 = nullptr, *mat64 = nullptr;
+    float *kinv32 = nullptr;
+    double *kinv64 = nullptr;
+    double *K64 = nullptr, *CB64 = nullptr, *seam = nullptr;
+    std::vector<chfemSolveStats_t> stats;
+    double C[36];
+};
+
+struct BatchRun {
+    std::vector<BatchSample> samples;
+    int analysis = -1;
+    double seconds = 0;
+};
+
+template <typename T> void bfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+template <typename T> bool bupload(T *&dst, const T *src, size_t n)
+{
+    if (hipMalloc(&dst, n * sizeof(T)) != hipSuccess) return false;
+    return hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice) == hipSuccess;
+}
+
+void free_sample(BatchSample &b)
+{
+    bfree(b.keys); bfree(b.mat32); bfree(b.mat64); bfree(b.kinv32); bfree(b.kinv64);
+    bfree(b.K64); bfree(b.CB64); bfree(b.seam);
+    delete b.m;
+    b.m = nullptr;
+}
+
+OpDesc sample_op(const BatchSample &b)
+{
+    const Model &m = *b.m;
+    OpDesc o;
+    memset(&o, 0, sizeof(o));
+    o.physics = m.analysis == HMG_ELASTIC ? PHY_ELASTIC : PHY_THERMAL;
+    o.ny = (int)m.ey; o.nx = (int)m.ex; o.nz = (int)m.ez;
+    o.N = (uint32_t)m.nelem;
+    o.ndof = m.ndof;
+    o.keys = b.keys;
+    o.K = b.K64;
+    o.nmat = (int)m.nmat;
+    return o;
+}
+
+bool upload_sample(BatchSample &b)
+{
+    const Model &m = *b.m;
+    const int nd = m.analysis == HMG_ELASTIC ? 3 : 1;
+    const std::vector<double> K = dense_K(m);
+    if (!bupload(b.keys, m.keys.data(), m.keys.size()) || !bupload(b.K64, K.data(), K.size()) ||
+        !bupload(b.CB64, m.CB.data(), m.CB.size()))
+        return false;
+    if (hipMalloc(&b.seam, 6 * m.nmat * sizeof(double)) != hipSuccess) return false;
+    std::vector<double> t64;
+    std::vector<float> t32;
+    voxel::inverse_diagonal_table(K.data(), m.nmat, nd, t64, t32);
+    if (!bupload(b.kinv64, t64.data(), t64.size()) || !bupload(b.kinv32, t32.data(), t32.size())) return false;
+    if (m.analysis == HMG_ELASTIC) {
+        double lm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        float lmf[8];
+        for (unsigned t = 0; t < m.nmat; t++)
+            if (!voxel::isotropic_params(K.data() + 576 * t, lm[2 * t], lm[2 * t + 1])) return false;
+        for (int i = 0; i < 8; i++) lmf[i] = (float)lm[i];
+        double *d64 = nullptr;
+        float *d32 = nullptr;
+        if (!bupload(d64, lm, 8) || !bupload(d32, lmf, 8)) return false;
+        b.mat64 = d64;
+        b.mat32 = d32;
+    } else {
+        std::vector<float> Kf(K.begin(), K.end());
+        double *d64 = nullptr;
+        float *d32 = nullptr;
+        if (!bupload(d64, K.data(), K.size()) || !bupload(d32, Kf.data(), Kf.size())) return false;
+        b.mat64 = d64;
+        b.mat32 = d32;
+    }
+    return true;
+}
+
+std::vector<double> seam_values(const Model &m, const LoadCase &lc)
+{
+    const int nd = m.analysis == HMG_ELASTIC ? 3 : 1, ke = 8 * nd, nr = nd == 3 ? 6 : 3;
+    std::vector<double> seam(6 * m.nmat, 0.0);
+    for (unsigned t = 0; t < m.nmat; t++)
+        for (int row = 0; row < nr; row++) {
+            double sum = 0;
+            for (int k = 0; k < 4; k++) sum += m.CB[(size_t)t * nr * ke + row * ke + nd * lc.S[k] + lc.comp];
+            seam[t * 6 + row] = sum * lc.L;
+        }
+    return seam;
+}
+
+}  // namespace
+
+extern "C" {
+
+void *chfemBatchCreate(void) { return new BatchRun(); }
+
+void chfemBatchDestroy(void *h)
+{
+    BatchRun *b = (BatchRun *)h;
+    if (!b) return;
+    for (BatchSample &s : b->samples) free_sample(s);
+    delete b;
+}
+
+int chfemBatchAddSample(void *h, const char *nf_file, const uint8_t *img)
+{
+    BatchRun *b = (BatchRun *)h;
+    if (!b || !nf_file || !img) return 0;
+    Model *m = new Model();
+    memset(m->props, 0, sizeof(m->props));
+    memset(m->alpha, 0, sizeof(m->alpha));
+    memset(m->props_keys, 0, sizeof(m->props_keys));
+    memset(m->C, 0, sizeof(m->C));
+    m->nf = nf_file;
+    bool ok = read_data(*m, nf_file) && init_sizes(*m);
+    if (ok && m->analysis == HMG_FLUID) {
+        printf("ERROR: the batched solver takes thermal and elastic models.\n");
+        ok = false;
+    }
+    if (ok && m->nmat > 4) {
+        printf("ERROR: the batched solver takes at most 4 materials per sample.\n");
+        ok = false;
+    }
+    if (ok && b->analysis >= 0 && m->analysis != b->analysis) {
+        printf("ERROR: all samples of a batch share one analysis type.\n");
+        ok = false;
+    }
+    ok = ok && read_image(*m, img);
+    if (!ok) {
+        delete m;
+        return 0;
+    }
+    build_keys(*m);
+    assemble_local(*m);
+    b->analysis = m->analysis;
+    BatchSample s;
+    s.m = m;
+    if (!upload_sample(s)) {
+        free_sample(s);
+        return 0;
+    }
+    b->samples.push_back(s);
+    return 1;
+}
+
+int chfemBatchSolve(void *h, double tol, unsigned int maxit)
+{
+    using namespace batch;
+    BatchRun *B = (BatchRun *)h;
+    if (!B || B->samples.empty()) return 0;
+    const int nd = B->analysis == HMG_ELASTIC ? 3 : 1, nloads = nd == 3 ? 6 : 3;
+    const double drop = 1e-2;
+    // systems and work tables
+    std::vector<Sys> sys;
+    std::vector<Tile> tiles;
+    std::vector<Chunk> chunks;
+    std::vector<std::pair<int, int>> owner;  // (sample, load) of each system
+    size_t off = 0;
+    for (size_t si = 0; si < B->samples.size(); si++) {
+        const Model &m = *B->samples[si].m;
+        for (int d = 0; d < nloads; d++) {
+            Sys S;
+            memset(&S, 0, sizeof(S));
+            S.ny = (int)m.ey; S.nx = (int)m.ex; S.nz = (int)m.ez;
+            S.LZ = 16;
+            S.N = (uint32_t)m.nelem;
+            S.off = off;
+            S.keys = B->samples[si].keys;
+            S.mat32 = B->samples[si].mat32; S.mat64 = B->samples[si].mat64;
+            S.kinv32 = B->samples[si].kinv32; S.kinv64 = B->samples[si].kinv64;
+            const int sid = (int)sys.size();
+            S.tb0 = (int)tiles.size();
+            if (nd == 3) {
+                const int gx = (S.ny + TILE_R - 2) / (TILE_R - 1), gy = (S.nx + TILE_C - 2) / (TILE_C - 1);
+                const int gz = (S.nz + S.LZ - 1) / S.LZ;
+                for (int z = 0; z < gz; z++)
+                    for (int y = 0; y < gy; y++)
+                        for (int x = 0; x < gx; x++) tiles.push_back(Tile{sid, x, y, z});
+            } else {
+                for (uint32_t c = 0; c * 256u < S.N; c++) tiles.push_back(Tile{sid, (int)c, 0, 0});
+            }
+            S.tb1 = (int)tiles.size();
+            S.vb0 = (int)chunks.size();
+            for (uint32_t i0 = 0; i0 < S.N; i0 += 256) chunks.push_back(Chunk{sid, i0, std::min(S.N, i0 + 256u)});
+            S.vb1 = (int)chunks.size();
+            sys.push_back(S);
+            owner.push_back({(int)si, d});
+            off += (size_t)nd * m.nelem;
+        }
+    }
+    const size_t total = off;
+    const int nsys = (int)sys.size();
+    hipStream_t st = nullptr;
+    if (hipStreamCreate(&st) != hipSuccess) return 0;
+    Sys *dsys = nullptr;
+    Tile *dtiles = nullptr;
+    Chunk *dchunks = nullptr;
+    double *Sb = nullptr, *part = nullptr, *x64 = nullptr, *r64 = nullptr, *e64 = nullptr, *q64 = nullptr;
+    double *rpart = nullptr, *rS = nullptr;
+    float *x32 = nullptr, *r32 = nullptr, *d32 = nullptr, *q32 = nullptr;
+    bool ok = bupload(dsys, sys.data(), sys.size()) && bupload(dtiles, tiles.data(), tiles.size()) &&
+              bupload(dchunks, chunks.data(), chunks.size()) &&
+              hipMalloc(&Sb, sizeof(double) * nsys * SB_STRIDE) == hipSuccess &&
+              hipMalloc(&part, sizeof(double) * std::max(tiles.size(), chunks.size())) == hipSuccess &&
+              hipMalloc(&x64, total * 8) == hipSuccess && hipMalloc(&r64, total * 8) == hipSuccess &&
+              hipMalloc(&e64, total * 8) == hipSuccess && hipMalloc(&q64, total * 8) == hipSuccess &&
+              hipMalloc(&x32, total * 4) == hipSuccess && hipMalloc(&r32, total * 4) == hipSuccess &&
+              hipMalloc(&d32, total * 4) == hipSuccess && hipMalloc(&q32, total * 4) == hipSuccess &&
+              hipMalloc(&rpart, sizeof(double) * MAX_RED_GRID * 8) == hipSuccess &&
+              hipMalloc(&rS, sizeof(double) * S_COUNT) == hipSuccess;
+    Batch bt;
+    bt.nsys = nsys; bt.nd = nd; bt.sys = dsys; bt.tiles = dtiles; bt.ntiles = (int)tiles.size();
+    bt.chunks = dchunks; bt.nchunks = (int)chunks.size(); bt.Sb = Sb; bt.part = part;
+    std::vector<double> Sh((size_t)nsys * SB_STRIDE, 0.0);
+    auto down = [&]() {
+        return hipMemcpyAsync(Sh.data(), Sb, Sh.size() * 8, hipMemcpyDeviceToHost, st) == hipSuccess &&
+               hipStreamSynchronize(st) == hipSuccess;
+    };
+    auto up = [&]() {
+        return hipMemcpyAsync(Sb, Sh.data(), Sh.size() * 8, hipMemcpyHostToDevice, st) == hipSuccess &&
+               hipStreamSynchronize(st) == hipSuccess;
+    };
+    auto R = [&](int s, int slot) -> double & { return Sh[(size_t)s * SB_STRIDE + slot]; };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ok = ok && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+    std::vector<double> d0v(nsys, 0.0), delta_rel(nsys, 0.0);
+    std::vector<unsigned> repl(nsys, 0);
+    std::vector<int> conv(nsys, 0);
+    if (ok) {
+        hipEventRecord(e0, st);
+        // right-hand sides (FP64) and x = 0
+        for (int s = 0; s < nsys; s++) {
+            BatchSample &bs = B->samples[owner[s].first];
+            const LoadCase lc = load_case(*bs.m, (unsigned)owner[s].second);
+            launch_rhs(sample_op(bs), bs.K64, lc, owner[s].second, r64 + sys[s].off, st);
+        }
+        launch_fill<double>(x64, 0.0, total, st);
+        delta0(bt, r64, st);
+        for (int s = 0; s < nsys; s++) R(s, SB_ACTIVE) = 0.0;
+        ok = up();
+        finish(bt, FIN_B_DELTA0, st);
+        ok = ok && down();
+        // fresh FP32 start of every system with a non-zero right-hand side
+        for (int s = 0; s < nsys; s++) {
+            d0v[s] = R(s, SB_DELTA0);
+            delta_rel[s] = d0v[s];
+            const bool live = fabs(d0v[s]) >= 1e-12;
+            conv[s] = live ? 0 : 1;
+            R(s, SB_ACTIVE) = live;
+            R(s, SB_START) = live;
+            R(s, SB_FRESH) = 1.0;
+            R(s, SB_DONE) = 0.0;
+            R(s, SB_REPL) = 0.0;
+            R(s, SB_ITER) = 0.0;
+            R(s, SB_THRESH) = std::max(tol * tol * fabs(d0v[s]), drop * fabs(d0v[s]));
+        }
+        ok = ok && up();
+        start(bt, r64, r32, d32, x32, st);
+        finish(bt, FIN_B_START, st);
+        ok = ok && down();
+        for (int s = 0; s < nsys; s++) R(s, SB_START) = 0.0;
+        ok = ok && up();
+    }
+    const unsigned nb = 16;
+    while (ok) {
+        int live = 0;
+        for (int s = 0; s < nsys; s++) live += R(s, SB_ACTIVE) != 0.0;
+        if (!live) break;
+        for (unsigned k = 0; k < nb; k++) {
+            dir(bt, d32, r32, st);
+            apply<float>(bt, d32, q32, OP_ITER, st);
+            finish(bt, FIN_B_ALPHA, st);
+            update(bt, x32, r32, d32, q32, st);
+            finish(bt, FIN_B_DELTA, st);
+        }
+        if (!(ok = down())) break;
+        int nrepl = 0;
+        for (int s = 0; s < nsys; s++) {
+            R(s, SB_REPL) = 0.0;
+            if (R(s, SB_ACTIVE) == 0.0) continue;
+            if (R(s, SB_DONE) != 0.0 || R(s, SB_ITER) >= maxit) {
+                R(s, SB_REPL) = 1.0;
+                nrepl++;
+            }
+        }
+        if (!nrepl) continue;
+        if (!(ok = up())) break;
+        // residual replacement of the selected systems
+        flush(bt, x64, e64, x32, st);
+        apply<double>(bt, e64, q64, OP_REPL, st);
+        resid(bt, r64, q64, r32, st);
+        finish(bt, FIN_B_TRUE, st);
+        if (!(ok = down())) break;
+        for (int s = 0; s < nsys; s++) {
+            if (R(s, SB_REPL) == 0.0) continue;
+            R(s, SB_REPL) = 0.0;
+            repl[s]++;
+            const double dl = R(s, SB_DELTA);
+            if (fabs(dl) <= tol * tol * fabs(d0v[s])) {
+                conv[s] = 1;
+                R(s, SB_ACTIVE) = 0.0;
+            } else if (R(s, SB_ITER) >= maxit || !(fabs(dl) < fabs(delta_rel[s]))) {
+                R(s, SB_ACTIVE) = 0.0;  // iteration cap, or no progress left in FP32
+            } else {
+                delta_rel[s] = dl;
+                R(s, SB_DONE) = 0.0;
+                R(s, SB_THRESH) = std::max(tol * tol * fabs(d0v[s]), drop * fabs(dl));
+            }
+        }
+        ok = up();
+    }
+    float ms = 0;
+    if (ok) {
+        hipEventRecord(e1, st);
+        ok = hipEventSynchronize(e1) == hipSuccess;
+        hipEventElapsedTime(&ms, e0, e1);
+    }
+    B->seconds = ms * 1e-3;
+    // effective tensors (FP64, on the device)
+    for (BatchSample &bs : B->samples) {
+        memset(bs.C, 0, sizeof(bs.C));
+        bs.stats.assign(nloads, chfemSolveStats_t());
+    }
+    std::vector<double> Shr(S_COUNT);
+    for (int s = 0; ok && s < nsys; s++) {
+        BatchSample &bs = B->samples[owner[s].first];
+        const Model &m = *bs.m;
+        const int d = owner[s].second;
+        const LoadCase lc = load_case(m, (unsigned)d);
+        const std::vector<double> seam = seam_values(m, lc);
+        ok = hipMemcpy(bs.seam, seam.data(), seam.size() * 8, hipMemcpyHostToDevice) == hipSuccess;
+        Reduce red = {rpart, rS, FIN_STORE, S_TMP0, 0};
+        launch_updateC(sample_op(bs), bs.CB64, bs.seam, lc, x64 + sys[s].off, &red, st);
+        ok = ok && hipMemcpyAsync(Shr.data(), rS, S_COUNT * 8, hipMemcpyDeviceToHost, st) == hipSuccess &&
+             hipStreamSynchronize(st) == hipSuccess;
+        const int nr = nd == 3 ? 6 : 3;
+        for (int row = 0; row < nr; row++) bs.C[row * nr + d] = Shr[S_TMP0 + row] / (double)m.nelem;
+        chfemSolveStats_t &stt = bs.stats[d];
+        stt.iterations = (unsigned)R(s, SB_ITER);
+        stt.sweeps = repl[s];
+        stt.solve_seconds = B->seconds;
+        stt.residual = sqrt(fabs(R(s, SB_DELTA)) / std::max(fabs(d0v[s]), 1e-300));
+        stt.x0_residual = 0;
+        stt.x0_used = 0;
+    }
+    int all = 1;
+    for (int s = 0; s < nsys; s++) all &= conv[s];
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    bfree(dsys); bfree(dtiles); bfree(dchunks); bfree(Sb); bfree(part); bfree(x64); bfree(r64); bfree(e64);
+    bfree(q64); bfree(x32); bfree(r32); bfree(d32); bfree(q32); bfree(rpart); bfree(rS);
+    (void)hipStreamDestroy(st);
+    if (!ok) return 0;
+    return all ? 1 : 2;
+}
+
+int chfemBatchGetC(void *h, unsigned int sample, double *C)
+{
+    BatchRun *b = (BatchRun *)h;
+    if (!b || sample >= b->samples.size() || !C) return 0;
+    const unsigned n = b->analysis == HMG_ELASTIC ? 6 : 3;
+    memcpy(C, b->samples[sample].C, n * n * sizeof(double));
+    return 1;
+}
+
+int chfemBatchGetStats(void *h, unsigned int sample, chfemSolveStats_t *stats, unsigned int *count)
+{
+    BatchRun *b = (BatchRun *)h;
+    if (!b || sample >= b->samples.size()) return 0;
+    const std::vector<chfemSolveStats_t> &v = b->samples[sample].stats;
+    if (count) *count = (unsigned)v.size();
+    if (stats)
+        for (size_t i = 0; i < v.size(); i++) stats[i] = v[i];
+    return 1;
+}
+
+double chfemBatchSeconds(void *h) { return h ? ((BatchRun *)h)->seconds : -1.0; }
 
 }  // extern "C"

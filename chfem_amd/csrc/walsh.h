@@ -1,0 +1,287 @@
+// walsh.h -- elastic operator q = A v for isotropic materials, computed
+// element by element in the Walsh (hierarchical) basis of the trilinear voxel
+// (device code shared by walsh.hip and the batched solver, batch.hip).
+//
+// For an element with nodal displacements u^a_k (component a, local node k)
+// the Walsh sums W^a_s = sum_k s_k u^a_k over the sign patterns
+// s in {x, y, z, xy, yz, xz, xyz} of the local nodes determine the
+// displacement gradient polynomial exactly.  Because 1, x-1/2, y-1/2, ... are
+// orthogonal on the unit cube, the element force vector int sigma(u):grad N_k
+// has Walsh coefficients F^a_s that are short linear combinations of the W^a_s
+// (lambda, mu weighted), and f^a_k = sum_s F^a_s s_k.  Each element costs
+// about 180 flops instead of the 24x24 product's 1152, with the same result
+// (the local matrix is lambda*A + mu*B for the unit-voxel element;
+// chfem/src/femhmg/physical_phenomena/femhmg_elastic_3D.c:51-128).
+//
+// No atomics; node results are deterministic.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "kernels.h"
+#include "reduce.h"
+
+namespace chfem {
+
+// local node k of an element sits at (row + DR, col + DC, layer + DL)
+#define WDR(k) ((k) == 0 || (k) == 1 || (k) == 4 || (k) == 5 ? 1 : 0)
+#define WDC(k) ((k) == 1 || (k) == 2 || (k) == 5 || (k) == 6 ? 1 : 0)
+#define WDL(k) ((k) >= 4 ? 1 : 0)
+// elastic element frame: x = col offset, y = 1 - row offset, z = 1 - layer offset
+// node index inside the 2x2x2 cube [zb][yb][xb]
+#define CUBE(xb, yb, zb) ((zb) == 1 ? ((yb) == 1 ? ((xb) == 1 ? 2 : 3) : ((xb) == 1 ? 1 : 0)) \
+                                    : ((yb) == 1 ? ((xb) == 1 ? 6 : 7) : ((xb) == 1 ? 5 : 4)))
+
+template <typename T>
+struct Walsh { T x, y, z, xy, yz, xz, xyz; };
+
+// forward: Walsh sums of 8 nodal values (local node order)
+template <typename T>
+__device__ __forceinline__ Walsh<T> wht(const T (&u)[8])
+{
+    // x stage: pairs (xb=1) - (xb=0) and (xb=1) + (xb=0) for each (yb, zb)
+    T a[2][2], b[2][2];
+#pragma unroll
+    for (int zb = 0; zb < 2; zb++)
+#pragma unroll
+        for (int yb = 0; yb < 2; yb++) {
+            const T u1 = u[CUBE(1, yb, zb)], u0 = u[CUBE(0, yb, zb)];
+            a[zb][yb] = u1 + u0;
+            b[zb][yb] = u1 - u0;
+        }
+    T as[2], ad[2], bs[2], bd[2];
+#pragma unroll
+    for (int zb = 0; zb < 2; zb++) {
+        as[zb] = a[zb][1] + a[zb][0];
+        ad[zb] = a[zb][1] - a[zb][0];
+        bs[zb] = b[zb][1] + b[zb][0];
+        bd[zb] = b[zb][1] - b[zb][0];
+    }
+    Walsh<T> w;
+    w.z = as[1] - as[0];
+    w.y = ad[1] + ad[0];
+    w.yz = ad[1] - ad[0];
+    w.x = bs[1] + bs[0];
+    w.xz = bs[1] - bs[0];
+    w.xy = bd[1] + bd[0];
+    w.xyz = bd[1] - bd[0];
+    return w;
+}
+
+// inverse: f_k = sum_s F_s s_k (no constant mode)
+template <typename T>
+__device__ __forceinline__ void iwht(const Walsh<T> &F, T (&f)[8])
+{
+#pragma unroll
+    for (int yb = 0; yb < 2; yb++) {
+        const T sy = yb ? T(1) : T(-1);
+        const T A = F.x + sy * F.xy, B = F.z + sy * F.yz, C = F.xz + sy * F.xyz;
+#pragma unroll
+        for (int xb = 0; xb < 2; xb++) {
+            const T sx = xb ? T(1) : T(-1);
+            const T P = sx * A + sy * F.y, Q = B + sx * C;
+            f[CUBE(xb, yb, 1)] = P + Q;
+            f[CUBE(xb, yb, 0)] = P - Q;
+        }
+    }
+}
+
+// element forces from nodal displacements u[k][a]; lam, mu of the element
+template <typename T>
+__device__ __forceinline__ void element_forces(const T (&u)[8][3], T lam, T mu, T (&f)[8][3])
+{
+    T c[8];
+    Walsh<T> W[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) c[k] = u[k][a];
+        W[a] = wht(c);
+    }
+    const T l2 = lam + 2 * mu, l3 = lam + 3 * mu, l4 = lam + 4 * mu;
+    const T r16 = T(1.0 / 16.0), r48 = T(1.0 / 48.0), r144 = T(1.0 / 144.0);
+    const T tr = W[0].x + W[1].y + W[2].z;
+    const T S = W[0].yz + W[1].xz + W[2].xy;
+    Walsh<T> F[3];
+    // component x
+    F[0].x = (lam * tr + 2 * mu * W[0].x) * r16;
+    F[0].y = mu * (W[0].y + W[1].x) * r16;
+    F[0].z = mu * (W[0].z + W[2].x) * r16;
+    F[0].xy = (l3 * W[0].xy + lam * W[2].yz) * r48;
+    F[0].xz = (l3 * W[0].xz + lam * W[1].yz) * r48;
+    F[0].yz = mu * (S + W[0].yz) * r48;
+    F[0].xyz = l4 * W[0].xyz * r144;
+    // component y
+    F[1].x = F[0].y;
+    F[1].y = (lam * tr + 2 * mu * W[1].y) * r16;
+    F[1].z = mu * (W[1].z + W[2].y) * r16;
+    F[1].xy = (l3 * W[1].xy + lam * W[2].xz) * r48;
+    F[1].yz = (l3 * W[1].yz + lam * W[0].xz) * r48;
+    F[1].xz = mu * (S + W[1].xz) * r48;
+    F[1].xyz = l4 * W[1].xyz * r144;
+    // component z
+    F[2].x = F[0].z;
+    F[2].y = F[1].z;
+    F[2].z = (lam * tr + 2 * mu * W[2].z) * r16;
+    F[2].xz = (l3 * W[2].xz + lam * W[1].xy) * r48;
+    F[2].yz = (l3 * W[2].yz + lam * W[0].xy) * r48;
+    F[2].xy = mu * (S + W[2].xy) * r48;
+    F[2].xyz = l4 * W[2].xyz * r144;
+    (void)l2;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        iwht(F[a], c);
+#pragma unroll
+        for (int k = 0; k < 8; k++) f[k][a] = c[k];
+    }
+}
+
+__device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// Work decomposition: a block of TR x TC work-items evaluates a TR x TC
+// patch of elements per element layer and outputs the (TR-1) x (TC-1) nodes
+// interior to the patch (work-item (tr, tc), tr, tc >= 1, owns node
+// (r0 + tr - 1, c0 + tc - 1)).  Every work-item evaluates exactly one element
+// per layer step.  The block marches through LZ node layers; node layers stream
+// through two LDS buffers, the next layer's values are fetched into registers
+// while the current layer's elements are evaluated, and the far-face forces of
+// each node are carried to the next step in registers.
+//
+// walsh_tile computes the tile (bx, by, bz) of one operator application and
+// returns this work-item's share of v.q (DOT); k_elastic_walsh maps tiles to
+// blocks one to one, the batched solver (batch.hip) through a tile table.
+template <typename T, int TR, int TC, bool DOT>
+__device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restrict__ q,
+                                             const uint16_t *__restrict__ keys, const T *__restrict__ lamu, int ebe,
+                                             int ny, int nx, int nz, int LZ, int bx, int by, int bz)
+{
+    constexpr int NT = TR * TC, NR = TR + 1, NC = TC + 1, NV = NR * NC * 3;
+    constexpr int SLOTS = (NV + NT - 1) / NT;
+    constexpr int SF = 25;  // padded element stride in the force buffer (fewer LDS bank conflicts)
+    __shared__ T su[2][NV];
+    __shared__ T sf[NT * SF];
+    __shared__ T smat[8];
+    const int tid = threadIdx.x;
+    const int tr = tid % TR, tc = tid / TR;
+    const int r0 = bx * (TR - 1), c0 = by * (TC - 1), l0 = bz * LZ;
+    const int vr = min(TR - 1, ny - r0), vc = min(TC - 1, nx - c0), vl = min(LZ, nz - l0);
+    const size_t nxy = (size_t)nx * ny;
+    if (tid < 8) smat[tid] = lamu[tid];
+
+    // node-layer load pattern: LDS index idx = (nc * NR + nr) * 3 + comp holds node
+    // (r0 - 1 + nr, c0 - 1 + nc); the in-layer global offset is the same for every layer
+    uint32_t goff[SLOTS];
+    bool gval[SLOTS];
+#pragma unroll
+    for (int k = 0; k < SLOTS; k++) {
+        const int idx = tid + k * NT;
+        const int comp = idx % 3, t2 = idx / 3, nr = t2 % NR, nc = t2 / NR;
+        gval[k] = idx < NV && nr <= vr + 1 && nc <= vc + 1;
+        const int gr = wrapi(r0 - 1 + nr, ny), gc = wrapi(c0 - 1 + nc, nx);
+        goff[k] = gval[k] ? 3u * ((uint32_t)gc * (uint32_t)ny + (uint32_t)gr) + (uint32_t)comp : 0u;
+    }
+    T pre[SLOTS];
+    auto fetch = [&](int lg) {
+        const T *base = v + 3 * (size_t)wrapi(lg, nz) * nxy;
+#pragma unroll
+        for (int k = 0; k < SLOTS; k++) pre[k] = gval[k] ? base[goff[k]] : T(0);
+    };
+    auto store = [&](T *dst) {
+#pragma unroll
+        for (int k = 0; k < SLOTS; k++)
+            if (gval[k]) dst[tid + k * NT] = pre[k];
+    };
+    // element of this work-item: origin (r0 - 1 + tr, c0 - 1 + tc)
+    const bool elem_ok = tr <= vr && tc <= vc;
+    const uint32_t eoff = (uint32_t)wrapi(c0 - 1 + tc, nx) * (uint32_t)ny + (uint32_t)wrapi(r0 - 1 + tr, ny);
+    auto material = [&](int le) -> int {
+        if (!elem_ok) return 0;
+        // NBN keys: node (r,c,l) is local node 3 of element (r,c,l); EBE: element map
+        const unsigned kv = keys[(size_t)wrapi(le, nz) * nxy + eoff];
+        return ebe ? (int)(kv & 3u) : (int)((kv >> 6) & 3u);
+    };
+    auto element = [&](const T *cur, const T *nxt, int m) {
+        T u[8][3], f[8][3];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const T *src = WDL(k) ? nxt : cur;
+            const int o = ((tc + WDC(k)) * NR + tr + WDR(k)) * 3;
+            u[k][0] = src[o];
+            u[k][1] = src[o + 1];
+            u[k][2] = src[o + 2];
+        }
+        element_forces(u, smat[2 * m], smat[2 * m + 1], f);
+        T *dst = sf + tid * SF;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            dst[3 * k] = f[k][0];
+            dst[3 * k + 1] = f[k][1];
+            dst[3 * k + 2] = f[k][2];
+        }
+    };
+    const bool own = tr >= 1 && tc >= 1 && tr - 1 < vr && tc - 1 < vc;
+    // force slot of the element for which this node is local node k
+    auto fslot = [&](int k) { return ((tc - WDC(k)) * TR + (tr - WDR(k))) * SF + 3 * k; };
+
+    T *cur = su[0], *nxt = su[1];
+    fetch(l0 - 1);
+    store(cur);
+    fetch(l0);
+    store(nxt);
+    int m = material(l0 - 1);
+    __syncthreads();
+    fetch(l0 + 1);
+    if (elem_ok) element(cur, nxt, m);
+    m = material(l0);
+    __syncthreads();
+    T acc[3] = {0, 0, 0};
+    if (own) {
+#pragma unroll
+        for (int k = 4; k < 8; k++) {
+            const T *sp = sf + fslot(k);
+            acc[0] += sp[0]; acc[1] += sp[1]; acc[2] += sp[2];
+        }
+    }
+    store(cur);  // layer l0 + 1 replaces layer l0 - 1
+    {
+        T *t = cur; cur = nxt; nxt = t;  // cur = layer l0, nxt = layer l0 + 1
+    }
+    __syncthreads();
+    double dsum = 0.0;
+    for (int s = 0; s < vl; s++) {
+        const int l = l0 + s;
+        const bool more = s + 1 < vl;
+        if (more) fetch(l + 2);
+        T own_u[3] = {0, 0, 0};
+        if (DOT && own) {
+            const T *uu = cur + (tc * NR + tr) * 3;
+            own_u[0] = uu[0]; own_u[1] = uu[1]; own_u[2] = uu[2];
+        }
+        if (elem_ok) element(cur, nxt, m);
+        if (more) m = material(l + 1);
+        __syncthreads();
+        if (own) {
+            T o[3] = {acc[0], acc[1], acc[2]};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const T *sp = sf + fslot(k);
+                o[0] += sp[0]; o[1] += sp[1]; o[2] += sp[2];
+            }
+            acc[0] = acc[1] = acc[2] = 0;
+#pragma unroll
+            for (int k = 4; k < 8; k++) {
+                const T *sp = sf + fslot(k);
+                acc[0] += sp[0]; acc[1] += sp[1]; acc[2] += sp[2];
+            }
+            T *qo = q + 3 * ((size_t)l * nxy + (size_t)(c0 + tc - 1) * ny + (r0 + tr - 1));
+            qo[0] = o[0]; qo[1] = o[1]; qo[2] = o[2];
+            if (DOT) dsum += (double)own_u[0] * o[0] + (double)own_u[1] * o[1] + (double)own_u[2] * o[2];
+        }
+        if (more) {
+            store(cur);  // layer l + 2 replaces layer l
+            T *t = cur; cur = nxt; nxt = t;
+        }
+        __syncthreads();
+    }
+    return dsum;
+}
+
+}  // namespace chfem

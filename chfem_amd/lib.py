@@ -126,6 +126,13 @@ SIGNATURES = [
     ("chfemGuessMfmaCheck", C.c_int, [vp, vp, vp]),
     ("chfemMfmaCheckF", C.c_int, [vp, vp, vp, vp, vp, vp]),
     ("cudapcgGetOperatorTile", u8, [u8, vp]),
+    ("chfemBatchCreate", vp, []),
+    ("chfemBatchDestroy", None, [vp]),
+    ("chfemBatchAddSample", C.c_int, [vp, cp, vp]),
+    ("chfemBatchSolve", C.c_int, [vp, dbl, uint]),
+    ("chfemBatchGetC", C.c_int, [vp, uint, vp]),
+    ("chfemBatchGetStats", C.c_int, [vp, uint, vp, C.POINTER(C.c_uint)]),
+    ("chfemBatchSeconds", dbl, [vp]),
 ]
 
 

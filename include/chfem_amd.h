@@ -129,6 +129,30 @@ int chfemGuessAccumulate(void *net, const uint16_t *elem, const double *props, u
                          unsigned int ny, unsigned int nz, const float *target, double *loss);
 int chfemGuessStep(void *net, float lr, float grad_scale);
 int hmgSetInitialGuessNetwork(void *net);
+/* Batched solver (no reference counterpart; the reference solves one model
+ * and one load case at a time, chfem/src/femhmg/femhmg.c:848-1198).  Every
+ * load case of every sample added is one system; all systems advance
+ * together, each kernel of a CG step being one launch over a table of
+ * operator tiles and vector chunks of all systems (batch.hip).  FP32 CG with
+ * FP64 scalars and per-system FP64 residual replacement, Jacobi from the
+ * packed keys, stop rule delta <= tol^2 delta0 per system (FP64).
+ *   chfemBatchAddSample: a .nf file (thermal or elastic, isotropic, <= 4
+ *     materials; every sample of a batch has the same analysis type) and its
+ *     uint8 image in file order; returns 1 on success
+ *   chfemBatchSolve: returns 1 when every system converged, 2 when some
+ *     did not (iteration cap or FP32 stagnation), 0 on failure
+ *   chfemBatchGetC: effective matrix of a sample (row-major n x n)
+ *   chfemBatchGetStats: per load case iterations, FP64 replacements and
+ *     final relative residual
+ *   chfemBatchSeconds: device time of the last solve (RHS to convergence) */
+void *chfemBatchCreate(void);
+void chfemBatchDestroy(void *batch);
+int chfemBatchAddSample(void *batch, const char *nf_file, const uint8_t *img);
+int chfemBatchSolve(void *batch, double tol, unsigned int maxit);
+int chfemBatchGetC(void *batch, unsigned int sample, double *C);
+int chfemBatchGetStats(void *batch, unsigned int sample, chfemSolveStats_t *stats, unsigned int *count);
+double chfemBatchSeconds(void *batch);
+
 /* BF16 matrix-core lane-layout check: D = A * B for 16x16 row-major BF16
  * A and B, FP32 D. */
 int chfemGuessMfmaCheck(const uint16_t *A, const uint16_t *B, float *D);
