@@ -269,7 +269,7 @@ static int make_ops()
     const bool fixed_tile = tile_env && strcmp(tile_env, "auto") != 0;
     if (fixed_tile) sscanf(tile_env, "%dx%dx%d", &o.tile_tr, &o.tile_tc, &o.tile_lz);
     const char *opsel = getenv("CHFEM_OPERATOR");
-    const bool use_walsh = g->walsh_ok && !(opsel && !strcmp(opsel, "generic"));
+    bool use_walsh = g->walsh_ok && !(opsel && !strcmp(opsel, "generic"));
     // key-Jacobi vector kernels: NBN thermal/elastic with the Jacobi
     // preconditioner on (CHFEM_VECTOR=stored keeps the stored-dinv kernels)
     // scalar density field: element-by-element operator only (the reference
@@ -284,6 +284,7 @@ static int make_ops()
         o.sdf.d = m->parametric_density_field;
         o.sdf.fmin = m->limits_density_field[0];
         o.sdf.fmax = m->limits_density_field[1];
+        use_walsh = false;  // the Walsh operator has no per-element scale
     }
     const char *vsel = getenv("CHFEM_VECTOR");
     const bool kj = g->kinv64 && g->precond && !o.ebe && o.physics != PHY_FLUID && !(vsel && !strcmp(vsel, "stored"));

@@ -45,7 +45,9 @@ def test_batch_homogeneous_and_tiny():
     a 1-voxel-thick sample in the same batch."""
     from tests.brute import isotropic_C
     a = np.zeros((6, 5, 7), np.uint8)
-    b = synthetic.spheres((3, 9, 1), rmin=1, rmax=2, fraction=0.3, seed=5)
+    b = np.zeros((3, 9, 1), np.uint8)
+    b[1, 2:5, 0] = 255
+    b[0, 7, 0] = 255
     Cs, info = batch.batch_homogenize([(a, {0: (2.0, 0.25)}), (b, {0: (1.0, 0.3), 255: (3.0, 0.3)})],
                                       "elasticity", tol=1e-9)
     assert info["converged"]
