@@ -170,6 +170,25 @@ static void timed_apply(const OpDesc &op, const T *v, T *q, const Reduce *red)
     }
 }
 
+// fused direction update in the CG loops (CHFEM_FUSE_DIR=0 keeps the separate
+// direction kernel)
+static bool fuse_dir_enabled()
+{
+    const char *e = getenv("CHFEM_FUSE_DIR");
+    return e ? atoi(e) != 0 : true;
+}
+
+template <typename T>
+static void timed_apply_dir(const OpDesc &op, const T *r, const T *dold, T *dnew, T *q, const Reduce *red)
+{
+    if (g->timing) hipEventRecord(g->opA, g->stream);
+    launch_apply_dir<T>(op, r, dold, dnew, q, red, g->stream);
+    if (g->timing) {
+        hipEventRecord(g->opB, g->stream);
+        op_pending = true;
+    }
+}
+
 static int sync_loop()
 {
     if (!sync_scalars()) return 0;
@@ -217,19 +236,18 @@ static cudapcgFlag_t alloc_solver_arrays()
     if (!g || !g->model) return CUDAPCG_FALSE;
     const size_t n = nvars();
     const size_t b8 = n * sizeof(double), b4 = n * sizeof(float);
-    const bool minres = g->solver_flag == CUDAPCG_MINRES_SOLVER || g->solver_flag == CUDAPCG_MINRES3_SOLVER ||
-                        g->solver_flag == CUDAPCG_MINRES2_SOLVER;
     if (!g->x64) CHECK_HIP(hipMalloc(&g->x64, b8));
     if (!g->d64) CHECK_HIP(hipMalloc(&g->d64, b8));
     if (!g->q64) CHECK_HIP(hipMalloc(&g->q64, b8));
-    if (minres && !g->s64) CHECK_HIP(hipMalloc(&g->s64, b8));
+    // s: MINRES's A M^-1 r, or CG's second direction buffer (fused direction update)
+    if (!g->s64) CHECK_HIP(hipMalloc(&g->s64, b8));
     if (!g->dinv64) CHECK_HIP(hipMalloc(&g->dinv64, b8));
     if (g->precision == CHFEM_PREC_MIXED) {
         if (!g->x32) CHECK_HIP(hipMalloc(&g->x32, b4));
         if (!g->r32) CHECK_HIP(hipMalloc(&g->r32, b4));
         if (!g->d32) CHECK_HIP(hipMalloc(&g->d32, b4));
         if (!g->q32) CHECK_HIP(hipMalloc(&g->q32, b4));
-        if (minres && !g->s32) CHECK_HIP(hipMalloc(&g->s32, b4));
+        if (!g->s32) CHECK_HIP(hipMalloc(&g->s32, b4));
         if (!g->dinv32) CHECK_HIP(hipMalloc(&g->dinv32, b4));
     }
     const size_t pn = (size_t)MAX_PARTIAL_BLOCKS * 8;
@@ -313,20 +331,30 @@ static int make_ops()
 // block, node layers per block) on the model's own grid in each precision the
 // solve uses and keeps the fastest.  Runs once per model, before its first
 // solve; CHFEM_WALSH_TILE=TRxTCxLZ fixes the tile instead.
+static bool is_minres();
 static void autotune_tiles()
 {
     if (!g->autotune || g->tuned) return;
     static const int cand[][3] = {{32, 8, 16}, {32, 8, 32}, {16, 16, 16}, {32, 16, 16}, {32, 16, 32}, {64, 4, 16}};
     const int nc = (int)(sizeof(cand) / sizeof(cand[0]));
     const size_t n = nvars();
-    auto tune = [&](OpDesc &op, auto *d, auto *q, int *best) {
+    // the CG loops' form of the operator: with the fused direction update
+    // (reads r and d_old, writes d), else the plain apply
+    const bool fused = fuse_dir_enabled() && !is_minres() && g->s64;
+    Reduce rt = make_reduce(FIN_STORE, S_TMP0);
+    auto tune = [&](OpDesc &op, auto *d, auto *q, auto *r, auto *d2, int *best) {
         using T = typename std::remove_pointer<decltype(d)>::type;
+        const bool fz = fused && d2 && apply_dir_ok(op);
+        auto run = [&]() {
+            if (fz) launch_apply_dir<T>(op, r, d, d2, q, &rt, g->stream);
+            else launch_apply<T>(op, d, q, nullptr, g->stream);
+        };
         float best_ms = 1e30f;
         for (int c = 0; c < nc; c++) {
             op.tile_tr = cand[c][0]; op.tile_tc = cand[c][1]; op.tile_lz = cand[c][2];
-            launch_apply<T>(op, d, q, nullptr, g->stream);
+            run();
             hipEventRecord(g->ev0, g->stream);
-            for (int r = 0; r < 3; r++) launch_apply<T>(op, d, q, nullptr, g->stream);
+            for (int k = 0; k < 3; k++) run();
             hipEventRecord(g->ev1, g->stream);
             if (hipEventSynchronize(g->ev1) != hipSuccess) return;
             float ms = 0;
@@ -338,11 +366,13 @@ static void autotune_tiles()
         }
         op.tile_tr = best[0]; op.tile_tc = best[1]; op.tile_lz = best[2];
     };
+    // r64 holds the right-hand side here and is only read
     launch_fill<double>(g->d64, 1.0, n, g->stream);
-    tune(g->dm.op64, g->d64, g->q64, g->tile64);
+    tune(g->dm.op64, g->d64, g->q64, g->r64, g->s64, g->tile64);
     if (g->precision == CHFEM_PREC_MIXED && g->d32) {
         launch_fill<float>(g->d32, 1.0, n, g->stream);
-        tune(g->dm.op32, g->d32, g->q32, g->tile32);
+        launch_fill<float>(g->r32, 1.0, n, g->stream);
+        tune(g->dm.op32, g->d32, g->q32, g->r32, g->s32, g->tile32);
     } else {
         memcpy(g->tile32, g->tile64, sizeof(g->tile32));
     }
@@ -369,13 +399,18 @@ static void note_x0(double delta, double delta0)
 // With op.kinv (thermal/elastic NBN) the Jacobi diagonal comes from the keys
 // and z = M^-1 r is folded into the direction update (d = M^-1 r + beta d);
 // otherwise z is stored in q and the diagonal read from dinv.
+// With d2 and an operator that supports it (apply_dir_ok), the direction
+// update runs inside the operator: iteration j writes its direction to
+// dbuf[j % 2] from r and dbuf[(j - 1) % 2].
 template <typename T>
 static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool x0set, double tol, unsigned maxit,
-                   unsigned &it, double &residual, double *delta0_out)
+                   unsigned &it, double &residual, double *delta0_out, T *d2)
 {
     const size_t n = op.ndof;
     hipStream_t st = g->stream;
     const bool kj = op.kinv != nullptr;
+    const bool fused = kj && d2 && fuse_dir_enabled() && apply_dir_ok(op);
+    T *dbuf[2] = {d, d2};
     if (!x0set) launch_fill<T>(x, 0.0, n, st);
     Reduce r0 = make_reduce(FIN_STORE, S_DELTA0);
     if (kj) launch_kj_precond<T>(op, r, d, &r0, st);
@@ -424,7 +459,16 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
     while (!done && it < maxit) {
         const unsigned nb = std::min(cg_batch(), maxit - it);
         for (unsigned b = 0; b < nb; b++) {
-            if (it + b > 0) {
+            const unsigned j = it + b;
+            if (fused) {
+                T *dn = dbuf[j & 1];
+                const T *dold = j > 0 ? dbuf[(j - 1) & 1] : nullptr;
+                if (b == 0) timed_apply_dir<T>(op, r, dold, dn, q, &ra);
+                else launch_apply_dir<T>(op, r, dold, dn, q, &ra, st);
+                launch_kj_pcg_update<T>(op, x, r, dn, q, &rdl, st);
+                continue;
+            }
+            if (j > 0) {
                 if (kj) launch_kj_pcg_dir<T>(op, d, r, g->S, st, 1);
                 else launch_pcg_dir<T>(d, q, g->S, 0, n, st, 1);
             }
@@ -545,6 +589,11 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
     const bool kj = o32.kinv != nullptr;
     float *x = g->x32, *r = g->r32, *d = g->d32, *q = g->q32;
     const float *dinv = g->dinv32;
+    // direction update inside the FP32 operator; iteration j (counted over
+    // the whole solve, the direction is kept across replacements) writes
+    // dbuf[j % 2]
+    const bool fused = kj && g->s32 && fuse_dir_enabled() && apply_dir_ok(o32);
+    float *dbuf[2] = {d, g->s32};
     Reduce r0 = make_reduce(FIN_STORE, S_DELTA0), rd = make_reduce(FIN_STORE, S_DELTA);
     launch_precond<double>(g->dinv64, g->r64, nullptr, n, &r0, st);
     if (!sync_scalars()) return -1;
@@ -595,7 +644,16 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
         while (!done && base + seg < maxit) {
             const unsigned nb = std::min(cg_batch(), maxit - base - seg);
             for (unsigned b = 0; b < nb; b++) {
-                if (base + seg + b > 0) {
+                const unsigned j = base + seg + b;
+                if (fused) {
+                    float *dn = dbuf[j & 1];
+                    const float *dold = j > 0 ? dbuf[(j - 1) & 1] : nullptr;
+                    if (b == 0) timed_apply_dir<float>(o32, r, dold, dn, q, &ra);
+                    else launch_apply_dir<float>(o32, r, dold, dn, q, &ra, st);
+                    launch_kj_pcg_update<float>(o32, x, r, dn, q, &rdl, st);
+                    continue;
+                }
+                if (j > 0) {
                     if (kj) launch_kj_pcg_dir<float>(o32, d, r, g->S, st, 1);
                     else launch_pcg_dir<float>(d, q, g->S, 0, n, st, 1);
                 }
@@ -675,7 +733,7 @@ static int run_mixed(bool x0set, unsigned &it_total, unsigned &sweeps, double &r
                                    g->max_iterations - it_total, it, res_in);
         else
             rc = run_pcg<float>(o32, g->x32, g->r32, g->d32, g->q32, g->dinv32, false, tau,
-                                g->max_iterations - it_total, it, res_in, nullptr);
+                                g->max_iterations - it_total, it, res_in, nullptr, g->s32);
         if (rc < 0) return -1;
         it_total += it;
         sweeps++;
@@ -719,7 +777,7 @@ int solve_device(const double *x0_dev)
                                 g->max_iterations, it, res);
     } else {
         rc = run_pcg<double>(g->dm.op64, g->x64, g->r64, g->d64, g->q64, g->dinv64, x0set, g->num_tol,
-                             g->max_iterations, it, res, nullptr);
+                             g->max_iterations, it, res, nullptr, g->s64);
     }
     hipEventRecord(g->ev1, g->stream);
     hipEventSynchronize(g->ev1);
@@ -1241,7 +1299,24 @@ double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps, cudapcgF
     }
     Reduce red = make_reduce(FIN_STORE, S_TMP0);
     const Reduce *rp = fused_dot ? &red : nullptr;
+    const bool dir = fused_dot == 2;
+    if (dir) {
+        // CG-loop form: r = x, directions alternate between d and s
+        if (!g->precond || !apply_dir_ok(precision == CHFEM_PREC_MIXED ? g->dm.op32 : g->dm.op64)) return -1.0;
+        if (!set_scalar(S_DELTA_OLD, 1.0) || !set_scalar(S_DELTA, 0.5)) return -1.0;
+        launch_fill<double>(g->s64, 1.0, n, g->stream);
+        if (precision == CHFEM_PREC_MIXED) launch_fill<float>(g->s32, 1.0, n, g->stream);
+    }
     auto once = [&](unsigned k) {
+        if (dir) {
+            if (precision == CHFEM_PREC_MIXED)
+                launch_apply_dir<float>(g->dm.op32, g->x32, k & 1 ? g->s32 : g->d32, k & 1 ? g->d32 : g->s32, g->q32,
+                                        &red, g->stream);
+            else
+                launch_apply_dir<double>(g->dm.op64, g->x64, k & 1 ? g->s64 : g->d64, k & 1 ? g->d64 : g->s64, g->q64,
+                                         &red, g->stream);
+            return;
+        }
         if (precision == CHFEM_PREC_MIXED) launch_apply<float>(g->dm.op32, k & 1 ? g->x32 : g->d32, g->q32, rp, g->stream);
         else launch_apply<double>(g->dm.op64, k & 1 ? g->x64 : g->d64, g->q64, rp, g->stream);
     };

@@ -154,7 +154,18 @@ namespace chfem {
 // per material {lambda, mu}.  Filled when every local matrix equals
 // lambda*A + mu*B for the unit-voxel trilinear element.
 struct WalshTiles { int TR, TC, LZ; };
+template <typename T> struct DirFuse;
+// fz non-null: v is d_old and the launch forms d = dinv r + beta d_old
+// (walsh.h DirFuse), writes it to fz->dnew and reduces d.q (red required)
 template <typename T>
 void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduce *red, WalshTiles t,
-                          hipStream_t s);
+                          hipStream_t s, const DirFuse<T> *fz = nullptr);
+
+// Jacobi-PCG step with the direction update inside the operator (isotropic
+// elastic NBN with key-Jacobi, see apply_dir_ok): d = dinv r + beta d_old
+// (beta = S[DELTA]/S[DELTA_OLD]; dold == nullptr: d = dinv r), dnew <- d,
+// q = A d, red <- d.q.  dnew must not alias dold, r or q.
+bool apply_dir_ok(const OpDesc &op);
+template <typename T>
+void launch_apply_dir(const OpDesc &op, const T *r, const T *dold, T *dnew, T *q, const Reduce *red, hipStream_t s);
 }  // namespace chfem

@@ -136,6 +136,20 @@ __device__ __forceinline__ void element_forces(const T (&u)[8][3], T lam, T mu, 
 
 __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
+// Direction update fused into the operator (Jacobi-PCG with key-Jacobi):
+// the operator's input d = dinv r + beta d_old is formed while the node
+// planes are fetched (dinv from the node's packed key, kinv[key * 3 + a]),
+// q = A d is computed from it, and d is written to dnew for the nodes the
+// block owns.  v is d_old; first: d = dinv r (v unused).
+template <typename T>
+struct DirFuse {
+    const T *r;
+    const T *kinv;
+    T *dnew;
+    T beta;
+    int first;
+};
+
 // Work decomposition: a block of TR x TC work-items evaluates a TR x TC
 // patch of elements per element layer and outputs the (TR-1) x (TC-1) nodes
 // interior to the patch (work-item (tr, tc), tr, tc >= 1, owns node
@@ -148,15 +162,22 @@ __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >
 // walsh_tile computes the tile (bx, by, bz) of one operator application and
 // returns this work-item's share of v.q (DOT); k_elastic_walsh maps tiles to
 // blocks one to one, the batched solver (batch.hip) through a tile table.
-template <typename T, int TR, int TC, bool DOT>
+// FUSE: the input is the fused direction update of DirFuse (the dot is then
+// d.q with the updated d).
+template <typename T, int TR, int TC, bool DOT, bool FUSE = false>
 __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restrict__ q,
                                              const uint16_t *__restrict__ keys, const T *__restrict__ lamu, int ebe,
-                                             int ny, int nx, int nz, int LZ, int bx, int by, int bz)
+                                             int ny, int nx, int nz, int LZ, int bx, int by, int bz,
+                                             DirFuse<T> fz = DirFuse<T>{})
 {
     constexpr int NT = TR * TC, NR = TR + 1, NC = TC + 1, NV = NR * NC * 3;
     constexpr int SLOTS = (NV + NT - 1) / NT;
     constexpr int SF = 25;  // padded element stride in the force buffer (fewer LDS bank conflicts)
-    __shared__ T su[2][NV];
+    // node planes padded to SLOTS * NT entries: every work-item stores every
+    // slot, so the plane loads and stores need no per-slot branches (a
+    // branch around a load makes the compiler wait for all loads in flight
+    // at its join)
+    __shared__ T su[2][SLOTS * NT];
     __shared__ T sf[NT * SF];
     __shared__ T smat[8];
     const int tid = threadIdx.x;
@@ -167,35 +188,76 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
     if (tid < 8) smat[tid] = lamu[tid];
 
     // node-layer load pattern: LDS index idx = (nc * NR + nr) * 3 + comp holds node
-    // (r0 - 1 + nr, c0 - 1 + nc); the in-layer global offset is the same for every layer
+    // (r0 - 1 + nr, c0 - 1 + nc); the in-layer global offset is the same for every
+    // layer.  Slots outside the tile's nodes load node 0 of the layer (in bounds);
+    // what they store to LDS is never read.
     uint32_t goff[SLOTS];
-    bool gval[SLOTS];
 #pragma unroll
     for (int k = 0; k < SLOTS; k++) {
         const int idx = tid + k * NT;
         const int comp = idx % 3, t2 = idx / 3, nr = t2 % NR, nc = t2 / NR;
-        gval[k] = idx < NV && nr <= vr + 1 && nc <= vc + 1;
+        const bool ok = idx < NV && nr <= vr + 1 && nc <= vc + 1;
         const int gr = wrapi(r0 - 1 + nr, ny), gc = wrapi(c0 - 1 + nc, nx);
-        goff[k] = gval[k] ? 3u * ((uint32_t)gc * (uint32_t)ny + (uint32_t)gr) + (uint32_t)comp : 0u;
+        goff[k] = ok ? 3u * ((uint32_t)gc * (uint32_t)ny + (uint32_t)gr) + (uint32_t)comp : 0u;
     }
+    // layer index modulo nz for any lg (the prefetches run up to the last layer + 2)
+    auto wrapl = [&](int lg) {
+        int lw = lg % nz;
+        return lw < 0 ? lw + nz : lw;
+    };
     T pre[SLOTS];
-    auto fetch = [&](int lg) {
-        const T *base = v + 3 * (size_t)wrapi(lg, nz) * nxy;
+    // FUSE: the direction update's operands are fetched with the plane and
+    // combined when the plane is stored to LDS.  The packed keys are loaded
+    // one layer ahead (kn), so the dependent kinv gather of a layer does not
+    // wait on its key load.  On the first iteration (no d_old) the d_old
+    // operand reads r and beta is 0.
+    T pr[FUSE ? SLOTS : 1], pm[FUSE ? SLOTS : 1];
+    uint32_t kn[FUSE ? SLOTS : 1];
+    const T *dsrc = v;
+    T beta = T(0);
+    if constexpr (FUSE) {
+        dsrc = fz.first ? fz.r : v;
+        beta = fz.first ? T(0) : fz.beta;
+    }
+    auto load_keys = [&](int lg) {
+        if constexpr (FUSE) {
+            const uint16_t *kb = keys + (size_t)wrapl(lg) * nxy;
 #pragma unroll
-        for (int k = 0; k < SLOTS; k++) pre[k] = gval[k] ? base[goff[k]] : T(0);
+            for (int k = 0; k < SLOTS; k++) kn[k] = kb[goff[k] / 3u];
+        }
+    };
+    auto fetch = [&](int lg) {
+        const size_t lo = 3 * (size_t)wrapl(lg) * nxy;
+        if constexpr (FUSE) {
+#pragma unroll
+            for (int k = 0; k < SLOTS; k++) {
+                const uint32_t o = goff[k];
+                pm[k] = fz.kinv[3u * kn[k] + (o - 3u * (o / 3u))];
+                pr[k] = fz.r[lo + o];
+                pre[k] = dsrc[lo + o];
+            }
+            load_keys(lg + 1);
+        } else {
+#pragma unroll
+            for (int k = 0; k < SLOTS; k++) pre[k] = v[lo + goff[k]];
+        }
     };
     auto store = [&](T *dst) {
 #pragma unroll
-        for (int k = 0; k < SLOTS; k++)
-            if (gval[k]) dst[tid + k * NT] = pre[k];
+        for (int k = 0; k < SLOTS; k++) {
+            if constexpr (FUSE) dst[tid + k * NT] = pm[k] * pr[k] + beta * pre[k];
+            else dst[tid + k * NT] = pre[k];
+        }
     };
-    // element of this work-item: origin (r0 - 1 + tr, c0 - 1 + tc)
+    // element of this work-item: origin (r0 - 1 + tr, c0 - 1 + tc).  Its
+    // material is read from the packed keys one layer ahead (mat_key) and
+    // decoded when needed (material).
     const bool elem_ok = tr <= vr && tc <= vc;
     const uint32_t eoff = (uint32_t)wrapi(c0 - 1 + tc, nx) * (uint32_t)ny + (uint32_t)wrapi(r0 - 1 + tr, ny);
-    auto material = [&](int le) -> int {
-        if (!elem_ok) return 0;
+    auto mat_key = [&](int le) -> uint32_t { return keys[(size_t)wrapl(le) * nxy + eoff]; };
+    auto material = [&](uint32_t kv) -> int {
         // NBN keys: node (r,c,l) is local node 3 of element (r,c,l); EBE: element map
-        const unsigned kv = keys[(size_t)wrapi(le, nz) * nxy + eoff];
+        if (!elem_ok) return 0;
         return ebe ? (int)(kv & 3u) : (int)((kv >> 6) & 3u);
     };
     auto element = [&](const T *cur, const T *nxt, int m) {
@@ -222,15 +284,17 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
     auto fslot = [&](int k) { return ((tc - WDC(k)) * TR + (tr - WDR(k))) * SF + 3 * k; };
 
     T *cur = su[0], *nxt = su[1];
+    load_keys(l0 - 1);
     fetch(l0 - 1);
+    uint32_t mk = mat_key(l0 - 1);
     store(cur);
     fetch(l0);
     store(nxt);
-    int m = material(l0 - 1);
+    int m = material(mk);
+    mk = mat_key(l0);
     __syncthreads();
     fetch(l0 + 1);
     if (elem_ok) element(cur, nxt, m);
-    m = material(l0);
     __syncthreads();
     T acc[3] = {0, 0, 0};
     if (own) {
@@ -241,6 +305,7 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
         }
     }
     store(cur);  // layer l0 + 1 replaces layer l0 - 1
+    m = material(mk);
     {
         T *t = cur; cur = nxt; nxt = t;  // cur = layer l0, nxt = layer l0 + 1
     }
@@ -248,16 +313,25 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
     double dsum = 0.0;
     for (int s = 0; s < vl; s++) {
         const int l = l0 + s;
-        const bool more = s + 1 < vl;
-        if (more) fetch(l + 2);
+        // loads for the next steps are issued first and consumed after this
+        // layer's elements: element materials of layer l + 1, plane l + 2
+        // (materials first: the wait for the plane then covers them, and no
+        // wait is left for after this layer's stores).  The last step issues
+        // them too (in bounds, unused): every step then has the same loads
+        // and none is pending across the loop's back edge.
+        mk = mat_key(l + 1);
+        fetch(l + 2);
         T own_u[3] = {0, 0, 0};
-        if (DOT && own) {
+        if ((DOT || FUSE) && own) {
             const T *uu = cur + (tc * NR + tr) * 3;
             own_u[0] = uu[0]; own_u[1] = uu[1]; own_u[2] = uu[2];
         }
         if (elem_ok) element(cur, nxt, m);
-        if (more) m = material(l + 1);
         __syncthreads();
+        // the prefetched plane goes to LDS before this layer's node results
+        // are stored, so waiting for the loads does not wait for the stores
+        store(cur);  // layer l + 2 replaces layer l
+        m = material(mk);
         if (own) {
             T o[3] = {acc[0], acc[1], acc[2]};
 #pragma unroll
@@ -271,12 +345,16 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
                 const T *sp = sf + fslot(k);
                 acc[0] += sp[0]; acc[1] += sp[1]; acc[2] += sp[2];
             }
-            T *qo = q + 3 * ((size_t)l * nxy + (size_t)(c0 + tc - 1) * ny + (r0 + tr - 1));
+            const size_t on = 3 * ((size_t)l * nxy + (size_t)(c0 + tc - 1) * ny + (r0 + tr - 1));
+            T *qo = q + on;
             qo[0] = o[0]; qo[1] = o[1]; qo[2] = o[2];
+            if (FUSE) {
+                T *dn = fz.dnew + on;
+                dn[0] = own_u[0]; dn[1] = own_u[1]; dn[2] = own_u[2];
+            }
             if (DOT) dsum += (double)own_u[0] * o[0] + (double)own_u[1] * o[1] + (double)own_u[2] * o[2];
         }
-        if (more) {
-            store(cur);  // layer l + 2 replaces layer l
+        {
             T *t = cur; cur = nxt; nxt = t;
         }
         __syncthreads();

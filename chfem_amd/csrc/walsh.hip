@@ -5,15 +5,24 @@
 
 namespace chfem {
 
-template <typename T, int TR, int TC, bool DOT>
+template <typename T, int TR, int TC, bool DOT, bool FUSE>
 __global__ void __launch_bounds__(TR *TC) k_elastic_walsh(const T *__restrict__ v, T *__restrict__ q,
                                                            const uint16_t *__restrict__ keys,
                                                            const T *__restrict__ lamu, int ebe, int ny, int nx,
-                                                           int nz, int LZ, Reduce red)
+                                                           int nz, int LZ, Reduce red, DirFuse<T> fz)
 {
     if (DOT && stopped(red)) return;
-    const double dsum = walsh_tile<T, TR, TC, DOT>(v, q, keys, lamu, ebe, ny, nx, nz, LZ, blockIdx.x, blockIdx.y,
-                                                    blockIdx.z);
+    // beta of the fused direction update, as k_kj_pcg_dir forms it
+    if (FUSE && !fz.first) fz.beta = (T)(red.S[S_DELTA] / red.S[S_DELTA_OLD]);
+    // 1-D grid; blocks are dispatched to the 8 XCDs round-robin (block b to
+    // XCD b % 8), so each XCD is given one contiguous range of tiles (rows of
+    // tiles fastest): tiles that share halo nodes then run on the same XCD at
+    // about the same time and meet each other's halo lines in its L2.
+    const unsigned gx = (unsigned)(ny + TR - 2) / (TR - 1), gy = (unsigned)(nx + TC - 2) / (TC - 1);
+    const unsigned b = blockIdx.x, nb = gridDim.x, per = nb / 8u, rem = nb % 8u, xcd = b % 8u, k = b / 8u;
+    const unsigned t = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + k;
+    const double dsum = walsh_tile<T, TR, TC, DOT, FUSE>(v, q, keys, lamu, ebe, ny, nx, nz, LZ, (int)(t % gx),
+                                                          (int)((t / gx) % gy), (int)(t / (gx * gy)), fz);
     if (DOT) {
         double vv[1] = {dsum};
         store_partials<1>(red, vv);
@@ -22,10 +31,12 @@ __global__ void __launch_bounds__(TR *TC) k_elastic_walsh(const T *__restrict__ 
 
 template <typename T>
 void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduce *red, WalshTiles t,
-                          hipStream_t s)
+                          hipStream_t s, const DirFuse<T> *fz)
 {
     Reduce rr = {nullptr, nullptr, 0, 0, 0};
     if (red) rr = *red;
+    DirFuse<T> f = {nullptr, nullptr, nullptr, T(0), 1};
+    if (fz) f = *fz;
     int LZ = t.LZ > 0 ? t.LZ : 16;
     {
         // keep the block count within the reduction partials buffer
@@ -35,13 +46,18 @@ void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, con
     }
 #define WALSH_LAUNCH(TRv, TCv)                                                                          \
     {                                                                                                   \
-        dim3 grid((op.ny + TRv - 2) / (TRv - 1), (op.nx + TCv - 2) / (TCv - 1), (op.nz + LZ - 1) / LZ);             \
-        if (red) {                                                                                      \
-            k_elastic_walsh<T, TRv, TCv, true><<<grid, TRv * TCv, 0, s>>>(v, q, op.keys, lamu, op.ebe,     \
-                                                                          op.ny, op.nx, op.nz, LZ, rr);    \
-            launch_reduce_finish(rr, grid.x * grid.y * grid.z, 1, s);                                   \
-        } else k_elastic_walsh<T, TRv, TCv, false><<<grid, TRv * TCv, 0, s>>>(v, q, op.keys, lamu, op.ebe,  \
-                                                                            op.ny, op.nx, op.nz, LZ, rr);      \
+        const unsigned grid = (unsigned)((op.ny + TRv - 2) / (TRv - 1)) * ((op.nx + TCv - 2) / (TCv - 1)) *    \
+                              ((op.nz + LZ - 1) / LZ);                                                  \
+        if (fz) {                                                                                       \
+            k_elastic_walsh<T, TRv, TCv, true, true><<<grid, TRv * TCv, 0, s>>>(                        \
+                v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);                           \
+            launch_reduce_finish(rr, grid, 1, s);                                   \
+        } else if (red) {                                                                               \
+            k_elastic_walsh<T, TRv, TCv, true, false><<<grid, TRv * TCv, 0, s>>>(                       \
+                v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);                           \
+            launch_reduce_finish(rr, grid, 1, s);                                   \
+        } else k_elastic_walsh<T, TRv, TCv, false, false><<<grid, TRv * TCv, 0, s>>>(                   \
+                v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);                           \
         return;                                                                                         \
     }
     if (t.TR == 16 && t.TC == 16) WALSH_LAUNCH(16, 16)
@@ -51,9 +67,27 @@ void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, con
 #undef WALSH_LAUNCH
 }
 
+bool apply_dir_ok(const OpDesc &op)
+{
+    // the operators launch_apply sends to the Walsh kernel, with key-Jacobi on
+    return op.physics == PHY_ELASTIC && !op.ebe && op.lamu && op.kinv;
+}
+
+template <typename T>
+void launch_apply_dir(const OpDesc &op, const T *r, const T *dold, T *dnew, T *q, const Reduce *red, hipStream_t s)
+{
+    const DirFuse<T> f = {r, (const T *)op.kinv, dnew, T(0), dold ? 0 : 1};
+    const WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
+    launch_elastic_walsh<T>(op, (const T *)op.lamu, dold, q, red, t, s, &f);
+}
+
+template void launch_apply_dir<float>(const OpDesc &, const float *, const float *, float *, float *, const Reduce *,
+                                      hipStream_t);
+template void launch_apply_dir<double>(const OpDesc &, const double *, const double *, double *, double *,
+                                       const Reduce *, hipStream_t);
 template void launch_elastic_walsh<float>(const OpDesc &, const float *, const float *, float *, const Reduce *,
-                                          WalshTiles, hipStream_t);
+                                          WalshTiles, hipStream_t, const DirFuse<float> *);
 template void launch_elastic_walsh<double>(const OpDesc &, const double *, const double *, double *, const Reduce *,
-                                           WalshTiles, hipStream_t);
+                                           WalshTiles, hipStream_t, const DirFuse<double> *);
 
 }  // namespace chfem

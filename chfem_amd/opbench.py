@@ -16,6 +16,9 @@ from .lib import lib, ptr
 
 PHYS = {"thermal": 0, "elastic": 1}
 BYTES = {("thermal", 0): 17.0, ("thermal", 1): 9.0, ("elastic", 0): 49.0, ("elastic", 1): 25.0}
+# CG-loop form with the direction update inside the operator: image, r and
+# the previous direction read, the direction and q written
+BYTES_DIR = {0: 1.0 + 4 * 24.0, 1: 1.0 + 4 * 12.0}
 
 
 def main():
@@ -27,6 +30,9 @@ def main():
     ap.add_argument("--strategy", type=int, default=0)
     ap.add_argument("--copy", action="store_true", help="also time the calibration copy kernel")
     ap.add_argument("--dot", action="store_true", help="with the fused v.Av reduction, as inside the solver")
+    ap.add_argument("--dir", action="store_true",
+                    help="the CG loop's form: direction update d = M^-1 r + beta d inside the operator (elastic)")
+    ap.add_argument("--both", action="store_true", help="time the --dot form, then the --dir form")
     a = ap.parse_args()
     n = a.size
     img = synthetic.spheres((n, n, n), rmin=6, rmax=12, fraction=0.3, seed=3)
@@ -40,11 +46,16 @@ def main():
     L.hmgSetParallelStrategyFlag(a.strategy)
     L.hmgSetPrecisionFlag(prec)
     assert L.hmgPrepare() == 1
-    ms = L.cudapcgOperatorBench(prec, a.reps, 1 if a.dot else 0)
     N = n ** 3
-    out = {"physics": a.physics, "size": n, "precision": a.precision, "strategy": a.strategy, "fused_dot": a.dot,
-           "avg_launch_ms": ms, "algorithmic_bytes": BYTES[(a.physics, prec)] * N,
-           "GBps": BYTES[(a.physics, prec)] * N / (ms * 1e-3) / 1e9}
+    out = {"physics": a.physics, "size": n, "precision": a.precision, "strategy": a.strategy}
+    forms = ["dot", "dir"] if a.both else ["dir" if a.dir else ("dot" if a.dot else "plain")]
+    for form in forms:
+        ms = L.cudapcgOperatorBench(prec, a.reps, {"plain": 0, "dot": 1, "dir": 2}[form])
+        if ms < 0:
+            raise SystemExit("operator bench failed (the direction form needs isotropic elastic NBN)")
+        per_node = BYTES_DIR[prec] if form == "dir" else BYTES[(a.physics, prec)]
+        out[form] = {"avg_launch_ms": ms, "algorithmic_bytes": per_node * N,
+                     "GBps": per_node * N / (ms * 1e-3) / 1e9}
     L.hmgRelease()
     L.hmgEnd()
     if a.copy:

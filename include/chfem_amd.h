@@ -175,9 +175,12 @@ cudapcgFlag_t hmgGetKernelTiming(double *op_ms_total, unsigned int *op_launches)
 /* Operator micro-benchmark on the current solver (after cudapcgAllocateArrays
  * or hmgPrepare): q = A v for reps launches alternating between two input
  * vectors, precision CHFEM_PREC_FP64 or CHFEM_PREC_MIXED (FP32 operator),
- * with the solver's fused v.Av reduction when fused_dot is set; returns the
- * mean launch time in ms from HIP events on the solver's stream, or a
- * negative value on failure. */
+ * with the solver's fused v.Av reduction when fused_dot is 1; fused_dot 2
+ * times the CG loop's form instead: the direction update d = M^-1 r + beta d
+ * inside the operator (isotropic elastic NBN with the Jacobi preconditioner;
+ * reads r and the previous direction, writes d and q).  Returns the mean
+ * launch time in ms from HIP events on the solver's stream, or a negative
+ * value on failure. */
 double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps, cudapcgFlag_t fused_dot);
 
 /* Calibration kernel for hardware-counter traffic: a grid-stride FP64 copy of

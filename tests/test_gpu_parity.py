@@ -266,3 +266,37 @@ def test_ebe_kernels(kernel, which, nphase, monkeypatch):
                 assert ok
         finally:
             s.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("fuse", ["1", "0"])
+def test_fused_direction_update(fuse, precision, monkeypatch):
+    """The CG direction update d = M^-1 r + beta d done inside the Walsh
+    operator (two alternating direction buffers; CHFEM_FUSE_DIR=1, the
+    default) and by the separate direction kernel (0), on a grid of several
+    blocks in every direction, from the zero guess and from a non-zero guess,
+    in FP64 and mixed precision (FP64 residual replacements keep the
+    direction), against the oracle."""
+    monkeypatch.setenv("CHFEM_FUSE_DIR", fuse)
+    monkeypatch.setenv("CHFEM_WALSH_TILE", "16x16x4")
+    img = three_phase((21, 34, 19), 51)
+    m = H.model(H.ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25], colors=(0, 100, 255))
+    xr, okr, itr, _ = m.solve(2, tol=1e-10)
+    assert okr
+    Cr = m.update_c(2, xr)
+    s = H.GpuSolver(m, precision=precision, tol=1e-10 if precision == 0 else 1e-9)
+    try:
+        x, ok, st = s.solve(m.rhs(2))
+        assert ok
+        assert rel(m.update_c(2, x), Cr) < 1e-7
+        if precision == 0:
+            assert rel(x, xr) < 1e-7
+            assert abs(int(st.iterations) - itr) <= 2
+        else:
+            assert st.sweeps >= 2
+        x1, ok1, st1 = s.solve(m.rhs(2), x0=0.5 * xr)
+        assert ok1 and st1.x0_used == 1
+        assert rel(m.update_c(2, x1), Cr) < 1e-7
+        assert st1.iterations < st.iterations + 2
+    finally:
+        s.close()
