@@ -179,10 +179,11 @@ static bool fuse_dir_enabled()
 }
 
 template <typename T>
-static void timed_apply_dir(const OpDesc &op, const T *r, const T *dold, T *dnew, T *q, const Reduce *red)
+static void timed_apply_dir(const OpDesc &op, const T *r, const T *dold, T *dnew, T *q, T *x, int xpend,
+                            const Reduce *red)
 {
     if (g->timing) hipEventRecord(g->opA, g->stream);
-    launch_apply_dir<T>(op, r, dold, dnew, q, red, g->stream);
+    launch_apply_dir<T>(op, r, dold, dnew, q, x, xpend, red, g->stream);
     if (g->timing) {
         hipEventRecord(g->opB, g->stream);
         op_pending = true;
@@ -345,8 +346,11 @@ static void autotune_tiles()
     auto tune = [&](OpDesc &op, auto *d, auto *q, auto *r, auto *d2, int *best) {
         using T = typename std::remove_pointer<decltype(d)>::type;
         const bool fz = fused && d2 && apply_dir_ok(op);
+        // x: the deferred iterate update is off (xpend 0), so x is
+        // rewritten unchanged (it may hold an initial guess)
+        T *xs = std::is_same<T, double>::value ? (T *)g->x64 : (T *)g->x32;
         auto run = [&]() {
-            if (fz) launch_apply_dir<T>(op, r, d, d2, q, &rt, g->stream);
+            if (fz) launch_apply_dir<T>(op, r, d, d2, q, xs, 0, &rt, g->stream);
             else launch_apply<T>(op, d, q, nullptr, g->stream);
         };
         float best_ms = 1e30f;
@@ -401,7 +405,9 @@ static void note_x0(double delta, double delta0)
 // otherwise z is stored in q and the diagonal read from dinv.
 // With d2 and an operator that supports it (apply_dir_ok), the direction
 // update runs inside the operator: iteration j writes its direction to
-// dbuf[j % 2] from r and dbuf[(j - 1) % 2].
+// dbuf[j % 2] from r and dbuf[(j - 1) % 2], and applies the previous
+// iteration's x += alpha d there too; the update kernel then only updates r.
+// The last iteration's x update is applied after the loop.
 template <typename T>
 static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool x0set, double tol, unsigned maxit,
                    unsigned &it, double &residual, double *delta0_out, T *d2)
@@ -463,9 +469,9 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
             if (fused) {
                 T *dn = dbuf[j & 1];
                 const T *dold = j > 0 ? dbuf[(j - 1) & 1] : nullptr;
-                if (b == 0) timed_apply_dir<T>(op, r, dold, dn, q, &ra);
-                else launch_apply_dir<T>(op, r, dold, dn, q, &ra, st);
-                launch_kj_pcg_update<T>(op, x, r, dn, q, &rdl, st);
+                if (b == 0) timed_apply_dir<T>(op, r, dold, dn, q, x, 1, &ra);
+                else launch_apply_dir<T>(op, r, dold, dn, q, x, 1, &ra, st);
+                launch_kj_pcg_update<T>(op, nullptr, r, nullptr, q, &rdl, st);
                 continue;
             }
             if (j > 0) {
@@ -481,6 +487,8 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
         it = (unsigned)g->Sh[S_ITER];
         done = g->Sh[S_DONE] != 0.0;
     }
+    // the last executed iteration's deferred x += alpha d
+    if (fused && it > 0) launch_axpy_alpha<T>(x, dbuf[(it - 1) & 1], g->S, n, st);
     delta = g->Sh[S_DELTA];
     residual = sqrt(fabs(delta) / fabs(delta0));
     return !above(delta, delta0, tol);
@@ -646,11 +654,15 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
             for (unsigned b = 0; b < nb; b++) {
                 const unsigned j = base + seg + b;
                 if (fused) {
+                    // x += alpha d of iteration j - 1 inside the operator, when
+                    // that iteration belongs to this segment (x was flushed
+                    // and zeroed at the segment start)
                     float *dn = dbuf[j & 1];
                     const float *dold = j > 0 ? dbuf[(j - 1) & 1] : nullptr;
-                    if (b == 0) timed_apply_dir<float>(o32, r, dold, dn, q, &ra);
-                    else launch_apply_dir<float>(o32, r, dold, dn, q, &ra, st);
-                    launch_kj_pcg_update<float>(o32, x, r, dn, q, &rdl, st);
+                    const int xpend = j > base ? 1 : 0;
+                    if (b == 0) timed_apply_dir<float>(o32, r, dold, dn, q, x, xpend, &ra);
+                    else launch_apply_dir<float>(o32, r, dold, dn, q, x, xpend, &ra, st);
+                    launch_kj_pcg_update<float>(o32, nullptr, r, nullptr, q, &rdl, st);
                     continue;
                 }
                 if (j > 0) {
@@ -668,6 +680,8 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
         }
         it_total = base + seg;
         const bool last = it_total >= maxit;
+        // the segment's last deferred x += alpha d
+        if (fused && seg > 0) launch_axpy_alpha<float>(x, dbuf[(it_total - 1) & 1], g->S, n, st);
         // reliable update: x64 += x ; r64 -= A64 x ; x = 0 ; r = (float) r64
         launch_add_correction(g->x64, x, n, st);
         launch_convert<float, double>(x, g->d64, n, st);
@@ -1303,18 +1317,21 @@ double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps, cudapcgF
     if (dir) {
         // CG-loop form: r = x, directions alternate between d and s
         if (!g->precond || !apply_dir_ok(precision == CHFEM_PREC_MIXED ? g->dm.op32 : g->dm.op64)) return -1.0;
-        if (!set_scalar(S_DELTA_OLD, 1.0) || !set_scalar(S_DELTA, 0.5)) return -1.0;
+        if (!set_scalar(S_DELTA_OLD, 1.0) || !set_scalar(S_DELTA, 0.5) || !set_scalar(S_ALPHA, 0.0)) return -1.0;
+        if (precision == CHFEM_PREC_MIXED && !g->dinv32) return -1.0;
         launch_fill<double>(g->s64, 1.0, n, g->stream);
         if (precision == CHFEM_PREC_MIXED) launch_fill<float>(g->s32, 1.0, n, g->stream);
     }
     auto once = [&](unsigned k) {
         if (dir) {
+            // r = x; the deferred iterate update runs on the stored Jacobi
+            // diagonal with S[ALPHA] = 0, which rewrites it unchanged
             if (precision == CHFEM_PREC_MIXED)
                 launch_apply_dir<float>(g->dm.op32, g->x32, k & 1 ? g->s32 : g->d32, k & 1 ? g->d32 : g->s32, g->q32,
-                                        &red, g->stream);
+                                        g->dinv32, 1, &red, g->stream);
             else
                 launch_apply_dir<double>(g->dm.op64, g->x64, k & 1 ? g->s64 : g->d64, k & 1 ? g->d64 : g->s64, g->q64,
-                                         &red, g->stream);
+                                         g->dinv64, 1, &red, g->stream);
             return;
         }
         if (precision == CHFEM_PREC_MIXED) launch_apply<float>(g->dm.op32, k & 1 ? g->x32 : g->d32, g->q32, rp, g->stream);

@@ -123,6 +123,7 @@ template <typename T> void launch_add(T *y, const T *x, size_t n, hipStream_t s)
 //   kj_precond:    out = dinv v ; red <- v.out
 //   kj_pcg_dir:    d = dinv r + (S[delta]/S[delta_old]) d
 //   kj_pcg_update: x += alpha d ; r -= alpha q ; red <- r.(dinv r)
+//                  (x == nullptr: r -= alpha q ; red <- r.(dinv r), d unused)
 //   kj_sub_precond: r -= q ; z = dinv r ; red <- r.z
 template <typename T> void launch_kj_precond(const OpDesc &op, const T *v, T *out, const Reduce *red, hipStream_t s);
 template <typename T> void launch_kj_pcg_dir(const OpDesc &op, T *d, const T *r, const double *S, hipStream_t s,
@@ -164,8 +165,13 @@ void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, con
 // Jacobi-PCG step with the direction update inside the operator (isotropic
 // elastic NBN with key-Jacobi, see apply_dir_ok): d = dinv r + beta d_old
 // (beta = S[DELTA]/S[DELTA_OLD]; dold == nullptr: d = dinv r), dnew <- d,
-// q = A d, red <- d.q.  dnew must not alias dold, r or q.
+// q = A d, red <- d.q, and the previous iteration's deferred iterate update
+// x += S[ALPHA] d_old when xpend (and dold) is set (x is rewritten
+// unchanged otherwise).  dnew must not alias dold, r, q or x.
 bool apply_dir_ok(const OpDesc &op);
 template <typename T>
-void launch_apply_dir(const OpDesc &op, const T *r, const T *dold, T *dnew, T *q, const Reduce *red, hipStream_t s);
+void launch_apply_dir(const OpDesc &op, const T *r, const T *dold, T *dnew, T *q, T *x, int xpend,
+                      const Reduce *red, hipStream_t s);
+// x += S[ALPHA] d (the last deferred iterate update of a loop)
+template <typename T> void launch_axpy_alpha(T *x, const T *d, const double *S, size_t n, hipStream_t s);
 }  // namespace chfem

@@ -623,8 +623,9 @@ __global__ void __launch_bounds__(BS) k_kj_pcg_dir(T *__restrict__ d, const T *_
     }
 }
 
-// x += a d ; r -= a q ; reduce r.(dinv r)
-template <typename T, int ND>
+// x += a d ; r -= a q ; reduce r.(dinv r).  WITHX false: the iterate update
+// is done elsewhere (deferred into the next fused operator) and x, d are unused.
+template <typename T, int ND, bool WITHX>
 __global__ void __launch_bounds__(BS) k_kj_pcg_update(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ d,
                                                       const T *__restrict__ q, const uint16_t *__restrict__ keys,
                                                       const T *__restrict__ kinv, size_t N,
@@ -638,18 +639,20 @@ __global__ void __launch_bounds__(BS) k_kj_pcg_update(T *__restrict__ x, T *__re
     {
         const size_t o = g_ * 4 * ND;
         Group<T, ND> xv, rv, dv, qv, m;
-        gload<T, ND>(x + o, xv);
+        if (WITHX) {
+            gload<T, ND>(x + o, xv);
+            gload<T, ND>(d + o, dv);
+        }
         gload<T, ND>(r + o, rv);
-        gload<T, ND>(d + o, dv);
         gload<T, ND>(q + o, qv);
         group_dinv<T, ND>(kinv, keys, g_, m);
 #pragma unroll
         for (int k = 0; k < 4 * ND; k++) {
-            xv.v[k] += a * dv.v[k];
+            if (WITHX) xv.v[k] += a * dv.v[k];
             rv.v[k] -= a * qv.v[k];
             acc += (double)rv.v[k] * (double)(m.v[k] * rv.v[k]);
         }
-        gstore<T, ND>(x + o, xv);
+        if (WITHX) gstore<T, ND>(x + o, xv);
         gstore<T, ND>(r + o, rv);
     }
     FOR_TAIL_NODES(N)
@@ -659,7 +662,7 @@ __global__ void __launch_bounds__(BS) k_kj_pcg_update(T *__restrict__ x, T *__re
 #pragma unroll
         for (int c = 0; c < ND; c++) {
             const size_t j = ND * i + c;
-            x[j] += a * d[j];
+            if (WITHX) x[j] += a * d[j];
             const T rn = r[j] - a * q[j];
             r[j] = rn;
             acc += (double)rn * (double)(dvn[c] * rn);
@@ -667,6 +670,15 @@ __global__ void __launch_bounds__(BS) k_kj_pcg_update(T *__restrict__ x, T *__re
     }
     double vv[1] = {acc};
     store_partials<1>(red, vv);
+}
+
+// x += S[ALPHA] d
+template <typename T>
+__global__ void __launch_bounds__(BS) k_axpy_alpha(T *__restrict__ x, const T *__restrict__ d,
+                                                   const double *__restrict__ S, size_t n)
+{
+    const T a = (T)S[S_ALPHA];
+    for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) x[i] += a * d[i];
 }
 
 // r -= q ; z = dinv r ; reduce r.z
@@ -1133,8 +1145,22 @@ template <typename T>
 void launch_kj_pcg_update(const OpDesc &op, T *x, T *r, const T *d, const T *q, const Reduce *red, hipStream_t s)
 {
     const int gr = group_grid(op, sizeof(T));
-    KJ_DISPATCH(k_kj_pcg_update, gr, x, r, d, q, op.keys, (const T *)op.kinv, (size_t)op.N, *red);
+    const T *kinv = (const T *)op.kinv;
+    const size_t N = op.N;
+    if (op.physics == PHY_ELASTIC) {
+        if (x) k_kj_pcg_update<T, 3, true><<<gr, BS, 0, s>>>(x, r, d, q, op.keys, kinv, N, *red);
+        else k_kj_pcg_update<T, 3, false><<<gr, BS, 0, s>>>(x, r, d, q, op.keys, kinv, N, *red);
+    } else {
+        if (x) k_kj_pcg_update<T, 1, true><<<gr, BS, 0, s>>>(x, r, d, q, op.keys, kinv, N, *red);
+        else k_kj_pcg_update<T, 1, false><<<gr, BS, 0, s>>>(x, r, d, q, op.keys, kinv, N, *red);
+    }
     launch_reduce_finish(*red, gr, 1, s);
+}
+
+template <typename T>
+void launch_axpy_alpha(T *x, const T *d, const double *S, size_t n, hipStream_t s)
+{
+    k_axpy_alpha<T><<<stream_grid(n, sizeof(T)), BS, 0, s>>>(x, d, S, n);
 }
 
 template <typename T>
@@ -1221,7 +1247,8 @@ void launch_interpl2(const double *coarse, double *fine, int rows, int cols, int
     template void launch_kj_pcg_dir<T>(const OpDesc &, T *, const T *, const double *, hipStream_t, int);     \
     template void launch_kj_pcg_update<T>(const OpDesc &, T *, T *, const T *, const T *, const Reduce *,     \
                                           hipStream_t);                                                       \
-    template void launch_kj_sub_precond<T>(const OpDesc &, T *, const T *, T *, const Reduce *, hipStream_t);
+    template void launch_kj_sub_precond<T>(const OpDesc &, T *, const T *, T *, const Reduce *, hipStream_t); \
+    template void launch_axpy_alpha<T>(T *, const T *, const double *, size_t, hipStream_t);
 INST(float)
 INST(double)
 template void launch_convert<double, float>(const double *, float *, size_t, hipStream_t);

@@ -140,14 +140,19 @@ __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >
 // the operator's input d = dinv r + beta d_old is formed while the node
 // planes are fetched (dinv from the node's packed key, kinv[key * 3 + a]),
 // q = A d is computed from it, and d is written to dnew for the nodes the
-// block owns.  v is d_old; first: d = dinv r (v unused).
+// block owns.  The previous iteration's iterate update is deferred to here
+// too: x += alpha d_old on the owned nodes (alpha = 0 leaves x as it is).
+// v is d_old; first: d = dinv r (v unused).
 template <typename T>
 struct DirFuse {
     const T *r;
     const T *kinv;
     T *dnew;
+    T *x;
     T beta;
+    T alpha;
     int first;
+    int xpend;  // apply the deferred x update with alpha = S[ALPHA]
 };
 
 // Work decomposition: a block of TR x TC work-items evaluates a TR x TC
@@ -253,7 +258,10 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
     // material is read from the packed keys one layer ahead (mat_key) and
     // decoded when needed (material).
     const bool elem_ok = tr <= vr && tc <= vc;
-    const uint32_t eoff = (uint32_t)wrapi(c0 - 1 + tc, nx) * (uint32_t)ny + (uint32_t)wrapi(r0 - 1 + tr, ny);
+    // (work-items without an element read node 0 of the layer: the key load
+    // is unconditional)
+    const uint32_t eoff = elem_ok ? (uint32_t)wrapi(c0 - 1 + tc, nx) * (uint32_t)ny + (uint32_t)wrapi(r0 - 1 + tr, ny)
+                                  : 0u;
     auto mat_key = [&](int le) -> uint32_t { return keys[(size_t)wrapl(le) * nxy + eoff]; };
     auto material = [&](uint32_t kv) -> int {
         // NBN keys: node (r,c,l) is local node 3 of element (r,c,l); EBE: element map
@@ -320,6 +328,18 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
         // them too (in bounds, unused): every step then has the same loads
         // and none is pending across the loop's back edge.
         mk = mat_key(l + 1);
+        // FUSE: x and d_old of this work-item's node at layer l for the
+        // deferred iterate update (work-items that own no node read node 0
+        // of the layer; only owners store)
+        T px[FUSE ? 3 : 1], pd[FUSE ? 3 : 1];
+        const size_t on = 3 * ((size_t)l * nxy + (own ? (size_t)(c0 + tc - 1) * ny + (r0 + tr - 1) : 0));
+        if constexpr (FUSE) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                px[c] = fz.x[on + c];
+                pd[c] = dsrc[on + c];
+            }
+        }
         fetch(l + 2);
         T own_u[3] = {0, 0, 0};
         if ((DOT || FUSE) && own) {
@@ -345,12 +365,15 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
                 const T *sp = sf + fslot(k);
                 acc[0] += sp[0]; acc[1] += sp[1]; acc[2] += sp[2];
             }
-            const size_t on = 3 * ((size_t)l * nxy + (size_t)(c0 + tc - 1) * ny + (r0 + tr - 1));
             T *qo = q + on;
             qo[0] = o[0]; qo[1] = o[1]; qo[2] = o[2];
-            if (FUSE) {
+            if constexpr (FUSE) {
                 T *dn = fz.dnew + on;
                 dn[0] = own_u[0]; dn[1] = own_u[1]; dn[2] = own_u[2];
+                T *xo = fz.x + on;
+                xo[0] = px[0] + fz.alpha * pd[0];
+                xo[1] = px[1] + fz.alpha * pd[1];
+                xo[2] = px[2] + fz.alpha * pd[2];
             }
             if (DOT) dsum += (double)own_u[0] * o[0] + (double)own_u[1] * o[1] + (double)own_u[2] * o[2];
         }

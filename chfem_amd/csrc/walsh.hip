@@ -14,6 +14,9 @@ __global__ void __launch_bounds__(TR *TC) k_elastic_walsh(const T *__restrict__ 
     if (DOT && stopped(red)) return;
     // beta of the fused direction update, as k_kj_pcg_dir forms it
     if (FUSE && !fz.first) fz.beta = (T)(red.S[S_DELTA] / red.S[S_DELTA_OLD]);
+    // alpha of the previous iteration for the deferred x update, as
+    // k_kj_pcg_update forms it
+    if (FUSE) fz.alpha = fz.xpend ? (T)red.S[S_ALPHA] : T(0);
     // 1-D grid; blocks are dispatched to the 8 XCDs round-robin (block b to
     // XCD b % 8), so each XCD is given one contiguous range of tiles (rows of
     // tiles fastest): tiles that share halo nodes then run on the same XCD at
@@ -35,7 +38,7 @@ void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, con
 {
     Reduce rr = {nullptr, nullptr, 0, 0, 0};
     if (red) rr = *red;
-    DirFuse<T> f = {nullptr, nullptr, nullptr, T(0), 1};
+    DirFuse<T> f = {nullptr, nullptr, nullptr, nullptr, T(0), T(0), 1, 0};
     if (fz) f = *fz;
     int LZ = t.LZ > 0 ? t.LZ : 16;
     {
@@ -74,17 +77,18 @@ bool apply_dir_ok(const OpDesc &op)
 }
 
 template <typename T>
-void launch_apply_dir(const OpDesc &op, const T *r, const T *dold, T *dnew, T *q, const Reduce *red, hipStream_t s)
+void launch_apply_dir(const OpDesc &op, const T *r, const T *dold, T *dnew, T *q, T *x, int xpend,
+                      const Reduce *red, hipStream_t s)
 {
-    const DirFuse<T> f = {r, (const T *)op.kinv, dnew, T(0), dold ? 0 : 1};
+    const DirFuse<T> f = {r, (const T *)op.kinv, dnew, x, T(0), T(0), dold ? 0 : 1, dold ? xpend : 0};
     const WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
     launch_elastic_walsh<T>(op, (const T *)op.lamu, dold, q, red, t, s, &f);
 }
 
-template void launch_apply_dir<float>(const OpDesc &, const float *, const float *, float *, float *, const Reduce *,
-                                      hipStream_t);
-template void launch_apply_dir<double>(const OpDesc &, const double *, const double *, double *, double *,
-                                       const Reduce *, hipStream_t);
+template void launch_apply_dir<float>(const OpDesc &, const float *, const float *, float *, float *, float *, int,
+                                      const Reduce *, hipStream_t);
+template void launch_apply_dir<double>(const OpDesc &, const double *, const double *, double *, double *, double *,
+                                       int, const Reduce *, hipStream_t);
 template void launch_elastic_walsh<float>(const OpDesc &, const float *, const float *, float *, const Reduce *,
                                           WalshTiles, hipStream_t, const DirFuse<float> *);
 template void launch_elastic_walsh<double>(const OpDesc &, const double *, const double *, double *, const Reduce *,
