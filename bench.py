@@ -209,9 +209,19 @@ def main():
 
     N = n ** 3
     op_avg_ms = fast["op_ms_total"] / max(fast["op_launches"], 1)
-    bytes_per_launch = 25.0 * N  # FP32 elastic apply: 1 B image + 12 B read + 12 B written per node
+    # Algorithmic bytes per node of the operator launch inside the CG loop.
+    # Fused form (default): 1 B image + r, d_old, x read + d, q, x written
+    # (6 x 3 values); plain apply: 1 B image + v read + q written.
+    fused = os.environ.get("CHFEM_FUSE_DIR", "1") != "0"
+    per_node32 = 1.0 + (6 if fused else 2) * 12.0
+    per_node64 = 1.0 + (6 if fused else 2) * 24.0
+    bytes_per_launch = per_node32 * N
     achieved = bytes_per_launch / (op_avg_ms * 1e-3) / 1e9
-    traffic = read_traffic(f"elastic_walsh_fp32_{n}")
+    traffic = read_traffic(f"elastic_walsh_{'dir_' if fused else ''}fp32_{n}")
+    tr_, tc_, lz_ = fast["tile"].split("x")
+    kernel = (f"k_elastic_walsh<float,{tr_},{tc_},true,true> (FP32 elastic NBN apply with the CG direction update "
+              f"d = M^-1 r + beta d, the deferred x += alpha d and d.Ad fused), LZ {lz_}" if fused else
+              f"k_elastic_walsh<float,{tr_},{tc_},true,false> (FP32 elastic NBN apply + fused d.Ad), LZ {lz_}")
     cpu = None
     if not args.skip_cpu and plain is not None:
         log(f"[bench] cpu baseline (oracle)")
@@ -237,7 +247,7 @@ def main():
         "plain_fp64": None if plain is None else {
             "seconds": plain["seconds"], "err_vs_fp64": err(plain["C"]), "iterations": plain["iterations"],
             "op_avg_ms": plain["op_ms_total"] / max(plain["op_launches"], 1),
-            "op_GBps": 49.0 * N / (plain["op_ms_total"] / max(plain["op_launches"], 1) * 1e-3) / 1e9},
+            "op_GBps": per_node64 * N / (plain["op_ms_total"] / max(plain["op_launches"], 1) * 1e-3) / 1e9},
         "mixed": None if mixed is None else {
             "seconds": mixed["seconds"], "err_vs_fp64": err(mixed["C"]), "iterations": mixed["iterations"],
             "sweeps": mixed["sweeps"]},
@@ -247,8 +257,7 @@ def main():
         "speedup_vs_plain": None if plain is None else plain["seconds"] / fast["seconds"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_elastic_walsh<float,%s,true> (FP32 elastic NBN apply + fused d.Ad), LZ %s"
-                               % (",".join(fast["tile"].split("x")[:2]), fast["tile"].split("x")[2]),
+                     "kernel": kernel, "bytes_per_node": per_node32,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": op_avg_ms,
                      "launches": fast["op_launches"]},
         "cpu_baseline": cpu,

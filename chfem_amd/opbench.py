@@ -16,9 +16,10 @@ from .lib import lib, ptr
 
 PHYS = {"thermal": 0, "elastic": 1}
 BYTES = {("thermal", 0): 17.0, ("thermal", 1): 9.0, ("elastic", 0): 49.0, ("elastic", 1): 25.0}
-# CG-loop form with the direction update inside the operator: image, r and
-# the previous direction read, the direction and q written
-BYTES_DIR = {0: 1.0 + 4 * 24.0, 1: 1.0 + 4 * 12.0}
+# CG-loop form with the direction update and the deferred iterate update
+# inside the operator: image, r, the previous direction and x read, the
+# direction, q and x written
+BYTES_DIR = {0: 1.0 + 6 * 24.0, 1: 1.0 + 6 * 12.0}
 
 
 def main():
