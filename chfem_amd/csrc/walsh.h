@@ -177,13 +177,15 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
 {
     constexpr int NT = TR * TC, NR = TR + 1, NC = TC + 1, NV = NR * NC * 3;
     constexpr int SLOTS = (NV + NT - 1) / NT;
-    constexpr int SF = 25;  // padded element stride in the force buffer (fewer LDS bank conflicts)
+    constexpr int SB = 7;  // padded stride of the row-combined forces in LDS (no bank conflicts)
     // node planes padded to SLOTS * NT entries: every work-item stores every
     // slot, so the plane loads and stores need no per-slot branches (a
     // branch around a load makes the compiler wait for all loads in flight
     // at its join)
     __shared__ T su[2][SLOTS * NT];
-    __shared__ T sf[NT * SF];
+    // FUSE: the fetched d_old planes, for the owned nodes' deferred x update
+    __shared__ T sdo[FUSE ? 2 : 1][FUSE ? SLOTS * NT : 1];
+    __shared__ T sb[NT * SB];
     __shared__ T smat[8];
     const int tid = threadIdx.x;
     const int tr = tid % TR, tc = tid / TR;
@@ -247,11 +249,15 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
             for (int k = 0; k < SLOTS; k++) pre[k] = v[lo + goff[k]];
         }
     };
-    auto store = [&](T *dst) {
+    auto store = [&](T *dst, T *ddst) {
 #pragma unroll
         for (int k = 0; k < SLOTS; k++) {
-            if constexpr (FUSE) dst[tid + k * NT] = pm[k] * pr[k] + beta * pre[k];
-            else dst[tid + k * NT] = pre[k];
+            if constexpr (FUSE) {
+                dst[tid + k * NT] = pm[k] * pr[k] + beta * pre[k];
+                ddst[tid + k * NT] = pre[k];
+            } else {
+                dst[tid + k * NT] = pre[k];
+            }
         }
     };
     // element of this work-item: origin (r0 - 1 + tr, c0 - 1 + tc).  Its
@@ -268,54 +274,71 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
         if (!elem_ok) return 0;
         return ebe ? (int)(kv & 3u) : (int)((kv >> 6) & 3u);
     };
-    auto element = [&](const T *cur, const T *nxt, int m) {
-        T u[8][3], f[8][3];
+    // Node (tr, tc) of a layer receives local node k's force from the element
+    // (tr - WDR(k), tc - WDC(k)): from element layer l (k = 0..3)
+    //   E(tr,tc).f3 + E(tr-1,tc).f0  +  E(tr,tc-1).f2 + E(tr-1,tc-1).f1
+    //   =        A(tr, tc)           +            B(tr, tc-1)
+    // and from element layer l - 1 (k = 4..7) the same with f7, f4 (Af) and
+    // f6, f5 (Bf).  E(tr-1, tc) is lane - 1 of the same wavefront (TR is 16,
+    // 32 or 64, and row 0 only feeds nodes no work-item owns), so A and Af
+    // are formed with cross-lane moves; B and Bf go through LDS (6 values per
+    // element).
+    static_assert(TR == 16 || TR == 32 || TR == 64, "rows per tile must tile a 64-wide wavefront");
+    auto element = [&](const T *cur, const T *nxt, int m, T (&A)[3], T (&Af)[3]) {
+        T f[8][3];
+        if (elem_ok) {
+            T u[8][3];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const T *src = WDL(k) ? nxt : cur;
-            const int o = ((tc + WDC(k)) * NR + tr + WDR(k)) * 3;
-            u[k][0] = src[o];
-            u[k][1] = src[o + 1];
-            u[k][2] = src[o + 2];
+            for (int k = 0; k < 8; k++) {
+                const T *src = WDL(k) ? nxt : cur;
+                const int o = ((tc + WDC(k)) * NR + tr + WDR(k)) * 3;
+                u[k][0] = src[o];
+                u[k][1] = src[o + 1];
+                u[k][2] = src[o + 2];
+            }
+            element_forces(u, smat[2 * m], smat[2 * m + 1], f);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; k++) f[k][0] = f[k][1] = f[k][2] = T(0);
         }
-        element_forces(u, smat[2 * m], smat[2 * m + 1], f);
-        T *dst = sf + tid * SF;
+        T *dst = sb + tid * SB;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            dst[3 * k] = f[k][0];
-            dst[3 * k + 1] = f[k][1];
-            dst[3 * k + 2] = f[k][2];
+        for (int a = 0; a < 3; a++) {
+            A[a] = f[3][a] + __shfl_up(f[0][a], 1);
+            Af[a] = f[7][a] + __shfl_up(f[4][a], 1);
+            dst[a] = f[2][a] + __shfl_up(f[1][a], 1);
+            dst[3 + a] = f[6][a] + __shfl_up(f[5][a], 1);
         }
     };
     const bool own = tr >= 1 && tc >= 1 && tr - 1 < vr && tc - 1 < vc;
-    // force slot of the element for which this node is local node k
-    auto fslot = [&](int k) { return ((tc - WDC(k)) * TR + (tr - WDR(k))) * SF + 3 * k; };
+    // B and Bf of the element left of this node's column, same row
+    const T *bleft = sb + ((tc >= 1 ? tc - 1 : 0) * TR + tr) * SB;
 
     T *cur = su[0], *nxt = su[1];
+    T *dcur = sdo[0], *dnxt = sdo[FUSE ? 1 : 0];  // d_old planes, swapped with cur / nxt
     load_keys(l0 - 1);
     fetch(l0 - 1);
     uint32_t mk = mat_key(l0 - 1);
-    store(cur);
+    store(cur, dcur);
     fetch(l0);
-    store(nxt);
+    store(nxt, dnxt);
     int m = material(mk);
     mk = mat_key(l0);
     __syncthreads();
     fetch(l0 + 1);
-    if (elem_ok) element(cur, nxt, m);
+    T A[3], Af[3];
+    element(cur, nxt, m, A, Af);
     __syncthreads();
     T acc[3] = {0, 0, 0};
     if (own) {
 #pragma unroll
-        for (int k = 4; k < 8; k++) {
-            const T *sp = sf + fslot(k);
-            acc[0] += sp[0]; acc[1] += sp[1]; acc[2] += sp[2];
-        }
+        for (int a = 0; a < 3; a++) acc[a] = Af[a] + bleft[3 + a];
     }
-    store(cur);  // layer l0 + 1 replaces layer l0 - 1
+    store(cur, dcur);  // layer l0 + 1 replaces layer l0 - 1
     m = material(mk);
     {
         T *t = cur; cur = nxt; nxt = t;  // cur = layer l0, nxt = layer l0 + 1
+        t = dcur; dcur = dnxt; dnxt = t;
     }
     __syncthreads();
     double dsum = 0.0;
@@ -328,42 +351,37 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
         // them too (in bounds, unused): every step then has the same loads
         // and none is pending across the loop's back edge.
         mk = mat_key(l + 1);
-        // FUSE: x and d_old of this work-item's node at layer l for the
-        // deferred iterate update (work-items that own no node read node 0
-        // of the layer; only owners store)
+        // FUSE: x of this work-item's node at layer l for the deferred
+        // iterate update (work-items that own no node read node 0 of the
+        // layer; only owners store); its d_old comes from the LDS plane
         T px[FUSE ? 3 : 1], pd[FUSE ? 3 : 1];
         const size_t on = 3 * ((size_t)l * nxy + (own ? (size_t)(c0 + tc - 1) * ny + (r0 + tr - 1) : 0));
         if constexpr (FUSE) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                px[c] = fz.x[on + c];
-                pd[c] = dsrc[on + c];
-            }
+            for (int c = 0; c < 3; c++) px[c] = fz.x[on + c];
         }
         fetch(l + 2);
         T own_u[3] = {0, 0, 0};
         if ((DOT || FUSE) && own) {
             const T *uu = cur + (tc * NR + tr) * 3;
             own_u[0] = uu[0]; own_u[1] = uu[1]; own_u[2] = uu[2];
+            if constexpr (FUSE) {
+                const T *dd = dcur + (tc * NR + tr) * 3;
+                pd[0] = dd[0]; pd[1] = dd[1]; pd[2] = dd[2];
+            }
         }
-        if (elem_ok) element(cur, nxt, m);
+        element(cur, nxt, m, A, Af);
         __syncthreads();
         // the prefetched plane goes to LDS before this layer's node results
         // are stored, so waiting for the loads does not wait for the stores
-        store(cur);  // layer l + 2 replaces layer l
+        store(cur, dcur);  // layer l + 2 replaces layer l
         m = material(mk);
         if (own) {
-            T o[3] = {acc[0], acc[1], acc[2]};
+            T o[3];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const T *sp = sf + fslot(k);
-                o[0] += sp[0]; o[1] += sp[1]; o[2] += sp[2];
-            }
-            acc[0] = acc[1] = acc[2] = 0;
-#pragma unroll
-            for (int k = 4; k < 8; k++) {
-                const T *sp = sf + fslot(k);
-                acc[0] += sp[0]; acc[1] += sp[1]; acc[2] += sp[2];
+            for (int a = 0; a < 3; a++) {
+                o[a] = acc[a] + A[a] + bleft[a];
+                acc[a] = Af[a] + bleft[3 + a];
             }
             T *qo = q + on;
             qo[0] = o[0]; qo[1] = o[1]; qo[2] = o[2];
@@ -379,6 +397,7 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
         }
         {
             T *t = cur; cur = nxt; nxt = t;
+            t = dcur; dcur = dnxt; dnxt = t;
         }
         __syncthreads();
     }
