@@ -403,11 +403,12 @@ static void note_x0(double delta, double delta0)
 // With op.kinv (thermal/elastic NBN) the Jacobi diagonal comes from the keys
 // and z = M^-1 r is folded into the direction update (d = M^-1 r + beta d);
 // otherwise z is stored in q and the diagonal read from dinv.
-// With d2 and an operator that supports it (apply_dir_ok), the direction
-// update runs inside the operator: iteration j writes its direction to
-// dbuf[j % 2] from r and dbuf[(j - 1) % 2], and applies the previous
-// iteration's x += alpha d there too; the update kernel then only updates r.
-// The last iteration's x update is applied after the loop.
+// With d2 and an operator that supports it (apply_dir_ok), the loop runs on
+// the preconditioned residual z = M^-1 r (held in r's buffer from the loop's
+// start) and the direction update runs inside the operator: iteration j
+// writes its direction to dbuf[j % 2] from z and dbuf[(j - 1) % 2], and
+// applies the previous iteration's x += alpha d there too; the update kernel
+// only updates z.  The last iteration's x update is applied after the loop.
 template <typename T>
 static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool x0set, double tol, unsigned maxit,
                    unsigned &it, double &residual, double *delta0_out, T *d2)
@@ -459,6 +460,10 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
     // iterate and the iteration count are those of a loop that synchronises
     // every iteration.
     Reduce ra = make_guarded(FIN_ALPHA, 0), rdl = make_guarded(FIN_DELTA, 0);
+    if (fused) {
+        Reduce rz = make_reduce(FIN_STORE, S_TMP0);
+        launch_kj_to_z<T>(op, r, &rz, st);
+    }
     if (!arm_guard(tol * tol * fabs(delta0))) return -1;
     it = 0;
     bool done = false;
@@ -471,7 +476,7 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
                 const T *dold = j > 0 ? dbuf[(j - 1) & 1] : nullptr;
                 if (b == 0) timed_apply_dir<T>(op, r, dold, dn, q, x, 1, &ra);
                 else launch_apply_dir<T>(op, r, dold, dn, q, x, 1, &ra, st);
-                launch_kj_pcg_update<T>(op, nullptr, r, nullptr, q, &rdl, st);
+                launch_kj_zupdate<T>(op, r, q, &rdl, st);
                 continue;
             }
             if (j > 0) {
@@ -597,11 +602,13 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
     const bool kj = o32.kinv != nullptr;
     float *x = g->x32, *r = g->r32, *d = g->d32, *q = g->q32;
     const float *dinv = g->dinv32;
-    // direction update inside the FP32 operator; iteration j (counted over
-    // the whole solve, the direction is kept across replacements) writes
+    // direction update inside the FP32 operator, on the preconditioned
+    // residual z = M^-1 r (kept in r's buffer); iteration j (counted over the
+    // whole solve, the direction is kept across replacements) writes
     // dbuf[j % 2]
     const bool fused = kj && g->s32 && fuse_dir_enabled() && apply_dir_ok(o32);
     float *dbuf[2] = {d, g->s32};
+    Reduce rz = make_reduce(FIN_STORE, S_TMP0);
     Reduce r0 = make_reduce(FIN_STORE, S_DELTA0), rd = make_reduce(FIN_STORE, S_DELTA);
     launch_precond<double>(g->dinv64, g->r64, nullptr, n, &r0, st);
     if (!sync_scalars()) return -1;
@@ -635,7 +642,8 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
     // FP32 start: r = (float) r64, x = 0, d = M^-1 r, S[delta] = r.d
     launch_convert<double, float>(g->r64, r, n, st);
     launch_fill<float>(x, 0.0, n, st);
-    if (kj) launch_kj_precond<float>(o32, r, d, &rd, st);
+    if (fused) launch_kj_to_z<float>(o32, r, &rd, st);
+    else if (kj) launch_kj_precond<float>(o32, r, d, &rd, st);
     else launch_precond<float>(dinv, r, d, n, &rd, st);
     // FP32 iterations are queued cg_batch() at a time; the device stops the
     // queue once r.z falls to the next replacement threshold
@@ -662,7 +670,7 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
                     const int xpend = j > base ? 1 : 0;
                     if (b == 0) timed_apply_dir<float>(o32, r, dold, dn, q, x, xpend, &ra);
                     else launch_apply_dir<float>(o32, r, dold, dn, q, x, xpend, &ra, st);
-                    launch_kj_pcg_update<float>(o32, nullptr, r, nullptr, q, &rdl, st);
+                    launch_kj_zupdate<float>(o32, r, q, &rdl, st);
                     continue;
                 }
                 if (j > 0) {
@@ -699,7 +707,8 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
         if (last || !(fabs(delta) < fabs(prev))) break;  // no progress left in FP32
         delta_rel = delta;
         launch_convert<double, float>(g->r64, r, n, st);
-        if (!kj) launch_precond<float>(dinv, r, q, n, nullptr, st);  // stored-dinv direction reads z from q
+        if (fused) launch_kj_to_z<float>(o32, r, &rz, st);
+        else if (!kj) launch_precond<float>(dinv, r, q, n, nullptr, st);  // stored-dinv direction reads z from q
         base = it_total;
         if (!arm_guard(thresh())) return -1;
     }

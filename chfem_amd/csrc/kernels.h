@@ -69,7 +69,8 @@ struct OpDesc {
     const void *lamu;
     int tile_tr, tile_tc, tile_lz;
     // inverse Jacobi diagonal per packed key, kinv[key * nd + a] (65536 keys,
-    // precision of K): non-null selects the key-Jacobi vector kernels
+    // precision of K), followed by the diagonal itself at kinv[65536 * nd +
+    // key * nd + a]: non-null selects the key-Jacobi vector kernels
     // (thermal/elastic NBN)
     const void *kinv;
     int nmat;
@@ -123,7 +124,9 @@ template <typename T> void launch_add(T *y, const T *x, size_t n, hipStream_t s)
 //   kj_precond:    out = dinv v ; red <- v.out
 //   kj_pcg_dir:    d = dinv r + (S[delta]/S[delta_old]) d
 //   kj_pcg_update: x += alpha d ; r -= alpha q ; red <- r.(dinv r)
-//                  (x == nullptr: r -= alpha q ; red <- r.(dinv r), d unused)
+//   kj_to_z:       r <- z = dinv r in place ; red <- r.z
+//   kj_zupdate:    z -= alpha dinv q ; red <- r.z = z.(diag z)
+// op.kinv holds dinv per key, then diag per key at offset 65536 * nd.
 //   kj_sub_precond: r -= q ; z = dinv r ; red <- r.z
 template <typename T> void launch_kj_precond(const OpDesc &op, const T *v, T *out, const Reduce *red, hipStream_t s);
 template <typename T> void launch_kj_pcg_dir(const OpDesc &op, T *d, const T *r, const double *S, hipStream_t s,
@@ -132,6 +135,8 @@ template <typename T> void launch_kj_pcg_update(const OpDesc &op, T *x, T *r, co
                                                 const Reduce *red, hipStream_t s);
 template <typename T> void launch_kj_sub_precond(const OpDesc &op, T *r, const T *q, T *z, const Reduce *red,
                                                  hipStream_t s);
+template <typename T> void launch_kj_to_z(const OpDesc &op, T *r, const Reduce *red, hipStream_t s);
+template <typename T> void launch_kj_zupdate(const OpDesc &op, T *z, const T *q, const Reduce *red, hipStream_t s);
 // dot reduction of two vectors
 template <typename T> void launch_dot(const T *a, const T *b, size_t n, const Reduce *red, hipStream_t s);
 
@@ -162,15 +167,17 @@ template <typename T>
 void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduce *red, WalshTiles t,
                           hipStream_t s, const DirFuse<T> *fz = nullptr);
 
-// Jacobi-PCG step with the direction update inside the operator (isotropic
-// elastic NBN with key-Jacobi, see apply_dir_ok): d = dinv r + beta d_old
-// (beta = S[DELTA]/S[DELTA_OLD]; dold == nullptr: d = dinv r), dnew <- d,
-// q = A d, red <- d.q, and the previous iteration's deferred iterate update
+// Jacobi-PCG step on the preconditioned residual z = dinv r with the
+// direction update inside the operator (isotropic elastic NBN with
+// key-Jacobi, see apply_dir_ok): d = z + beta d_old (beta =
+// S[DELTA]/S[DELTA_OLD]; dold == nullptr: d = z), dnew <- d, q = A d,
+// red <- d.q, and the previous iteration's deferred iterate update
 // x += S[ALPHA] d_old when xpend (and dold) is set (x is rewritten
-// unchanged otherwise).  dnew must not alias dold, r, q or x.
+// unchanged otherwise).  dnew must not alias dold, z, q or x.  The loop's
+// other kernels: launch_kj_to_z (r -> z), launch_kj_zupdate.
 bool apply_dir_ok(const OpDesc &op);
 template <typename T>
-void launch_apply_dir(const OpDesc &op, const T *r, const T *dold, T *dnew, T *q, T *x, int xpend,
+void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q, T *x, int xpend,
                       const Reduce *red, hipStream_t s);
 // x += S[ALPHA] d (the last deferred iterate update of a loop)
 template <typename T> void launch_axpy_alpha(T *x, const T *d, const double *S, size_t n, hipStream_t s);

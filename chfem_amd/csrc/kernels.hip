@@ -623,9 +623,8 @@ __global__ void __launch_bounds__(BS) k_kj_pcg_dir(T *__restrict__ d, const T *_
     }
 }
 
-// x += a d ; r -= a q ; reduce r.(dinv r).  WITHX false: the iterate update
-// is done elsewhere (deferred into the next fused operator) and x, d are unused.
-template <typename T, int ND, bool WITHX>
+// x += a d ; r -= a q ; reduce r.(dinv r)
+template <typename T, int ND, bool WITHX = true>
 __global__ void __launch_bounds__(BS) k_kj_pcg_update(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ d,
                                                       const T *__restrict__ q, const uint16_t *__restrict__ keys,
                                                       const T *__restrict__ kinv, size_t N,
@@ -666,6 +665,87 @@ __global__ void __launch_bounds__(BS) k_kj_pcg_update(T *__restrict__ x, T *__re
             const T rn = r[j] - a * q[j];
             r[j] = rn;
             acc += (double)rn * (double)(dvn[c] * rn);
+        }
+    }
+    double vv[1] = {acc};
+    store_partials<1>(red, vv);
+}
+
+// PCG on the preconditioned residual z = dinv r (the CG loop of the fused
+// elastic operator): z -= a dinv q ; reduce r.z = z.(diag z).  kinv holds
+// dinv per key, then diag per key at offset 65536 * ND.
+template <typename T, int ND>
+__global__ void __launch_bounds__(BS) k_kj_zupdate(T *__restrict__ z, const T *__restrict__ q,
+                                                   const uint16_t *__restrict__ keys, const T *__restrict__ kinv,
+                                                   size_t N, Reduce red)
+{
+    if (stopped(red)) return;
+    const T a = (T)red.S[S_ALPHA];
+    const T *kdiag = kinv + (size_t)65536 * ND;
+    double acc = 0;
+    GROUP_RANGE(N)
+    FOR_GROUPS
+    {
+        const size_t o = g_ * 4 * ND;
+        Group<T, ND> zv, qv, m, dg;
+        gload<T, ND>(z + o, zv);
+        gload<T, ND>(q + o, qv);
+        group_dinv<T, ND>(kinv, keys, g_, m);
+        group_dinv<T, ND>(kdiag, keys, g_, dg);
+#pragma unroll
+        for (int k = 0; k < 4 * ND; k++) {
+            zv.v[k] -= a * (m.v[k] * qv.v[k]);
+            acc += (double)zv.v[k] * (double)(dg.v[k] * zv.v[k]);
+        }
+        gstore<T, ND>(z + o, zv);
+    }
+    FOR_TAIL_NODES(N)
+    {
+        T mv[ND], dv[ND];
+        node_dinv<T, ND>(kinv, keys[i], mv);
+        node_dinv<T, ND>(kdiag, keys[i], dv);
+#pragma unroll
+        for (int c = 0; c < ND; c++) {
+            const size_t j = ND * i + c;
+            const T zn = z[j] - a * (mv[c] * q[j]);
+            z[j] = zn;
+            acc += (double)zn * (double)(dv[c] * zn);
+        }
+    }
+    double vv[1] = {acc};
+    store_partials<1>(red, vv);
+}
+
+// r <- z = dinv r in place ; reduce r.z
+template <typename T, int ND>
+__global__ void __launch_bounds__(BS) k_kj_to_z(T *r, const uint16_t *__restrict__ keys,
+                                                const T *__restrict__ kinv, size_t N, Reduce red)
+{
+    double acc = 0;
+    GROUP_RANGE(N)
+    FOR_GROUPS
+    {
+        const size_t o = g_ * 4 * ND;
+        Group<T, ND> rv, m;
+        gload<T, ND>(r + o, rv);
+        group_dinv<T, ND>(kinv, keys, g_, m);
+#pragma unroll
+        for (int k = 0; k < 4 * ND; k++) {
+            const T zk = m.v[k] * rv.v[k];
+            acc += (double)rv.v[k] * (double)zk;
+            rv.v[k] = zk;
+        }
+        gstore<T, ND>(r + o, rv);
+    }
+    FOR_TAIL_NODES(N)
+    {
+        T mv[ND];
+        node_dinv<T, ND>(kinv, keys[i], mv);
+#pragma unroll
+        for (int c = 0; c < ND; c++) {
+            const T zk = mv[c] * r[ND * i + c];
+            acc += (double)r[ND * i + c] * (double)zk;
+            r[ND * i + c] = zk;
         }
     }
     double vv[1] = {acc};
@@ -1147,13 +1227,24 @@ void launch_kj_pcg_update(const OpDesc &op, T *x, T *r, const T *d, const T *q, 
     const int gr = group_grid(op, sizeof(T));
     const T *kinv = (const T *)op.kinv;
     const size_t N = op.N;
-    if (op.physics == PHY_ELASTIC) {
-        if (x) k_kj_pcg_update<T, 3, true><<<gr, BS, 0, s>>>(x, r, d, q, op.keys, kinv, N, *red);
-        else k_kj_pcg_update<T, 3, false><<<gr, BS, 0, s>>>(x, r, d, q, op.keys, kinv, N, *red);
-    } else {
-        if (x) k_kj_pcg_update<T, 1, true><<<gr, BS, 0, s>>>(x, r, d, q, op.keys, kinv, N, *red);
-        else k_kj_pcg_update<T, 1, false><<<gr, BS, 0, s>>>(x, r, d, q, op.keys, kinv, N, *red);
-    }
+    if (op.physics == PHY_ELASTIC) k_kj_pcg_update<T, 3, true><<<gr, BS, 0, s>>>(x, r, d, q, op.keys, kinv, N, *red);
+    else k_kj_pcg_update<T, 1, true><<<gr, BS, 0, s>>>(x, r, d, q, op.keys, kinv, N, *red);
+    launch_reduce_finish(*red, gr, 1, s);
+}
+
+template <typename T>
+void launch_kj_zupdate(const OpDesc &op, T *z, const T *q, const Reduce *red, hipStream_t s)
+{
+    const int gr = group_grid(op, sizeof(T));
+    KJ_DISPATCH(k_kj_zupdate, gr, z, q, op.keys, (const T *)op.kinv, (size_t)op.N, *red);
+    launch_reduce_finish(*red, gr, 1, s);
+}
+
+template <typename T>
+void launch_kj_to_z(const OpDesc &op, T *r, const Reduce *red, hipStream_t s)
+{
+    const int gr = group_grid(op, sizeof(T));
+    KJ_DISPATCH(k_kj_to_z, gr, r, op.keys, (const T *)op.kinv, (size_t)op.N, *red);
     launch_reduce_finish(*red, gr, 1, s);
 }
 
@@ -1248,7 +1339,9 @@ void launch_interpl2(const double *coarse, double *fine, int rows, int cols, int
     template void launch_kj_pcg_update<T>(const OpDesc &, T *, T *, const T *, const T *, const Reduce *,     \
                                           hipStream_t);                                                       \
     template void launch_kj_sub_precond<T>(const OpDesc &, T *, const T *, T *, const Reduce *, hipStream_t); \
-    template void launch_axpy_alpha<T>(T *, const T *, const double *, size_t, hipStream_t);
+    template void launch_axpy_alpha<T>(T *, const T *, const double *, size_t, hipStream_t);                 \
+    template void launch_kj_zupdate<T>(const OpDesc &, T *, const T *, const Reduce *, hipStream_t);          \
+    template void launch_kj_to_z<T>(const OpDesc &, T *, const Reduce *, hipStream_t);
 INST(float)
 INST(double)
 template void launch_convert<double, float>(const double *, float *, size_t, hipStream_t);

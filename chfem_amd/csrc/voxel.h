@@ -72,16 +72,18 @@ inline bool isotropic_params(const double *K, double &lam, double &mu)
 }
 
 // Inverse Jacobi diagonal of every 16-bit packed key (2 bits per local node,
-// at most 4 materials): 1 / sum_e K[m_e][e][a] summed over e = 0..7 in FP64.
+// at most 4 materials): 1 / sum_e K[m_e][e][a] summed over e = 0..7 in FP64,
+// at [key * nd + a]; the diagonal itself (the sum) at [65536 * nd + key * nd + a].
 // K: dense (8 nd)^2 per material; entries of keys with a material >= nmat
-// stay 0.  t64/t32 get 65536 * nd entries.
+// stay 0.  t64/t32 get 2 * 65536 * nd entries.
 template <typename V64, typename V32>
 inline void inverse_diagonal_table(const double *K, unsigned nmat, int nd, V64 &t64, V32 &t32)
 {
     const int ke = 8 * nd;
     const size_t ksz = (size_t)ke * ke;
-    t64.assign((size_t)65536 * nd, 0.0);
-    t32.assign((size_t)65536 * nd, 0.f);
+    const size_t half = (size_t)65536 * nd;
+    t64.assign(2 * half, 0.0);
+    t32.assign(2 * half, 0.f);
     for (unsigned key = 0; key < 65536; key++) {
         bool valid = true;
         for (int e = 0; e < 8; e++) valid = valid && ((key >> (2 * e)) & 3u) < nmat;
@@ -91,6 +93,8 @@ inline void inverse_diagonal_table(const double *K, unsigned nmat, int nd, V64 &
             for (int e = 0; e < 8; e++) sum += K[(size_t)((key >> (2 * e)) & 3u) * ksz + (size_t)(nd * e + a) * ke + nd * e + a];
             t64[(size_t)key * nd + a] = 1.0 / sum;
             t32[(size_t)key * nd + a] = (float)(1.0 / sum);
+            t64[half + (size_t)key * nd + a] = sum;
+            t32[half + (size_t)key * nd + a] = (float)sum;
         }
     }
 }

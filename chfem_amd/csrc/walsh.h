@@ -136,17 +136,23 @@ __device__ __forceinline__ void element_forces(const T (&u)[8][3], T lam, T mu, 
 
 __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
-// Direction update fused into the operator (Jacobi-PCG with key-Jacobi):
-// the operator's input d = dinv r + beta d_old is formed while the node
-// planes are fetched (dinv from the node's packed key, kinv[key * 3 + a]),
-// q = A d is computed from it, and d is written to dnew for the nodes the
-// block owns.  The previous iteration's iterate update is deferred to here
-// too: x += alpha d_old on the owned nodes (alpha = 0 leaves x as it is).
-// v is d_old; first: d = dinv r (v unused).
+// Keeps a value's computation at this point of the instruction stream (an
+// empty asm that consumes it; no instruction is emitted).  Used where the
+// compiler would sink a decode of loaded data below later stores, whose wait
+// (vmcnt counts stores too) would then also wait for the stores.
+__device__ __forceinline__ void pin_here(int v) { asm volatile("" ::"v"(v)); }
+
+// Direction update fused into the operator.  The CG loop that launches this
+// form iterates on the preconditioned residual z = M^-1 r (Jacobi from the
+// packed keys in its update kernel), so the operator's input
+// d = z + beta d_old is formed from two streamed vectors while the node
+// planes are fetched; q = A d is computed from it, and d is written to dnew
+// for the nodes the block owns.  The previous iteration's iterate update is
+// deferred to here too: x += alpha d_old on the owned nodes (alpha = 0 leaves
+// x as it is).  v is d_old; first: d = z (v unused).
 template <typename T>
 struct DirFuse {
-    const T *r;
-    const T *kinv;
+    const T *z;
     T *dnew;
     T *x;
     T beta;
@@ -213,47 +219,30 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
         return lw < 0 ? lw + nz : lw;
     };
     T pre[SLOTS];
-    // FUSE: the direction update's operands are fetched with the plane and
-    // combined when the plane is stored to LDS.  The packed keys are loaded
-    // one layer ahead (kn), so the dependent kinv gather of a layer does not
-    // wait on its key load.  On the first iteration (no d_old) the d_old
-    // operand reads r and beta is 0.
-    T pr[FUSE ? SLOTS : 1], pm[FUSE ? SLOTS : 1];
-    uint32_t kn[FUSE ? SLOTS : 1];
+    // FUSE: the direction update's operands (z and d_old) are fetched with
+    // the plane and combined when the plane is stored to LDS; d_old is kept
+    // in LDS as well, for the owned nodes' x update.  On the first iteration
+    // (no d_old) the d_old operand reads z and beta is 0.
+    T pz[FUSE ? SLOTS : 1];
     const T *dsrc = v;
     T beta = T(0);
     if constexpr (FUSE) {
-        dsrc = fz.first ? fz.r : v;
+        dsrc = fz.first ? fz.z : v;
         beta = fz.first ? T(0) : fz.beta;
     }
-    auto load_keys = [&](int lg) {
-        if constexpr (FUSE) {
-            const uint16_t *kb = keys + (size_t)wrapl(lg) * nxy;
-#pragma unroll
-            for (int k = 0; k < SLOTS; k++) kn[k] = kb[goff[k] / 3u];
-        }
-    };
     auto fetch = [&](int lg) {
         const size_t lo = 3 * (size_t)wrapl(lg) * nxy;
-        if constexpr (FUSE) {
 #pragma unroll
-            for (int k = 0; k < SLOTS; k++) {
-                const uint32_t o = goff[k];
-                pm[k] = fz.kinv[3u * kn[k] + (o - 3u * (o / 3u))];
-                pr[k] = fz.r[lo + o];
-                pre[k] = dsrc[lo + o];
-            }
-            load_keys(lg + 1);
-        } else {
-#pragma unroll
-            for (int k = 0; k < SLOTS; k++) pre[k] = v[lo + goff[k]];
+        for (int k = 0; k < SLOTS; k++) {
+            if constexpr (FUSE) pz[k] = fz.z[lo + goff[k]];
+            pre[k] = dsrc[lo + goff[k]];
         }
     };
     auto store = [&](T *dst, T *ddst) {
 #pragma unroll
         for (int k = 0; k < SLOTS; k++) {
             if constexpr (FUSE) {
-                dst[tid + k * NT] = pm[k] * pr[k] + beta * pre[k];
+                dst[tid + k * NT] = pz[k] + beta * pre[k];
                 ddst[tid + k * NT] = pre[k];
             } else {
                 dst[tid + k * NT] = pre[k];
@@ -316,7 +305,6 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
 
     T *cur = su[0], *nxt = su[1];
     T *dcur = sdo[0], *dnxt = sdo[FUSE ? 1 : 0];  // d_old planes, swapped with cur / nxt
-    load_keys(l0 - 1);
     fetch(l0 - 1);
     uint32_t mk = mat_key(l0 - 1);
     store(cur, dcur);
@@ -342,8 +330,21 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
     }
     __syncthreads();
     double dsum = 0.0;
+    // this work-item's node offset in a layer (work-items that own no node
+    // use node 0 for their unconditional loads; only owners store)
+    const size_t own_off = own ? (size_t)(c0 + tc - 1) * ny + (r0 + tr - 1) : 0;
+    // FUSE: x of the owned node, loaded one layer ahead for the deferred
+    // iterate update, which is stored at the top of its layer's step (its
+    // store then completes during the step instead of holding up the next)
+    T pxn[FUSE ? 3 : 1];
+    if constexpr (FUSE) {
+        const size_t o0 = 3 * ((size_t)l0 * nxy + own_off);
+#pragma unroll
+        for (int c = 0; c < 3; c++) pxn[c] = fz.x[o0 + c];
+    }
     for (int s = 0; s < vl; s++) {
         const int l = l0 + s;
+        const size_t on = 3 * ((size_t)l * nxy + own_off);
         // loads for the next steps are issued first and consumed after this
         // layer's elements: element materials of layer l + 1, plane l + 2
         // (materials first: the wait for the plane then covers them, and no
@@ -351,31 +352,32 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
         // them too (in bounds, unused): every step then has the same loads
         // and none is pending across the loop's back edge.
         mk = mat_key(l + 1);
-        // FUSE: x of this work-item's node at layer l for the deferred
-        // iterate update (work-items that own no node read node 0 of the
-        // layer; only owners store); its d_old comes from the LDS plane
-        T px[FUSE ? 3 : 1], pd[FUSE ? 3 : 1];
-        const size_t on = 3 * ((size_t)l * nxy + (own ? (size_t)(c0 + tc - 1) * ny + (r0 + tr - 1) : 0));
-        if constexpr (FUSE) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) px[c] = fz.x[on + c];
-        }
-        fetch(l + 2);
         T own_u[3] = {0, 0, 0};
         if ((DOT || FUSE) && own) {
             const T *uu = cur + (tc * NR + tr) * 3;
             own_u[0] = uu[0]; own_u[1] = uu[1]; own_u[2] = uu[2];
-            if constexpr (FUSE) {
-                const T *dd = dcur + (tc * NR + tr) * 3;
-                pd[0] = dd[0]; pd[1] = dd[1]; pd[2] = dd[2];
-            }
         }
+        if constexpr (FUSE) {
+            // deferred x += alpha d_old of layer l (d_old from the LDS plane)
+            if (own) {
+                const T *dd = dcur + (tc * NR + tr) * 3;
+                T *xo = fz.x + on;
+                xo[0] = pxn[0] + fz.alpha * dd[0];
+                xo[1] = pxn[1] + fz.alpha * dd[1];
+                xo[2] = pxn[2] + fz.alpha * dd[2];
+            }
+            const size_t o1 = 3 * ((size_t)wrapl(l + 1) * nxy + own_off);
+#pragma unroll
+            for (int c = 0; c < 3; c++) pxn[c] = fz.x[o1 + c];
+        }
+        fetch(l + 2);
         element(cur, nxt, m, A, Af);
         __syncthreads();
         // the prefetched plane goes to LDS before this layer's node results
         // are stored, so waiting for the loads does not wait for the stores
         store(cur, dcur);  // layer l + 2 replaces layer l
         m = material(mk);
+        pin_here(m);
         if (own) {
             T o[3];
 #pragma unroll
@@ -388,10 +390,6 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
             if constexpr (FUSE) {
                 T *dn = fz.dnew + on;
                 dn[0] = own_u[0]; dn[1] = own_u[1]; dn[2] = own_u[2];
-                T *xo = fz.x + on;
-                xo[0] = px[0] + fz.alpha * pd[0];
-                xo[1] = px[1] + fz.alpha * pd[1];
-                xo[2] = px[2] + fz.alpha * pd[2];
             }
             if (DOT) dsum += (double)own_u[0] * o[0] + (double)own_u[1] * o[1] + (double)own_u[2] * o[2];
         }

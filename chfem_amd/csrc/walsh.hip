@@ -38,7 +38,7 @@ void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, con
 {
     Reduce rr = {nullptr, nullptr, 0, 0, 0};
     if (red) rr = *red;
-    DirFuse<T> f = {nullptr, nullptr, nullptr, nullptr, T(0), T(0), 1, 0};
+    DirFuse<T> f = {nullptr, nullptr, nullptr, T(0), T(0), 1, 0};
     if (fz) f = *fz;
     int LZ = t.LZ > 0 ? t.LZ : 16;
     {
@@ -77,10 +77,10 @@ bool apply_dir_ok(const OpDesc &op)
 }
 
 template <typename T>
-void launch_apply_dir(const OpDesc &op, const T *r, const T *dold, T *dnew, T *q, T *x, int xpend,
+void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q, T *x, int xpend,
                       const Reduce *red, hipStream_t s)
 {
-    const DirFuse<T> f = {r, (const T *)op.kinv, dnew, x, T(0), T(0), dold ? 0 : 1, dold ? xpend : 0};
+    const DirFuse<T> f = {z, dnew, x, T(0), T(0), dold ? 0 : 1, dold ? xpend : 0};
     const WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
     launch_elastic_walsh<T>(op, (const T *)op.lamu, dold, q, red, t, s, &f);
 }
