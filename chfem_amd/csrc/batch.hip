@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(TR *TC) k_apply_elastic(const Sys *__restrict_
     if (!selected<T>(Sb, t.sys, mode)) return;
     const Sys &S = sys[t.sys];
     const T *lamu = (const T *)(sizeof(T) == 4 ? S.mat32 : S.mat64);
-    const double d = walsh_tile<T, TR, TC, true>(v + S.off, q + S.off, S.keys, lamu, 0, S.ny, S.nx, S.nz, S.LZ,
+    const double d = walsh_tile<T, 3, TR, TC, true>(v + S.off, q + S.off, S.keys, lamu, 0, S.ny, S.nx, S.nz, S.LZ,
                                                   t.bx, t.by, t.bz);
     const double s = bsum(d);
     if (threadIdx.x == 0) part[blockIdx.x] = s;

@@ -1220,23 +1220,27 @@ cudapcgFlag_t cudapcgSetLclMtxs(cudapcgVar_t *L)
         e1 = hipMemcpy(g->dm.K64, K, ksz * nk * sizeof(double), hipMemcpyHostToDevice);
         e2 = hipMemcpy(g->dm.K32, Kf, ksz * nk * sizeof(float), hipMemcpyHostToDevice);
     }
-    free(K);
     free(Kf);
     if (e1 != hipSuccess || e2 != hipSuccess) {
+        free(K);
         fprintf(stderr, "chfem: failed to upload local matrices\n");
         return CUDAPCG_FALSE;
     }
     g->dm.ksz = ksz;
     g->dm.nmat = nk;
     g->dinv_valid = false;
-    // isotropic elastic materials: the operator can use the Walsh-basis kernel
+    // isotropic elastic or conduction materials: the operator can use the
+    // Walsh-basis kernel (per material {lambda, mu} or {k, 0})
     g->walsh_ok = false;
     g->tuned = false;
-    if (ph == PHY_ELASTIC && nk <= 4) {
+    if ((ph == PHY_ELASTIC || ph == PHY_THERMAL) && nk <= 4) {
         double lm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         float lmf[8];
         bool ok = true;
-        for (unsigned m = 0; m < nk && ok; m++) ok = voxel::isotropic_params(L + (size_t)m * 576, lm[2 * m], lm[2 * m + 1]);
+        for (unsigned m = 0; m < nk && ok; m++) {
+            if (ph == PHY_ELASTIC) ok = voxel::isotropic_params(L + (size_t)m * 576, lm[2 * m], lm[2 * m + 1]);
+            else ok = voxel::conduction_param(K + (size_t)m * 64, lm[2 * m]);
+        }
         for (int i = 0; i < 8; i++) lmf[i] = (float)lm[i];
         if (ok) {
             if (!g->lamu64) CHECK_HIP(hipMalloc(&g->lamu64, 8 * sizeof(double)));
@@ -1246,6 +1250,7 @@ cudapcgFlag_t cudapcgSetLclMtxs(cudapcgVar_t *L)
             g->walsh_ok = true;
         }
     }
+    free(K);
     return CUDAPCG_TRUE;
 }
 

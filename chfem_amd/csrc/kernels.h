@@ -64,8 +64,9 @@ struct OpDesc {
     const uint32_t *dof2node;  // DOF id -> periodic node
     const uint32_t *node2dof;  // periodic node -> DOF id
     const uint8_t *bkeys;      // border node fluid keys
-    // isotropic elastic materials: {lambda, mu} per material (precision of K);
-    // non-null selects the Walsh-basis operator (walsh.hip)
+    // isotropic elastic materials: {lambda, mu} per material, or conduction:
+    // {k, 0} per material (precision of K); non-null selects the Walsh-basis
+    // operator (walsh.hip)
     const void *lamu;
     int tile_tr, tile_tc, tile_lz;
     // inverse Jacobi diagonal per packed key, kinv[key * nd + a] (65536 keys,
@@ -156,20 +157,21 @@ int reduce_blocks(size_t n);
 }  // namespace chfem
 
 namespace chfem {
-// Isotropic elastic materials in the form the Walsh-basis operator uses:
-// per material {lambda, mu}.  Filled when every local matrix equals
-// lambda*A + mu*B for the unit-voxel trilinear element.
+// Walsh-basis operators (walsh.hip) for isotropic elastic materials (lamu:
+// {lambda, mu} per material, when every local matrix equals lambda*A + mu*B
+// for the unit-voxel trilinear element) and for conduction (lamu: {k, 0} per
+// material, when every local matrix is k times the unit conduction matrix).
 struct WalshTiles { int TR, TC, LZ; };
 template <typename T> struct DirFuse;
-// fz non-null: v is d_old and the launch forms d = dinv r + beta d_old
+// fz non-null: v is d_old and the launch forms d = z + beta d_old
 // (walsh.h DirFuse), writes it to fz->dnew and reduces d.q (red required)
 template <typename T>
-void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduce *red, WalshTiles t,
-                          hipStream_t s, const DirFuse<T> *fz = nullptr);
+void launch_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduce *red, WalshTiles t, hipStream_t s,
+                  const DirFuse<T> *fz = nullptr);
 
 // Jacobi-PCG step on the preconditioned residual z = dinv r with the
-// direction update inside the operator (isotropic elastic NBN with
-// key-Jacobi, see apply_dir_ok): d = z + beta d_old (beta =
+// direction update inside the operator (Walsh-basis thermal or isotropic
+// elastic NBN with key-Jacobi, see apply_dir_ok): d = z + beta d_old (beta =
 // S[DELTA]/S[DELTA_OLD]; dold == nullptr: d = z), dnew <- d, q = A d,
 // red <- d.q, and the previous iteration's deferred iterate update
 // x += S[ALPHA] d_old when xpend (and dold) is set (x is rewritten

@@ -1116,9 +1116,10 @@ void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStre
         launch_ebe_mfma<T>(op, v, q, red, s);
         return;
     }
-    if (op.physics == PHY_ELASTIC && op.lamu) {
+    if (op.lamu) {
+        // Walsh-basis elastic (isotropic) or conduction operator, NBN or EBE
         WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
-        launch_elastic_walsh<T>(op, (const T *)op.lamu, v, q, red, t, s);
+        launch_walsh<T>(op, (const T *)op.lamu, v, q, red, t, s);
         return;
     }
     if (op.ebe) {

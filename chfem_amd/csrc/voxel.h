@@ -71,6 +71,26 @@ inline bool isotropic_params(const double *K, double &lam, double &mu)
     return kmax > 0.0 && dmax <= 1e-12 * kmax;
 }
 
+// Conductivity k of an 8x8 matrix if it is the unit-voxel trilinear conduction
+// matrix k * {1/3 on the diagonal, 0 between nodes that differ in one
+// coordinate, -1/12 between nodes that differ in two or three}
+// (local node order of DR/DC/DL).
+inline bool conduction_param(const double *K, double &k)
+{
+    static const int dr[8] = {1, 1, 0, 0, 1, 1, 0, 0}, dc[8] = {0, 1, 1, 0, 0, 1, 1, 0}, dl[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    k = 3.0 * K[0];
+    double kmax = 0.0, dmax = 0.0;
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 8; j++) {
+            const int diff = (dr[i] != dr[j]) + (dc[i] != dc[j]) + (dl[i] != dl[j]);
+            const double r = diff == 0 ? k / 3.0 : (diff == 1 ? 0.0 : -k / 12.0);
+            const double v = K[i * 8 + j];
+            kmax = fabs(v) > kmax ? fabs(v) : kmax;
+            dmax = fabs(v - r) > dmax ? fabs(v - r) : dmax;
+        }
+    return kmax > 0.0 && dmax <= 1e-12 * kmax;
+}
+
 // Inverse Jacobi diagonal of every 16-bit packed key (2 bits per local node,
 // at most 4 materials): 1 / sum_e K[m_e][e][a] summed over e = 0..7 in FP64,
 // at [key * nd + a]; the diagonal itself (the sum) at [65536 * nd + key * nd + a].

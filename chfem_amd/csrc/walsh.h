@@ -1,6 +1,7 @@
-// walsh.h -- elastic operator q = A v for isotropic materials, computed
-// element by element in the Walsh (hierarchical) basis of the trilinear voxel
-// (device code shared by walsh.hip and the batched solver, batch.hip).
+// walsh.h -- operators q = A v for isotropic elastic and for conduction
+// materials, computed element by element in the Walsh (hierarchical) basis of
+// the trilinear voxel (device code shared by walsh.hip and the batched
+// solver, batch.hip).
 //
 // For an element with nodal displacements u^a_k (component a, local node k)
 // the Walsh sums W^a_s = sum_k s_k u^a_k over the sign patterns
@@ -134,6 +135,30 @@ __device__ __forceinline__ void element_forces(const T (&u)[8][3], T lam, T mu, 
     }
 }
 
+// Conduction element forces f = K u for conductivity k.  The Walsh modes
+// diagonalise the unit-voxel trilinear conduction matrix: with u written as
+// (1/8) sum_s W_s prod_{d in s} 2(x_d - 1/2), the gradients of the modes are
+// orthogonal on the cube and u^T K u = k sum_s a_s W_s^2 with a_s = 1/16,
+// 1/24, 1/48 for modes of 1, 2, 3 axes, so F_s = k a_s W_s and f = iwht(F).
+// This reproduces k * {1/3 diagonal, 0 between nodes that differ in one
+// coordinate, -1/12 across face and body diagonals}
+// (chfem/src/femhmg/physical_phenomena/femhmg_thermal_3D.c:42-120).
+template <typename T>
+__device__ __forceinline__ void conduction_forces(const T (&u)[8], T k, T (&f)[8])
+{
+    const Walsh<T> W = wht(u);
+    const T a1 = k * T(1.0 / 16.0), a2 = k * T(1.0 / 24.0), a3 = k * T(1.0 / 48.0);
+    Walsh<T> F;
+    F.x = a1 * W.x;
+    F.y = a1 * W.y;
+    F.z = a1 * W.z;
+    F.xy = a2 * W.xy;
+    F.yz = a2 * W.yz;
+    F.xz = a2 * W.xz;
+    F.xyz = a3 * W.xyz;
+    iwht(F, f);
+}
+
 __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
 // Keeps a value's computation at this point of the instruction stream (an
@@ -170,20 +195,25 @@ struct DirFuse {
 // while the current layer's elements are evaluated, and the far-face forces of
 // each node are carried to the next step in registers.
 //
+// ND = 3: isotropic elastic (lamu holds {lambda, mu} per material); ND = 1:
+// conduction (lamu holds {k, 0} per material).  Vectors hold ND values per
+// node, [ND * node + a].
+//
 // walsh_tile computes the tile (bx, by, bz) of one operator application and
-// returns this work-item's share of v.q (DOT); k_elastic_walsh maps tiles to
-// blocks one to one, the batched solver (batch.hip) through a tile table.
-// FUSE: the input is the fused direction update of DirFuse (the dot is then
-// d.q with the updated d).
-template <typename T, int TR, int TC, bool DOT, bool FUSE = false>
+// returns this work-item's share of v.q (DOT); k_walsh maps tiles to blocks
+// one to one, the batched solver (batch.hip) through a tile table.  FUSE: the
+// input is the fused direction update of DirFuse (the dot is then d.q with
+// the updated d).
+template <typename T, int ND, int TR, int TC, bool DOT, bool FUSE = false>
 __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restrict__ q,
                                              const uint16_t *__restrict__ keys, const T *__restrict__ lamu, int ebe,
                                              int ny, int nx, int nz, int LZ, int bx, int by, int bz,
                                              DirFuse<T> fz = DirFuse<T>{})
 {
-    constexpr int NT = TR * TC, NR = TR + 1, NC = TC + 1, NV = NR * NC * 3;
+    static_assert(ND == 1 || ND == 3, "1 (conduction) or 3 (elastic) values per node");
+    constexpr int NT = TR * TC, NR = TR + 1, NC = TC + 1, NV = NR * NC * ND;
     constexpr int SLOTS = (NV + NT - 1) / NT;
-    constexpr int SB = 7;  // padded stride of the row-combined forces in LDS (no bank conflicts)
+    constexpr int SB = 2 * ND + 1;  // padded stride of the row-combined forces in LDS (no bank conflicts)
     // node planes padded to SLOTS * NT entries: every work-item stores every
     // slot, so the plane loads and stores need no per-slot branches (a
     // branch around a load makes the compiler wait for all loads in flight
@@ -200,7 +230,7 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
     const size_t nxy = (size_t)nx * ny;
     if (tid < 8) smat[tid] = lamu[tid];
 
-    // node-layer load pattern: LDS index idx = (nc * NR + nr) * 3 + comp holds node
+    // node-layer load pattern: LDS index idx = (nc * NR + nr) * ND + comp holds node
     // (r0 - 1 + nr, c0 - 1 + nc); the in-layer global offset is the same for every
     // layer.  Slots outside the tile's nodes load node 0 of the layer (in bounds);
     // what they store to LDS is never read.
@@ -208,10 +238,10 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
 #pragma unroll
     for (int k = 0; k < SLOTS; k++) {
         const int idx = tid + k * NT;
-        const int comp = idx % 3, t2 = idx / 3, nr = t2 % NR, nc = t2 / NR;
+        const int comp = idx % ND, t2 = idx / ND, nr = t2 % NR, nc = t2 / NR;
         const bool ok = idx < NV && nr <= vr + 1 && nc <= vc + 1;
         const int gr = wrapi(r0 - 1 + nr, ny), gc = wrapi(c0 - 1 + nc, nx);
-        goff[k] = ok ? 3u * ((uint32_t)gc * (uint32_t)ny + (uint32_t)gr) + (uint32_t)comp : 0u;
+        goff[k] = ok ? (uint32_t)ND * ((uint32_t)gc * (uint32_t)ny + (uint32_t)gr) + (uint32_t)comp : 0u;
     }
     // layer index modulo nz for any lg (the prefetches run up to the last layer + 2)
     auto wrapl = [&](int lg) {
@@ -231,7 +261,7 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
         beta = fz.first ? T(0) : fz.beta;
     }
     auto fetch = [&](int lg) {
-        const size_t lo = 3 * (size_t)wrapl(lg) * nxy;
+        const size_t lo = ND * (size_t)wrapl(lg) * nxy;
 #pragma unroll
         for (int k = 0; k < SLOTS; k++) {
             if constexpr (FUSE) pz[k] = fz.z[lo + goff[k]];
@@ -270,33 +300,43 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
     // and from element layer l - 1 (k = 4..7) the same with f7, f4 (Af) and
     // f6, f5 (Bf).  E(tr-1, tc) is lane - 1 of the same wavefront (TR is 16,
     // 32 or 64, and row 0 only feeds nodes no work-item owns), so A and Af
-    // are formed with cross-lane moves; B and Bf go through LDS (6 values per
-    // element).
+    // are formed with cross-lane moves; B and Bf go through LDS (2 ND values
+    // per element).
     static_assert(TR == 16 || TR == 32 || TR == 64, "rows per tile must tile a 64-wide wavefront");
-    auto element = [&](const T *cur, const T *nxt, int m, T (&A)[3], T (&Af)[3]) {
-        T f[8][3];
+    auto element = [&](const T *cur, const T *nxt, int m, T (&A)[ND], T (&Af)[ND]) {
+        T f[8][ND];
         if (elem_ok) {
-            T u[8][3];
+            T u[8][ND];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 const T *src = WDL(k) ? nxt : cur;
-                const int o = ((tc + WDC(k)) * NR + tr + WDR(k)) * 3;
-                u[k][0] = src[o];
-                u[k][1] = src[o + 1];
-                u[k][2] = src[o + 2];
+                const int o = ((tc + WDC(k)) * NR + tr + WDR(k)) * ND;
+#pragma unroll
+                for (int a = 0; a < ND; a++) u[k][a] = src[o + a];
             }
-            element_forces(u, smat[2 * m], smat[2 * m + 1], f);
+            if constexpr (ND == 3) {
+                element_forces(u, smat[2 * m], smat[2 * m + 1], f);
+            } else {
+                T u1[8], f1[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) u1[k] = u[k][0];
+                conduction_forces(u1, smat[2 * m], f1);
+#pragma unroll
+                for (int k = 0; k < 8; k++) f[k][0] = f1[k];
+            }
         } else {
 #pragma unroll
-            for (int k = 0; k < 8; k++) f[k][0] = f[k][1] = f[k][2] = T(0);
+            for (int k = 0; k < 8; k++)
+#pragma unroll
+                for (int a = 0; a < ND; a++) f[k][a] = T(0);
         }
         T *dst = sb + tid * SB;
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
+        for (int a = 0; a < ND; a++) {
             A[a] = f[3][a] + __shfl_up(f[0][a], 1);
             Af[a] = f[7][a] + __shfl_up(f[4][a], 1);
             dst[a] = f[2][a] + __shfl_up(f[1][a], 1);
-            dst[3 + a] = f[6][a] + __shfl_up(f[5][a], 1);
+            dst[ND + a] = f[6][a] + __shfl_up(f[5][a], 1);
         }
     };
     const bool own = tr >= 1 && tc >= 1 && tr - 1 < vr && tc - 1 < vc;
@@ -314,14 +354,12 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
     mk = mat_key(l0);
     __syncthreads();
     fetch(l0 + 1);
-    T A[3], Af[3];
+    T A[ND], Af[ND];
     element(cur, nxt, m, A, Af);
     __syncthreads();
-    T acc[3] = {0, 0, 0};
-    if (own) {
+    T acc[ND];
 #pragma unroll
-        for (int a = 0; a < 3; a++) acc[a] = Af[a] + bleft[3 + a];
-    }
+    for (int a = 0; a < ND; a++) acc[a] = own ? Af[a] + bleft[ND + a] : T(0);
     store(cur, dcur);  // layer l0 + 1 replaces layer l0 - 1
     m = material(mk);
     {
@@ -336,15 +374,15 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
     // FUSE: x of the owned node, loaded one layer ahead for the deferred
     // iterate update, which is stored at the top of its layer's step (its
     // store then completes during the step instead of holding up the next)
-    T pxn[FUSE ? 3 : 1];
+    T pxn[FUSE ? ND : 1];
     if constexpr (FUSE) {
-        const size_t o0 = 3 * ((size_t)l0 * nxy + own_off);
+        const size_t o0 = ND * ((size_t)l0 * nxy + own_off);
 #pragma unroll
-        for (int c = 0; c < 3; c++) pxn[c] = fz.x[o0 + c];
+        for (int a = 0; a < ND; a++) pxn[a] = fz.x[o0 + a];
     }
     for (int s = 0; s < vl; s++) {
         const int l = l0 + s;
-        const size_t on = 3 * ((size_t)l * nxy + own_off);
+        const size_t on = ND * ((size_t)l * nxy + own_off);
         // loads for the next steps are issued first and consumed after this
         // layer's elements: element materials of layer l + 1, plane l + 2
         // (materials first: the wait for the plane then covers them, and no
@@ -352,23 +390,25 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
         // them too (in bounds, unused): every step then has the same loads
         // and none is pending across the loop's back edge.
         mk = mat_key(l + 1);
-        T own_u[3] = {0, 0, 0};
+        T own_u[ND];
+#pragma unroll
+        for (int a = 0; a < ND; a++) own_u[a] = T(0);
         if ((DOT || FUSE) && own) {
-            const T *uu = cur + (tc * NR + tr) * 3;
-            own_u[0] = uu[0]; own_u[1] = uu[1]; own_u[2] = uu[2];
+            const T *uu = cur + (tc * NR + tr) * ND;
+#pragma unroll
+            for (int a = 0; a < ND; a++) own_u[a] = uu[a];
         }
         if constexpr (FUSE) {
             // deferred x += alpha d_old of layer l (d_old from the LDS plane)
             if (own) {
-                const T *dd = dcur + (tc * NR + tr) * 3;
+                const T *dd = dcur + (tc * NR + tr) * ND;
                 T *xo = fz.x + on;
-                xo[0] = pxn[0] + fz.alpha * dd[0];
-                xo[1] = pxn[1] + fz.alpha * dd[1];
-                xo[2] = pxn[2] + fz.alpha * dd[2];
-            }
-            const size_t o1 = 3 * ((size_t)wrapl(l + 1) * nxy + own_off);
 #pragma unroll
-            for (int c = 0; c < 3; c++) pxn[c] = fz.x[o1 + c];
+                for (int a = 0; a < ND; a++) xo[a] = pxn[a] + fz.alpha * dd[a];
+            }
+            const size_t o1 = ND * ((size_t)wrapl(l + 1) * nxy + own_off);
+#pragma unroll
+            for (int a = 0; a < ND; a++) pxn[a] = fz.x[o1 + a];
         }
         fetch(l + 2);
         element(cur, nxt, m, A, Af);
@@ -379,19 +419,24 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
         m = material(mk);
         pin_here(m);
         if (own) {
-            T o[3];
+            T o[ND];
 #pragma unroll
-            for (int a = 0; a < 3; a++) {
+            for (int a = 0; a < ND; a++) {
                 o[a] = acc[a] + A[a] + bleft[a];
-                acc[a] = Af[a] + bleft[3 + a];
+                acc[a] = Af[a] + bleft[ND + a];
             }
             T *qo = q + on;
-            qo[0] = o[0]; qo[1] = o[1]; qo[2] = o[2];
+#pragma unroll
+            for (int a = 0; a < ND; a++) qo[a] = o[a];
             if constexpr (FUSE) {
                 T *dn = fz.dnew + on;
-                dn[0] = own_u[0]; dn[1] = own_u[1]; dn[2] = own_u[2];
+#pragma unroll
+                for (int a = 0; a < ND; a++) dn[a] = own_u[a];
             }
-            if (DOT) dsum += (double)own_u[0] * o[0] + (double)own_u[1] * o[1] + (double)own_u[2] * o[2];
+            if (DOT) {
+#pragma unroll
+                for (int a = 0; a < ND; a++) dsum += (double)own_u[a] * o[a];
+            }
         }
         {
             T *t = cur; cur = nxt; nxt = t;

@@ -1,15 +1,14 @@
-// walsh.hip -- kernel and launcher of the Walsh-basis elastic operator
+// walsh.hip -- kernel and launcher of the Walsh-basis elastic and conduction operators
 // (device code in walsh.h).
 #include <hip/hip_runtime.h>
 #include "walsh.h"
 
 namespace chfem {
 
-template <typename T, int TR, int TC, bool DOT, bool FUSE>
-__global__ void __launch_bounds__(TR *TC) k_elastic_walsh(const T *__restrict__ v, T *__restrict__ q,
-                                                           const uint16_t *__restrict__ keys,
-                                                           const T *__restrict__ lamu, int ebe, int ny, int nx,
-                                                           int nz, int LZ, Reduce red, DirFuse<T> fz)
+template <typename T, int ND, int TR, int TC, bool DOT, bool FUSE>
+__global__ void __launch_bounds__(TR *TC) k_walsh(const T *__restrict__ v, T *__restrict__ q,
+                                                   const uint16_t *__restrict__ keys, const T *__restrict__ lamu,
+                                                   int ebe, int ny, int nx, int nz, int LZ, Reduce red, DirFuse<T> fz)
 {
     if (DOT && stopped(red)) return;
     // beta of the fused direction update, as k_kj_pcg_dir forms it
@@ -24,17 +23,17 @@ __global__ void __launch_bounds__(TR *TC) k_elastic_walsh(const T *__restrict__ 
     const unsigned gx = (unsigned)(ny + TR - 2) / (TR - 1), gy = (unsigned)(nx + TC - 2) / (TC - 1);
     const unsigned b = blockIdx.x, nb = gridDim.x, per = nb / 8u, rem = nb % 8u, xcd = b % 8u, k = b / 8u;
     const unsigned t = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + k;
-    const double dsum = walsh_tile<T, TR, TC, DOT, FUSE>(v, q, keys, lamu, ebe, ny, nx, nz, LZ, (int)(t % gx),
-                                                          (int)((t / gx) % gy), (int)(t / (gx * gy)), fz);
+    const double dsum = walsh_tile<T, ND, TR, TC, DOT, FUSE>(v, q, keys, lamu, ebe, ny, nx, nz, LZ, (int)(t % gx),
+                                                              (int)((t / gx) % gy), (int)(t / (gx * gy)), fz);
     if (DOT) {
         double vv[1] = {dsum};
         store_partials<1>(red, vv);
     }
 }
 
-template <typename T>
-void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduce *red, WalshTiles t,
-                          hipStream_t s, const DirFuse<T> *fz)
+template <typename T, int ND>
+static void launch_walsh_nd(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduce *red, WalshTiles t,
+                            hipStream_t s, const DirFuse<T> *fz)
 {
     Reduce rr = {nullptr, nullptr, 0, 0, 0};
     if (red) rr = *red;
@@ -52,14 +51,14 @@ void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, con
         const unsigned grid = (unsigned)((op.ny + TRv - 2) / (TRv - 1)) * ((op.nx + TCv - 2) / (TCv - 1)) *    \
                               ((op.nz + LZ - 1) / LZ);                                                  \
         if (fz) {                                                                                       \
-            k_elastic_walsh<T, TRv, TCv, true, true><<<grid, TRv * TCv, 0, s>>>(                        \
+            k_walsh<T, ND, TRv, TCv, true, true><<<grid, TRv * TCv, 0, s>>>(                            \
                 v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);                           \
-            launch_reduce_finish(rr, grid, 1, s);                                   \
+            launch_reduce_finish(rr, grid, 1, s);                                                       \
         } else if (red) {                                                                               \
-            k_elastic_walsh<T, TRv, TCv, true, false><<<grid, TRv * TCv, 0, s>>>(                       \
+            k_walsh<T, ND, TRv, TCv, true, false><<<grid, TRv * TCv, 0, s>>>(                           \
                 v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);                           \
-            launch_reduce_finish(rr, grid, 1, s);                                   \
-        } else k_elastic_walsh<T, TRv, TCv, false, false><<<grid, TRv * TCv, 0, s>>>(                   \
+            launch_reduce_finish(rr, grid, 1, s);                                                       \
+        } else k_walsh<T, ND, TRv, TCv, false, false><<<grid, TRv * TCv, 0, s>>>(                       \
                 v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);                           \
         return;                                                                                         \
     }
@@ -70,10 +69,18 @@ void launch_elastic_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, con
 #undef WALSH_LAUNCH
 }
 
+template <typename T>
+void launch_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduce *red, WalshTiles t, hipStream_t s,
+                  const DirFuse<T> *fz)
+{
+    if (op.physics == PHY_ELASTIC) launch_walsh_nd<T, 3>(op, lamu, v, q, red, t, s, fz);
+    else launch_walsh_nd<T, 1>(op, lamu, v, q, red, t, s, fz);
+}
+
 bool apply_dir_ok(const OpDesc &op)
 {
     // the operators launch_apply sends to the Walsh kernel, with key-Jacobi on
-    return op.physics == PHY_ELASTIC && !op.ebe && op.lamu && op.kinv;
+    return op.physics != PHY_FLUID && !op.ebe && op.lamu && op.kinv;
 }
 
 template <typename T>
@@ -82,16 +89,16 @@ void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q
 {
     const DirFuse<T> f = {z, dnew, x, T(0), T(0), dold ? 0 : 1, dold ? xpend : 0};
     const WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
-    launch_elastic_walsh<T>(op, (const T *)op.lamu, dold, q, red, t, s, &f);
+    launch_walsh<T>(op, (const T *)op.lamu, dold, q, red, t, s, &f);
 }
 
 template void launch_apply_dir<float>(const OpDesc &, const float *, const float *, float *, float *, float *, int,
                                       const Reduce *, hipStream_t);
 template void launch_apply_dir<double>(const OpDesc &, const double *, const double *, double *, double *, double *,
                                        int, const Reduce *, hipStream_t);
-template void launch_elastic_walsh<float>(const OpDesc &, const float *, const float *, float *, const Reduce *,
-                                          WalshTiles, hipStream_t, const DirFuse<float> *);
-template void launch_elastic_walsh<double>(const OpDesc &, const double *, const double *, double *, const Reduce *,
-                                           WalshTiles, hipStream_t, const DirFuse<double> *);
+template void launch_walsh<float>(const OpDesc &, const float *, const float *, float *, const Reduce *, WalshTiles,
+                                  hipStream_t, const DirFuse<float> *);
+template void launch_walsh<double>(const OpDesc &, const double *, const double *, double *, const Reduce *,
+                                   WalshTiles, hipStream_t, const DirFuse<double> *);
 
 }  // namespace chfem

@@ -19,7 +19,8 @@ BYTES = {("thermal", 0): 17.0, ("thermal", 1): 9.0, ("elastic", 0): 49.0, ("elas
 # CG-loop form with the direction update and the deferred iterate update
 # inside the operator: image, r, the previous direction and x read, the
 # direction, q and x written
-BYTES_DIR = {0: 1.0 + 6 * 24.0, 1: 1.0 + 6 * 12.0}
+BYTES_DIR = {("thermal", 0): 1.0 + 6 * 8.0, ("thermal", 1): 1.0 + 6 * 4.0,
+             ("elastic", 0): 1.0 + 6 * 24.0, ("elastic", 1): 1.0 + 6 * 12.0}
 
 
 def main():
@@ -32,7 +33,7 @@ def main():
     ap.add_argument("--copy", action="store_true", help="also time the calibration copy kernel")
     ap.add_argument("--dot", action="store_true", help="with the fused v.Av reduction, as inside the solver")
     ap.add_argument("--dir", action="store_true",
-                    help="the CG loop's form: direction update d = M^-1 r + beta d inside the operator (elastic)")
+                    help="the CG loop's form: direction update d = z + beta d inside the operator (Walsh NBN)")
     ap.add_argument("--both", action="store_true", help="time the --dot form, then the --dir form")
     a = ap.parse_args()
     n = a.size
@@ -53,8 +54,8 @@ def main():
     for form in forms:
         ms = L.cudapcgOperatorBench(prec, a.reps, {"plain": 0, "dot": 1, "dir": 2}[form])
         if ms < 0:
-            raise SystemExit("operator bench failed (the direction form needs isotropic elastic NBN)")
-        per_node = BYTES_DIR[prec] if form == "dir" else BYTES[(a.physics, prec)]
+            raise SystemExit("operator bench failed (the direction form needs the Walsh-basis NBN operator)")
+        per_node = BYTES_DIR[(a.physics, prec)] if form == "dir" else BYTES[(a.physics, prec)]
         out[form] = {"avg_launch_ms": ms, "algorithmic_bytes": per_node * N,
                      "GBps": per_node * N / (ms * 1e-3) / 1e9}
     L.hmgRelease()

@@ -147,12 +147,19 @@ def test_mixed_precision_elastic(ragged_elastic):
 
 @pytest.mark.parametrize("tile", ["32x8x16", "16x16x4", "64x4x7", "32x16x3"])
 @pytest.mark.parametrize("shape", [(40, 70, 9), (3, 1, 2), (17, 33, 65)])
-def test_walsh_operator_tiles(tile, shape, monkeypatch):
-    """Walsh-basis elastic operator (isotropic materials) on ragged, tiny and
-    multi-chunk grids for every tile shape, against the oracle's 24x24 form."""
+@pytest.mark.parametrize("which", ["elastic", "thermal"])
+def test_walsh_operator_tiles(which, tile, shape, monkeypatch):
+    """Walsh-basis operators (isotropic elastic, conduction) on ragged, tiny
+    and multi-chunk grids for every tile shape, NBN and EBE (atomic-form
+    strategy, which also routes to the Walsh kernel), against the oracle's
+    dense local matrices."""
     monkeypatch.setenv("CHFEM_WALSH_TILE", tile)
+    monkeypatch.setenv("CHFEM_EBE", "atomic")
     img = three_phase(shape, 21)
-    m = H.model(H.ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25], colors=(0, 100, 255))
+    if which == "elastic":
+        m = H.model(H.ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25], colors=(0, 100, 255))
+    else:
+        m = H.model(H.THERMAL, img, [1.0, 10.0, 3.5], colors=(0, 100, 255))
     for strategy in (0, 1):
         s = H.GpuSolver(m, strategy=strategy)
         try:
@@ -162,10 +169,12 @@ def test_walsh_operator_tiles(tile, shape, monkeypatch):
             s.close()
 
 
-def test_generic_operator_path(ragged_elastic, monkeypatch):
-    """The 24x24 per-material kernel (used for non-isotropic local matrices)."""
+@pytest.mark.parametrize("which", ["elastic", "thermal"])
+def test_generic_operator_path(which, ragged_elastic, ragged_thermal, monkeypatch):
+    """The dense per-material kernels (24x24 elastic, 8x8 conduction; used for
+    local matrices that are not isotropic / unit conduction matrices)."""
     monkeypatch.setenv("CHFEM_OPERATOR", "generic")
-    m = ragged_elastic
+    m = ragged_elastic if which == "elastic" else ragged_thermal
     s = H.GpuSolver(m)
     try:
         v = np.random.default_rng(8).standard_normal(m.ndof)
@@ -268,9 +277,10 @@ def test_ebe_kernels(kernel, which, nphase, monkeypatch):
             s.close()
 
 
+@pytest.mark.parametrize("which", ["elastic", "thermal"])
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("fuse", ["1", "0"])
-def test_fused_direction_update(fuse, precision, monkeypatch):
+def test_fused_direction_update(fuse, precision, which, monkeypatch):
     """The CG direction update d = M^-1 r + beta d done inside the Walsh
     operator (two alternating direction buffers; CHFEM_FUSE_DIR=1, the
     default) and by the separate direction kernel (0), on a grid of several
@@ -280,7 +290,10 @@ def test_fused_direction_update(fuse, precision, monkeypatch):
     monkeypatch.setenv("CHFEM_FUSE_DIR", fuse)
     monkeypatch.setenv("CHFEM_WALSH_TILE", "16x16x4")
     img = three_phase((21, 34, 19), 51)
-    m = H.model(H.ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25], colors=(0, 100, 255))
+    if which == "elastic":
+        m = H.model(H.ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25], colors=(0, 100, 255))
+    else:
+        m = H.model(H.THERMAL, img, [1.0, 10.0, 3.5], colors=(0, 100, 255))
     xr, okr, itr, _ = m.solve(2, tol=1e-10)
     assert okr
     Cr = m.update_c(2, xr)
