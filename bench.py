@@ -210,7 +210,7 @@ def main():
     N = n ** 3
     op_avg_ms = fast["op_ms_total"] / max(fast["op_launches"], 1)
     # Algorithmic bytes per node of the operator launch inside the CG loop.
-    # Fused form (default): 1 B image + r, d_old, x read + d, q, x written
+    # Fused form (default): 1 B image + z, d_old, x read + d, q, x written
     # (6 x 3 values); plain apply: 1 B image + v read + q written.
     fused = os.environ.get("CHFEM_FUSE_DIR", "1") != "0"
     per_node32 = 1.0 + (6 if fused else 2) * 12.0
@@ -219,9 +219,9 @@ def main():
     achieved = bytes_per_launch / (op_avg_ms * 1e-3) / 1e9
     traffic = read_traffic(f"elastic_walsh_{'dir_' if fused else ''}fp32_{n}")
     tr_, tc_, lz_ = fast["tile"].split("x")
-    kernel = (f"k_elastic_walsh<float,{tr_},{tc_},true,true> (FP32 elastic NBN apply with the CG direction update "
-              f"d = M^-1 r + beta d, the deferred x += alpha d and d.Ad fused), LZ {lz_}" if fused else
-              f"k_elastic_walsh<float,{tr_},{tc_},true,false> (FP32 elastic NBN apply + fused d.Ad), LZ {lz_}")
+    kernel = (f"k_walsh<float,3,{tr_},{tc_},true,true> (FP32 elastic NBN apply with the CG direction update "
+              f"d = z + beta d (z = M^-1 r), the deferred x += alpha d and d.Ad fused), LZ {lz_}" if fused else
+              f"k_walsh<float,3,{tr_},{tc_},true,false> (FP32 elastic NBN apply + fused d.Ad), LZ {lz_}")
     cpu = None
     if not args.skip_cpu and plain is not None:
         log(f"[bench] cpu baseline (oracle)")
