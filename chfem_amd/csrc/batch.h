@@ -7,24 +7,24 @@
 namespace chfem {
 namespace batch {
 
-// Walsh tile of the batched elastic operator (rows x columns per block)
+// Walsh tile of the batched operator (rows x columns per block)
 #define TILE_R 32
 #define TILE_C 8
 
 // one linear system (a load case of a sample), device view
 struct Sys {
-    int ny, nx, nz, LZ;       // grid (elements = nodes) and node layers per elastic tile
+    int ny, nx, nz, LZ;       // grid (elements = nodes) and node layers per tile
     uint32_t N;               // nodes
     size_t off;               // DOF offset in the concatenated vectors
     const uint16_t *keys;     // packed NBN keys of the sample
-    const void *mat32, *mat64;   // elastic: {lambda, mu} per material; thermal: dense 8x8 per material
+    const void *mat32, *mat64;   // elastic: {lambda, mu} per material; thermal: {k, 0} per material
     const void *kinv32, *kinv64; // inverse Jacobi diagonal per key, [65536][nd]
     int tb0, tb1;             // operator tiles [tb0, tb1) of this system
     int vb0, vb1;             // vector chunks [vb0, vb1)
 };
 
 struct Tile {
-    int sys, bx, by, bz;      // elastic: Walsh tile; thermal: bx = chunk of 256 nodes
+    int sys, bx, by, bz;      // Walsh tile of system sys
 };
 
 struct Chunk {

@@ -3,8 +3,7 @@
 // one launch per kernel and iteration for the whole batch.
 //
 // Work is described by tables built on the host: operator tiles
-// (system, tile coordinates) -- Walsh tiles for elastic systems, node ranges
-// for thermal ones -- and vector chunks (system, node range).  Vectors of all
+// (system, Walsh tile coordinates) and vector chunks (system, node range).  Vectors of all
 // systems are concatenated; each system keeps its own FP64 scalars
 // (delta, alpha, stop flags) in a row of Sb, and its block partials sit in a
 // contiguous range of the partials buffer, summed in a fixed order by one
@@ -12,7 +11,6 @@
 // and chunks, so converged systems cost only the launch.
 #include <hip/hip_runtime.h>
 #include "batch.h"
-#include "nbn.h"
 #include "walsh.h"
 
 namespace chfem {
@@ -41,41 +39,26 @@ __device__ __forceinline__ bool selected(const double *Sb, int sys, int mode)
     return mode == OP_ITER ? (flag(Sb, sys, SB_ACTIVE) && !flag(Sb, sys, SB_DONE)) : flag(Sb, sys, SB_REPL);
 }
 
-template <typename T, int TR, int TC>
-__global__ void __launch_bounds__(TR *TC) k_apply_elastic(const Sys *__restrict__ sys, const Tile *__restrict__ tiles,
-                                                           const T *v, T *q, const double *__restrict__ Sb,
-                                                           double *__restrict__ part, int mode)
+// Walsh-basis operator over the tile table (ND = 3 elastic, 1 conduction).
+// Blocks are dispatched to the 8 XCDs round-robin; block b takes table entry
+// start(b % 8) + b / 8, so each XCD walks one contiguous range of the table
+// (a system's tiles are contiguous, rows of tiles fastest) and neighbouring
+// tiles share its L2.
+template <typename T, int ND, int TR, int TC>
+__global__ void __launch_bounds__(TR *TC) k_apply_walsh(const Sys *__restrict__ sys, const Tile *__restrict__ tiles,
+                                                         const T *v, T *q, const double *__restrict__ Sb,
+                                                         double *__restrict__ part, int mode)
 {
-    const Tile t = tiles[blockIdx.x];
+    const unsigned b = blockIdx.x, nb = gridDim.x, per = nb / 8u, rem = nb % 8u, xcd = b % 8u, k = b / 8u;
+    const unsigned ti = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + k;
+    const Tile t = tiles[ti];
     if (!selected<T>(Sb, t.sys, mode)) return;
     const Sys &S = sys[t.sys];
     const T *lamu = (const T *)(sizeof(T) == 4 ? S.mat32 : S.mat64);
-    const double d = walsh_tile<T, 3, TR, TC, true>(v + S.off, q + S.off, S.keys, lamu, 0, S.ny, S.nx, S.nz, S.LZ,
-                                                  t.bx, t.by, t.bz);
+    const double d = walsh_tile<T, ND, TR, TC, true>(v + S.off, q + S.off, S.keys, lamu, 0, S.ny, S.nx, S.nz, S.LZ,
+                                                     t.bx, t.by, t.bz);
     const double s = bsum(d);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(BB) k_apply_thermal(const Sys *__restrict__ sys, const Tile *__restrict__ tiles,
-                                                      const T *v, T *q, const double *__restrict__ Sb,
-                                                      double *__restrict__ part, int mode)
-{
-    const Tile t = tiles[blockIdx.x];
-    if (!selected<T>(Sb, t.sys, mode)) return;
-    const Sys &S = sys[t.sys];
-    const T *K = (const T *)(sizeof(T) == 4 ? S.mat32 : S.mat64);
-    const T *vs = v + S.off;
-    T *qs = q + S.off;
-    double acc = 0;
-    const uint32_t i = (uint32_t)t.bx * BB + threadIdx.x;  // node chunk of BB nodes
-    if (i < S.N) {
-        const T res = thermal_node<T>(vs, S.keys, K, S.ny, S.nx, S.nz, i);
-        qs[i] = res;
-        acc = (double)vs[i] * (double)res;
-    }
-    const double s = bsum(acc);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
+    if (threadIdx.x == 0) part[ti] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -280,9 +263,9 @@ void apply(const Batch &b, const T *v, T *q, int mode, hipStream_t s)
 {
     if (b.ntiles == 0) return;
     if (b.nd == 3)
-        k_apply_elastic<T, TILE_R, TILE_C><<<b.ntiles, TILE_R * TILE_C, 0, s>>>(b.sys, b.tiles, v, q, b.Sb, b.part, mode);
+        k_apply_walsh<T, 3, TILE_R, TILE_C><<<b.ntiles, TILE_R * TILE_C, 0, s>>>(b.sys, b.tiles, v, q, b.Sb, b.part, mode);
     else
-        k_apply_thermal<T><<<b.ntiles, BB, 0, s>>>(b.sys, b.tiles, v, q, b.Sb, b.part, mode);
+        k_apply_walsh<T, 1, TILE_R, TILE_C><<<b.ntiles, TILE_R * TILE_C, 0, s>>>(b.sys, b.tiles, v, q, b.Sb, b.part, mode);
 }
 template void apply<float>(const Batch &, const float *, float *, int, hipStream_t);
 template void apply<double>(const Batch &, const double *, double *, int, hipStream_t);

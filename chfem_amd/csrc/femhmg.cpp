@@ -1269,10 +1269,15 @@ bool upload_sample(BatchSample &b)
         b.mat64 = d64;
         b.mat32 = d32;
     } else {
-        std::vector<float> Kf(K.begin(), K.end());
+        // conduction: {k, 0} per material for the Walsh-basis operator
+        double kk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        float kf[8];
+        for (unsigned t = 0; t < m.nmat; t++)
+            if (!voxel::conduction_param(K.data() + 64 * t, kk[2 * t])) return false;
+        for (int i = 0; i < 8; i++) kf[i] = (float)kk[i];
         double *d64 = nullptr;
         float *d32 = nullptr;
-        if (!bupload(d64, K.data(), K.size()) || !bupload(d32, Kf.data(), Kf.size())) return false;
+        if (!bupload(d64, kk, 8) || !bupload(d32, kf, 8)) return false;
         b.mat64 = d64;
         b.mat32 = d32;
     }
@@ -1374,14 +1379,12 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
             S.kinv32 = B->samples[si].kinv32; S.kinv64 = B->samples[si].kinv64;
             const int sid = (int)sys.size();
             S.tb0 = (int)tiles.size();
-            if (nd == 3) {
+            {
                 const int gx = (S.ny + TILE_R - 2) / (TILE_R - 1), gy = (S.nx + TILE_C - 2) / (TILE_C - 1);
                 const int gz = (S.nz + S.LZ - 1) / S.LZ;
                 for (int z = 0; z < gz; z++)
                     for (int y = 0; y < gy; y++)
                         for (int x = 0; x < gx; x++) tiles.push_back(Tile{sid, x, y, z});
-            } else {
-                for (uint32_t c = 0; c * 256u < S.N; c++) tiles.push_back(Tile{sid, (int)c, 0, 0});
             }
             S.tb1 = (int)tiles.size();
             S.vb0 = (int)chunks.size();
