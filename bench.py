@@ -217,7 +217,7 @@ def main():
     per_node64 = 1.0 + (6 if fused else 2) * 24.0
     bytes_per_launch = per_node32 * N
     achieved = bytes_per_launch / (op_avg_ms * 1e-3) / 1e9
-    traffic = read_traffic(f"elastic_walsh_{'dir_' if fused else ''}fp32_{n}")
+    traffic = read_traffic(f"elastic_walsh_{'dir_' if fused else ''}fp32_{n}_{fast['tile']}")
     tr_, tc_, lz_ = fast["tile"].split("x")
     kernel = (f"k_walsh<float,3,{tr_},{tc_},true,true> (FP32 elastic NBN apply with the CG direction update "
               f"d = z + beta d (z = M^-1 r), the deferred x += alpha d and d.Ad fused), LZ {lz_}" if fused else

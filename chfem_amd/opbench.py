@@ -2,6 +2,11 @@
 plain copy kernel of known traffic for scaling hardware byte counters.
 
     python -m chfem_amd.opbench --physics elastic --size 256 --precision mixed --reps 20
+
+Thermal and elastic use bench.py's sphere image (seed 3, inclusion fraction
+0.3); fluid uses solid spheres until the porosity is 0.35 (seed 4, bumped
+until the pore space percolates), the Stokes model of the permeability
+workload.
 """
 import argparse
 import json
@@ -14,7 +19,7 @@ from . import io as cio
 from . import synthetic
 from .lib import lib, ptr
 
-PHYS = {"thermal": 0, "elastic": 1}
+PHYS = {"thermal": 0, "elastic": 1, "fluid": 2}
 BYTES = {("thermal", 0): 17.0, ("thermal", 1): 9.0, ("elastic", 0): 49.0, ("elastic", 1): 25.0}
 # CG-loop form with the direction update and the deferred iterate update
 # inside the operator: image, r, the previous direction and x read, the
@@ -37,8 +42,13 @@ def main():
     ap.add_argument("--both", action="store_true", help="time the --dot form, then the --dir form")
     a = ap.parse_args()
     n = a.size
-    img = synthetic.spheres((n, n, n), rmin=6, rmax=12, fraction=0.3, seed=3)
-    props = {0: 1.0, 255: 10.0} if a.physics == "thermal" else {0: (1.0, 0.3), 255: (10.0, 0.2)}
+    if a.physics == "fluid":
+        # solid spheres (radius 6-12) until porosity <= 0.35, pore space percolating
+        img, _ = synthetic.porous_spheres((n, n, n), rmin=6, rmax=12, porosity=0.35, seed=4)
+        props = None
+    else:
+        img = synthetic.spheres((n, n, n), rmin=6, rmax=12, fraction=0.3, seed=3)
+        props = {0: 1.0, 255: 10.0} if a.physics == "thermal" else {0: (1.0, 0.3), 255: (10.0, 0.2)}
     d = tempfile.mkdtemp(prefix="chfem_op_")
     f = os.path.join(d, "img.raw")
     cio.export_for_chfem(f, img, PHYS[a.physics], props, export_raw=False)
@@ -50,14 +60,25 @@ def main():
     assert L.hmgPrepare() == 1
     N = n ** 3
     out = {"physics": a.physics, "size": n, "precision": a.precision, "strategy": a.strategy}
+    if a.physics == "fluid":
+        # Stokes: pore nodes read a DOF id (4 B) and 4 values, write 4; border
+        # nodes read their DOF id and fluid key (5 B) and 1 value, write 1
+        import ctypes as C
+        ne, nd_, nv, nb = C.c_uint(), C.c_uint(), C.c_uint(), C.c_uint()
+        L.hmgGetSizes(C.byref(ne), C.byref(nd_), C.byref(nv), C.byref(nb))
+        w = 4 if prec == 1 else 8
+        fluid_bytes = nv.value * (4 + 8 * w) + nb.value * (5 + 2 * w)
+        out.update({"pore_nodes": nv.value, "border_nodes": nb.value, "dofs": nd_.value})
     forms = ["dot", "dir"] if a.both else ["dir" if a.dir else ("dot" if a.dot else "plain")]
     for form in forms:
         ms = L.cudapcgOperatorBench(prec, a.reps, {"plain": 0, "dot": 1, "dir": 2}[form])
         if ms < 0:
             raise SystemExit("operator bench failed (the direction form needs the Walsh-basis NBN operator)")
-        per_node = BYTES_DIR[(a.physics, prec)] if form == "dir" else BYTES[(a.physics, prec)]
-        out[form] = {"avg_launch_ms": ms, "algorithmic_bytes": per_node * N,
-                     "GBps": per_node * N / (ms * 1e-3) / 1e9}
+        if a.physics == "fluid":
+            nbytes = fluid_bytes
+        else:
+            nbytes = (BYTES_DIR[(a.physics, prec)] if form == "dir" else BYTES[(a.physics, prec)]) * N
+        out[form] = {"avg_launch_ms": ms, "algorithmic_bytes": nbytes, "GBps": nbytes / (ms * 1e-3) / 1e9}
     L.hmgRelease()
     L.hmgEnd()
     if a.copy:
