@@ -21,7 +21,8 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
-SOURCES = ["kernels.hip", "walsh.hip", "ebe_mfma.hip", "batch.hip", "guess.hip", "cudapcg.cpp", "femhmg.cpp", "guess.cpp"]
+SOURCES = ["kernels.hip", "walsh.hip", "stokes.hip", "ebe_mfma.hip", "batch.hip", "guess.hip", "cudapcg.cpp",
+           "femhmg.cpp", "guess.cpp"]
 
 
 def _run(cmd, verbose):

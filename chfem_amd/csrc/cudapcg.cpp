@@ -67,6 +67,9 @@ struct Solver {
     double *lamu64 = nullptr;
     float *lamu32 = nullptr;
     bool walsh_ok = false;
+    // per-node fluid keys for the Walsh-basis Stokes operator (N bytes)
+    uint8_t *fkeys = nullptr;
+    size_t fkeys_n = 0;
     // operator-launch timing
     bool timing = false;
     hipEvent_t opA = nullptr, opB = nullptr;
@@ -280,6 +283,7 @@ static int make_ops()
     o.dof2node = m->DOF2periodic_map;
     o.node2dof = m->periodic2DOF_map;
     o.bkeys = m->border_pore_map;
+    o.fkeys = nullptr;
     o.lamu = nullptr;
     o.tile_tr = 32;
     o.tile_tc = 8;
@@ -289,6 +293,24 @@ static int make_ops()
     if (fixed_tile) sscanf(tile_env, "%dx%dx%d", &o.tile_tr, &o.tile_tc, &o.tile_lz);
     const char *opsel = getenv("CHFEM_OPERATOR");
     bool use_walsh = g->walsh_ok && !(opsel && !strcmp(opsel, "generic"));
+    // Stokes: Walsh-basis operator over tiles, with per-node fluid keys built
+    // from the DOF maps (the generic node kernel for CHFEM_OPERATOR=generic,
+    // and for pore spaces without a fully fluid node or under 3 DOFs)
+    if (o.physics == PHY_FLUID && !(opsel && !strcmp(opsel, "generic")) && o.nvel > 0 && o.nvel + o.nborder >= 3 &&
+        o.node2dof && o.dof2node && (o.bkeys || o.nborder == 0)) {
+        if (g->fkeys_n != (size_t)o.N) {
+            dfree(g->fkeys);
+            if (hipMalloc(&g->fkeys, o.N) != hipSuccess) {
+                g->fkeys = nullptr;
+                g->fkeys_n = 0;
+                printf("ERROR: could not allocate the fluid node keys.\n");
+                return 0;
+            }
+            g->fkeys_n = o.N;
+        }
+        launch_fluid_node_keys(o, g->fkeys, g->stream);
+        o.fkeys = g->fkeys;
+    }
     // key-Jacobi vector kernels: NBN thermal/elastic with the Jacobi
     // preconditioner on (CHFEM_VECTOR=stored keeps the stored-dinv kernels)
     // scalar density field: element-by-element operator only (the reference
@@ -500,9 +522,14 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
 }
 
 // Jacobi-MINRES in precision T (pminres.cu:28-247).  r holds b on entry.
+// stagnation > 0 (the FP32 inner solves of mixed precision): the loop also
+// ends once the residual measure has not fallen below 0.9 of its best value
+// for that many iterations plus a tenth of those done, i.e. when the solve
+// has reached what its precision can attain; the FP64 refinement then
+// restarts it from the FP64 residual.
 template <typename T>
 static int run_minres(const OpDesc &op, T *x, T *r, T *d, T *q, T *s, const T *dinv, bool x0set, double tol,
-                      unsigned maxit, unsigned &it, double &residual)
+                      unsigned maxit, unsigned &it, double &residual, unsigned stagnation = 0)
 {
     const size_t n = op.ndof;
     hipStream_t st = g->stream;
@@ -543,6 +570,8 @@ static int run_minres(const OpDesc &op, T *x, T *r, T *d, T *q, T *s, const T *d
     if (!sync_loop()) return -1;
     double stop = g->Sh[S_STOP];
     bool cont = above(stop, res0, tol);
+    double best = fabs(stop);
+    unsigned best_it = it;
     while (cont && it < maxit) {
         it++;
         launch_minres_dir<T>(d, q, r, s, dinv, g->S, 0, &ra, n, st);
@@ -551,6 +580,14 @@ static int run_minres(const OpDesc &op, T *x, T *r, T *d, T *q, T *s, const T *d
         if (!sync_loop()) return -1;
         stop = g->Sh[S_STOP];
         cont = above(stop, res0, tol);
+        if (stagnation) {
+            if (fabs(stop) < 0.9 * best) {
+                best = fabs(stop);
+                best_it = it;
+            } else if (it - best_it > stagnation + it / 10) {
+                break;
+            }
+        }
     }
     residual = sqrt(fabs(stop) / fabs(res0));
     return residual <= tol;
@@ -716,6 +753,15 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
     return converged ? 1 : 0;
 }
 
+// iterations without a 10 % drop of the FP32 inner MINRES residual after
+// which a sweep ends (CHFEM_MINRES_STAGNATION, default 100)
+static unsigned minres_stagnation()
+{
+    const char *e = getenv("CHFEM_MINRES_STAGNATION");
+    const int v = e ? atoi(e) : 100;
+    return v < 1 ? 1u : (unsigned)v;
+}
+
 // FP32 inner Krylov + FP64 outer iterative refinement.
 static int run_mixed(bool x0set, unsigned &it_total, unsigned &sweeps, double &residual)
 {
@@ -753,7 +799,7 @@ static int run_mixed(bool x0set, unsigned &it_total, unsigned &sweeps, double &r
         int rc;
         if (is_minres())
             rc = run_minres<float>(o32, g->x32, g->r32, g->d32, g->q32, g->s32, g->dinv32, false, tau,
-                                   g->max_iterations - it_total, it, res_in);
+                                   g->max_iterations - it_total, it, res_in, minres_stagnation());
         else
             rc = run_pcg<float>(o32, g->x32, g->r32, g->d32, g->q32, g->dinv32, false, tau,
                                 g->max_iterations - it_total, it, res_in, nullptr, g->s32);
@@ -934,6 +980,8 @@ cudapcgFlag_t cudapcgEnd(void)
     dfree(g->dm.K32);
     dfree(g->lamu64);
     dfree(g->lamu32);
+    dfree(g->fkeys);
+    g->fkeys_n = 0;
     dfree(g->kinv64);
     dfree(g->kinv32);
     if (g->model && g->model->freeAllowed_flag) {
@@ -1291,6 +1339,29 @@ cudapcgFlag_t cudapcgApplyOperator(const cudapcgVar_t *v, cudapcgVar_t *q)
     CHECK_HIP(hipMemcpyAsync(q, g->q64, n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
     CHECK_HIP(hipStreamSynchronize(g->stream));
     return CUDAPCG_TRUE;
+}
+
+cudapcgFlag_t cudapcgApplyOperatorF32(const float *v, float *q)
+{
+    if (!g || !g->model || !v || !q) return CUDAPCG_FALSE;
+    if (!g->arrays && alloc_solver_arrays() != CUDAPCG_TRUE) return CUDAPCG_FALSE;
+    make_ops();
+    const size_t n = nvars();
+    float *dv = nullptr, *dq = nullptr;
+    if (hipMalloc(&dv, n * sizeof(float)) != hipSuccess || hipMalloc(&dq, n * sizeof(float)) != hipSuccess) {
+        dfree(dv);
+        return CUDAPCG_FALSE;
+    }
+    bool ok = hipMemcpyAsync(dv, v, n * sizeof(float), hipMemcpyHostToDevice, g->stream) == hipSuccess;
+    if (ok) {
+        launch_apply<float>(g->dm.op32, dv, dq, nullptr, g->stream);
+        ok = hipGetLastError() == hipSuccess &&
+             hipMemcpyAsync(q, dq, n * sizeof(float), hipMemcpyDeviceToHost, g->stream) == hipSuccess &&
+             hipStreamSynchronize(g->stream) == hipSuccess;
+    }
+    dfree(dv);
+    dfree(dq);
+    return ok ? CUDAPCG_TRUE : CUDAPCG_FALSE;
 }
 
 cudapcgFlag_t cudapcgApplyPreconditioner(const cudapcgVar_t *v, cudapcgVar_t *z)

@@ -64,6 +64,9 @@ struct OpDesc {
     const uint32_t *dof2node;  // DOF id -> periodic node
     const uint32_t *node2dof;  // periodic node -> DOF id
     const uint8_t *bkeys;      // border node fluid keys
+    // per-node fluid keys (255 pore, 1..254 border, 0 solid); non-null
+    // selects the Walsh-basis Stokes operator (stokes.hip)
+    const uint8_t *fkeys;
     // isotropic elastic materials: {lambda, mu} per material, or conduction:
     // {k, 0} per material (precision of K); non-null selects the Walsh-basis
     // operator (walsh.hip)
@@ -138,6 +141,10 @@ template <typename T> void launch_kj_sub_precond(const OpDesc &op, T *r, const T
                                                  hipStream_t s);
 template <typename T> void launch_kj_to_z(const OpDesc &op, T *r, const Reduce *red, hipStream_t s);
 template <typename T> void launch_kj_zupdate(const OpDesc &op, T *z, const T *q, const Reduce *red, hipStream_t s);
+// Walsh-basis Stokes operator over layer-march tiles (stokes.hip; needs
+// op.fkeys), and the per-node fluid keys it reads, built from the DOF maps
+template <typename T> void launch_stokes_walsh(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s);
+void launch_fluid_node_keys(const OpDesc &op, uint8_t *out, hipStream_t s);
 // dot reduction of two vectors
 template <typename T> void launch_dot(const T *a, const T *b, size_t n, const Reduce *red, hipStream_t s);
 

@@ -1106,6 +1106,10 @@ void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStre
 {
     const Reduce rr = red ? *red : noreduce();
     const T *K = (const T *)op.K;
+    if (op.physics == PHY_FLUID && op.fkeys) {
+        launch_stokes_walsh<T>(op, v, q, red, s);
+        return;
+    }
     if (op.physics == PHY_FLUID) {
         const uint32_t nt = op.nvel + op.nborder;
         if (red) LAUNCH_RED(1, (k_fluid<T, true>), grid_for(nt), s, rr, v, q, op.dof2node, op.node2dof, op.bkeys, K, op.ny, op.nx, op.nz, op.nvel, op.nborder, rr);

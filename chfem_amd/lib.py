@@ -94,6 +94,7 @@ SIGNATURES = [
     ("cudapcgGetSolveStats", u8, [C.POINTER(SolveStats)]),
     ("hmgGetSolveStats", u8, [C.POINTER(SolveStats), C.POINTER(uint)]),
     ("cudapcgApplyOperator", u8, [vp, vp]),
+    ("cudapcgApplyOperatorF32", u8, [vp, vp]),
     ("cudapcgApplyPreconditioner", u8, [vp, vp]),
     ("hmgAssembleRHS", u8, [uint, vp]),
     ("hmgUpdateC", u8, [uint, vp, vp]),

@@ -58,6 +58,9 @@ cudapcgFlag_t hmgGetSolveStats(chfemSolveStats_t *stats, unsigned int *count);
 /* Test hooks on the current solver model: q = A v and z = M^-1 v on host
  * arrays of nvars entries, through the same device kernels the solver uses. */
 cudapcgFlag_t cudapcgApplyOperator(const cudapcgVar_t *v, cudapcgVar_t *q);
+/* q = A v with the FP32 operator the mixed-precision solver uses (float host
+ * arrays of nvars entries). */
+cudapcgFlag_t cudapcgApplyOperatorF32(const float *v, float *q);
 cudapcgFlag_t cudapcgApplyPreconditioner(const cudapcgVar_t *v, cudapcgVar_t *z);
 
 /* Right-hand side and effective-tensor column computed on the device for the

@@ -78,6 +78,11 @@ class GpuSolver:
         assert self.L.cudapcgApplyOperator(self.ptr(np.ascontiguousarray(v, dtype=np.float64)), self.ptr(q)) == 1
         return q
 
+    def apply32(self, v):
+        q = np.zeros(self.m.ndof, dtype=np.float32)
+        assert self.L.cudapcgApplyOperatorF32(self.ptr(np.ascontiguousarray(v, dtype=np.float32)), self.ptr(q)) == 1
+        return q
+
     def precond(self, v):
         z = np.zeros(self.m.ndof)
         assert self.L.cudapcgApplyPreconditioner(self.ptr(np.ascontiguousarray(v, dtype=np.float64)),

@@ -61,7 +61,12 @@ def test_operator_apply(which, strategy, ragged_thermal, ragged_elastic):
         s.close()
 
 
-def test_operator_fluid(porous):
+@pytest.mark.parametrize("kernel", ["walsh", "generic"])
+def test_operator_fluid(porous, kernel, monkeypatch):
+    """Stokes operator (Walsh-basis tiles by default, the node-by-node
+    kernel with CHFEM_OPERATOR=generic) and Jacobi against the oracle."""
+    if kernel == "generic":
+        monkeypatch.setenv("CHFEM_OPERATOR", "generic")
     m = porous
     assert m.nvel > 0 and m.nborder > 0
     s = H.GpuSolver(m)
@@ -69,6 +74,62 @@ def test_operator_fluid(porous):
         v = np.random.default_rng(6).standard_normal(m.ndof)
         assert rel(s.apply(v), m.apply(v)) < 1e-12
         assert rel(s.precond(v), m.jacobi_inv() * v) < 1e-14
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("kernel", ["default", "generic"])
+@pytest.mark.parametrize("which", ["thermal", "elastic", "fluid"])
+def test_operator_fp32(which, kernel, ragged_thermal, ragged_elastic, porous, monkeypatch):
+    """The FP32 operators of the mixed-precision solver against the oracle
+    (FP64) on an FP32-representable input: every entry within 1e-5 of the
+    largest (FP32 rounding of sums over 8 elements)."""
+    if kernel == "generic":
+        monkeypatch.setenv("CHFEM_OPERATOR", "generic")
+    m = {"thermal": ragged_thermal, "elastic": ragged_elastic, "fluid": porous}[which]
+    s = H.GpuSolver(m)
+    try:
+        v = np.random.default_rng(9).standard_normal(m.ndof).astype(np.float32)
+        q32 = s.apply32(v)
+        ref = m.apply(v.astype(np.float64))
+        assert np.abs(q32 - ref).max() < 1e-5 * np.abs(ref).max()
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("kernel", ["walsh", "generic"])
+def test_fluid_mixed_precision(porous, kernel, monkeypatch):
+    """Permeability load case with FP32 inner MINRES and FP64 refinement,
+    against the oracle's FP64 solve."""
+    if kernel == "generic":
+        monkeypatch.setenv("CHFEM_OPERATOR", "generic")
+    m = porous
+    s = H.GpuSolver(m, solver=1, precision=1, tol=1e-8)
+    try:
+        x, ok, st = s.solve(m.rhs(1))
+        xr, okr, _, _ = m.solve(1, solver=1, tol=1e-10)
+        assert ok and okr, (st.iterations, st.sweeps, st.residual)
+        assert rel(m.update_c(1, x), m.update_c(1, xr)) < 1e-6
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("shape", [(40, 70, 9), (17, 33, 65), (3, 5, 4)])
+def test_operator_fluid_tiles(shape, monkeypatch):
+    """Walsh-basis Stokes operator (FP64) on ragged, multi-block and tiny
+    grids with random pore spaces (several tiles per direction, layer
+    chunks), against the oracle."""
+    monkeypatch.setenv("CHFEM_WALSH_TILE", "32x8x7")
+    rng = np.random.default_rng(sum(shape))
+    img = np.where(rng.random(shape) < 0.35, 255, 0).astype(np.uint8)
+    img[0, 0, 0] = 0
+    m = H.model(H.FLUID, img, [0.0], colors=(0, 255))
+    if m.nvel == 0 or m.nvel + m.nborder < 3:
+        pytest.skip("pore space without a fully fluid node")
+    s = H.GpuSolver(m)
+    try:
+        v = rng.standard_normal(m.ndof)
+        assert rel(s.apply(v), m.apply(v)) < 1e-12
     finally:
         s.close()
 
