@@ -84,6 +84,7 @@ def run_method(L, precision, tol, steps, warmup, label, net=None):
         sweeps = [st[i].sweeps for i in range(cnt.value)]
         tile = np.zeros(3, dtype=np.int32)
         L.cudapcgGetOperatorTile(precision, tile.ctypes.data_as(C.c_void_p))
+        form = int(L.cudapcgGetLoopForm(precision))
         x0res = [st[i].x0_residual for i in range(cnt.value)]
         x0used = [st[i].x0_used for i in range(cnt.value)]
         op_ms = C.c_double(0)
@@ -94,7 +95,8 @@ def run_method(L, precision, tol, steps, warmup, label, net=None):
         L.hmgSetInitialGuessNetwork(None)
     return {"seconds": float(np.mean(times)) if times else None, "times": times, "C": Cm.reshape(6, 6).copy(),
             "iterations": its, "sweeps": sweeps, "op_ms_total": op_ms.value, "op_launches": op_n.value,
-            "x0_residual": x0res, "x0_used": x0used, "tile": "x".join(str(int(t)) for t in tile)}
+            "x0_residual": x0res, "x0_used": x0used, "tile": "x".join(str(int(t)) for t in tile),
+            "form": form}
 
 
 def train_guess(n_samples, size, epochs, seed):
@@ -209,19 +211,27 @@ def main():
 
     N = n ** 3
     op_avg_ms = fast["op_ms_total"] / max(fast["op_launches"], 1)
-    # Algorithmic bytes per node of the operator launch inside the CG loop.
-    # Fused form (default): 1 B image + z, d_old, x read + d, q, x written
-    # (6 x 3 values); plain apply: 1 B image + v read + q written.
-    fused = os.environ.get("CHFEM_FUSE_DIR", "1") != "0"
-    per_node32 = 1.0 + (6 if fused else 2) * 12.0
-    per_node64 = 1.0 + (6 if fused else 2) * 24.0
+    # Algorithmic bytes per node of the operator launch inside the CG loop,
+    # by the loop form the library used (1 B image + 3-vectors of 12 B):
+    # form 2 (single-kernel iteration): z_old, q_old, d_old, x read + z, d, q,
+    # x written; form 1: z, d_old, x read + d, q, x written; form 0: v read +
+    # q written.
+    form = fast["form"]
+    nvec = {2: 8, 1: 6}.get(form, 2)
+    per_node32 = 1.0 + nvec * 12.0
+    plain_form = plain.get("form", form) if plain is not None else form
+    per_node64 = 1.0 + {2: 8, 1: 6}.get(plain_form, 2) * 24.0
     bytes_per_launch = per_node32 * N
     achieved = bytes_per_launch / (op_avg_ms * 1e-3) / 1e9
-    traffic = read_traffic(f"elastic_walsh_{'dir_' if fused else ''}fp32_{n}_{fast['tile']}")
+    key = {2: "cg1_", 1: "dir_"}.get(form, "")
+    traffic = read_traffic(f"elastic_walsh_{key}fp32_{n}_{fast['tile']}")
     tr_, tc_, lz_ = fast["tile"].split("x")
-    kernel = (f"k_walsh<float,3,{tr_},{tc_},true,true> (FP32 elastic NBN apply with the CG direction update "
-              f"d = z + beta d (z = M^-1 r), the deferred x += alpha d and d.Ad fused), LZ {lz_}" if fused else
-              f"k_walsh<float,3,{tr_},{tc_},true,false> (FP32 elastic NBN apply + fused d.Ad), LZ {lz_}")
+    kernel = {
+        2: f"k_walsh<float,3,{tr_},{tc_},true,2> (FP32 elastic NBN single-kernel CG iteration: z -= alpha M^-1 q, "
+           f"d = z + beta d, q = A d, x += alpha d, d.q z.q q.M^-1q z.Dz), LZ {lz_}",
+        1: f"k_walsh<float,3,{tr_},{tc_},true,1> (FP32 elastic NBN apply with the CG direction update "
+           f"d = z + beta d (z = M^-1 r), the deferred x += alpha d and d.Ad fused), LZ {lz_}",
+    }.get(form, f"k_walsh<float,3,{tr_},{tc_},true,0> (FP32 elastic NBN apply + fused d.Ad), LZ {lz_}")
     cpu = None
     if not args.skip_cpu and plain is not None:
         log(f"[bench] cpu baseline (oracle)")
@@ -247,7 +257,8 @@ def main():
         "plain_fp64": None if plain is None else {
             "seconds": plain["seconds"], "err_vs_fp64": err(plain["C"]), "iterations": plain["iterations"],
             "op_avg_ms": plain["op_ms_total"] / max(plain["op_launches"], 1),
-            "op_GBps": per_node64 * N / (plain["op_ms_total"] / max(plain["op_launches"], 1) * 1e-3) / 1e9},
+            "op_GBps": per_node64 * N / (plain["op_ms_total"] / max(plain["op_launches"], 1) * 1e-3) / 1e9,
+            "loop_form": plain_form, "tile": plain["tile"]},
         "mixed": None if mixed is None else {
             "seconds": mixed["seconds"], "err_vs_fp64": err(mixed["C"]), "iterations": mixed["iterations"],
             "sweeps": mixed["sweeps"]},
@@ -257,7 +268,7 @@ def main():
         "speedup_vs_plain": None if plain is None else plain["seconds"] / fast["seconds"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kernel, "bytes_per_node": per_node32,
+                     "kernel": kernel, "loop_form": form, "bytes_per_node": per_node32,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": op_avg_ms,
                      "launches": fast["op_launches"]},
         "cpu_baseline": cpu,

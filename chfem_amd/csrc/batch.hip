@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(TR *TC) k_apply_walsh(const Sys *__restrict__ 
     const Sys &S = sys[t.sys];
     const T *lamu = (const T *)(sizeof(T) == 4 ? S.mat32 : S.mat64);
     const double d = walsh_tile<T, ND, TR, TC, true>(v + S.off, q + S.off, S.keys, lamu, 0, S.ny, S.nx, S.nz, S.LZ,
-                                                     t.bx, t.by, t.bz);
+                                                     t.bx, t.by, t.bz).dq;
     const double s = bsum(d);
     if (threadIdx.x == 0) part[ti] = s;
 }

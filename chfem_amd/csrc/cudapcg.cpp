@@ -51,6 +51,9 @@ struct Solver {
     DeviceModel dm;
     double *r64 = nullptr, *x64 = nullptr, *d64 = nullptr, *q64 = nullptr, *s64 = nullptr, *dinv64 = nullptr;
     float *x32 = nullptr, *r32 = nullptr, *d32 = nullptr, *q32 = nullptr, *s32 = nullptr, *dinv32 = nullptr;
+    // second z and q buffers of the single-kernel CG loop
+    double *z2_64 = nullptr, *q2_64 = nullptr;
+    float *z2_32 = nullptr, *q2_32 = nullptr;
     bool arrays = false, user_arrays = false, x0_set = false, dinv_valid = false;
     double *partials = nullptr, *S = nullptr, *Sh = nullptr;
     size_t partials_n = 0;
@@ -63,6 +66,8 @@ struct Solver {
     // operator tiles chosen by the autotuner (rows, columns, layers per block)
     bool autotune = false, tuned = false;
     int tile64[3] = {32, 8, 16}, tile32[3] = {32, 8, 16};
+    // CG loop form per precision (see fuse_mode_env), set by the autotuner
+    int mode64 = 1, mode32 = 1;
     // isotropic elastic materials for the Walsh-basis operator
     double *lamu64 = nullptr;
     float *lamu32 = nullptr;
@@ -173,12 +178,32 @@ static void timed_apply(const OpDesc &op, const T *v, T *q, const Reduce *red)
     }
 }
 
-// fused direction update in the CG loops (CHFEM_FUSE_DIR=0 keeps the separate
-// direction kernel)
-static bool fuse_dir_enabled()
+// form of the CG loops on the Walsh-basis operators (CHFEM_FUSE_DIR): 0
+// separate operator, direction and update kernels; 1 direction and iterate
+// updates inside the operator, z update kernel; 2 single-kernel iteration,
+// the z update inside the operator too.  Unset or "auto": the autotuner
+// times forms 1 and 2 with every tile and keeps the faster per precision
+// (form 1 when the tiles are fixed).
+static int fuse_mode_env()
 {
     const char *e = getenv("CHFEM_FUSE_DIR");
-    return e ? atoi(e) != 0 : true;
+    if (!e || !strcmp(e, "auto")) return -1;
+    const int m = atoi(e);
+    return m < 0 ? 0 : (m > 2 ? 2 : m);
+}
+template <typename T> static int loop_mode();
+static bool fuse_dir_enabled() { return fuse_mode_env() != 0; }
+
+template <typename T>
+static void timed_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *dold, T *znew, T *dnew, T *qnew,
+                            T *x, int pend, const Reduce *red, bool timed)
+{
+    if (timed && g->timing) hipEventRecord(g->opA, g->stream);
+    launch_apply_cg1<T>(op, zold, qold, dold, znew, dnew, qnew, x, pend, red, g->stream);
+    if (timed && g->timing) {
+        hipEventRecord(g->opB, g->stream);
+        op_pending = true;
+    }
 }
 
 template <typename T>
@@ -226,6 +251,7 @@ static cudapcgFlag_t free_solver_arrays()
     if (!g->x0_set) dfree(g->x64);
     dfree(g->d64); dfree(g->q64); dfree(g->s64); dfree(g->dinv64);
     dfree(g->x32); dfree(g->r32); dfree(g->d32); dfree(g->q32); dfree(g->s32); dfree(g->dinv32);
+    dfree(g->z2_64); dfree(g->q2_64); dfree(g->z2_32); dfree(g->q2_32);
     dfree(g->partials); dfree(g->S);
     if (g->Sh) hipHostFree(g->Sh);
     g->Sh = nullptr;
@@ -245,6 +271,8 @@ static cudapcgFlag_t alloc_solver_arrays()
     if (!g->q64) CHECK_HIP(hipMalloc(&g->q64, b8));
     // s: MINRES's A M^-1 r, or CG's second direction buffer (fused direction update)
     if (!g->s64) CHECK_HIP(hipMalloc(&g->s64, b8));
+    if (!g->z2_64) CHECK_HIP(hipMalloc(&g->z2_64, b8));
+    if (!g->q2_64) CHECK_HIP(hipMalloc(&g->q2_64, b8));
     if (!g->dinv64) CHECK_HIP(hipMalloc(&g->dinv64, b8));
     if (g->precision == CHFEM_PREC_MIXED) {
         if (!g->x32) CHECK_HIP(hipMalloc(&g->x32, b4));
@@ -252,6 +280,8 @@ static cudapcgFlag_t alloc_solver_arrays()
         if (!g->d32) CHECK_HIP(hipMalloc(&g->d32, b4));
         if (!g->q32) CHECK_HIP(hipMalloc(&g->q32, b4));
         if (!g->s32) CHECK_HIP(hipMalloc(&g->s32, b4));
+        if (!g->z2_32) CHECK_HIP(hipMalloc(&g->z2_32, b4));
+        if (!g->q2_32) CHECK_HIP(hipMalloc(&g->q2_32, b4));
         if (!g->dinv32) CHECK_HIP(hipMalloc(&g->dinv32, b4));
     }
     const size_t pn = (size_t)MAX_PARTIAL_BLOCKS * 8;
@@ -361,48 +391,77 @@ static void autotune_tiles()
     static const int cand[][3] = {{32, 8, 16}, {32, 8, 32}, {16, 16, 16}, {32, 16, 16}, {32, 16, 32}, {64, 4, 16}};
     const int nc = (int)(sizeof(cand) / sizeof(cand[0]));
     const size_t n = nvars();
-    // the CG loops' form of the operator: with the fused direction update
-    // (reads r and d_old, writes d), else the plain apply
-    const bool fused = fuse_dir_enabled() && !is_minres() && g->s64;
+    // the CG loops' kernels per iteration: form 1 = operator with the
+    // direction update + z update kernel, form 2 = single-kernel iteration,
+    // form 0 (MINRES, CHFEM_FUSE_DIR=0) = the plain apply
+    const int env = fuse_mode_env();
+    const int m0 = is_minres() ? 0 : (env >= 0 ? env : 1), m1 = is_minres() ? 0 : (env >= 0 ? env : 2);
     Reduce rt = make_reduce(FIN_STORE, S_TMP0);
-    auto tune = [&](OpDesc &op, auto *d, auto *q, auto *r, auto *d2, int *best) {
+    g->Sh[S_ALPHA] = 0.0;  // the deferred updates of the timed launches change nothing
+    if (hipMemcpyAsync(g->S + S_ALPHA, g->Sh + S_ALPHA, sizeof(double), hipMemcpyHostToDevice, g->stream) != hipSuccess)
+        return;
+    auto tune = [&](OpDesc &op, auto *d, auto *q, auto *d2, int *best, int &best_mode) {
         using T = typename std::remove_pointer<decltype(d)>::type;
-        const bool fz = fused && d2 && apply_dir_ok(op);
-        // x: the deferred iterate update is off (xpend 0), so x is
-        // rewritten unchanged (it may hold an initial guess)
-        T *xs = std::is_same<T, double>::value ? (T *)g->x64 : (T *)g->x32;
-        auto run = [&]() {
-            if (fz) launch_apply_dir<T>(op, r, d, d2, q, xs, 0, &rt, g->stream);
-            else launch_apply<T>(op, d, q, nullptr, g->stream);
+        // x: the deferred iterate updates run with alpha = 0, so x is
+        // rewritten unchanged (it may hold an initial guess); z2, q2 scratch
+        const bool dbl = std::is_same<T, double>::value;
+        T *xs = dbl ? (T *)g->x64 : (T *)g->x32;
+        T *z2 = dbl ? (T *)g->z2_64 : (T *)g->z2_32;
+        T *q2 = dbl ? (T *)g->q2_64 : (T *)g->q2_32;
+        const bool fz = d2 && z2 && q2 && apply_dir_ok(op);
+        launch_fill<T>(z2, 1.0, n, g->stream);
+        launch_fill<T>(q2, 0.5, n, g->stream);
+        auto run = [&](int mode) {
+            if (fz && mode == 2) {
+                // with the deferred updates pending (alpha = 0): the launch
+                // streams q_old as in the loop; z is rewritten in place with
+                // the values it holds
+                launch_apply_cg1<T>(op, z2, q2, d, z2, d2, q, xs, 1, &rt, g->stream);
+            } else if (fz && mode == 1) {
+                launch_apply_dir<T>(op, z2, d, d2, q, xs, 0, &rt, g->stream);
+                launch_kj_zupdate<T>(op, z2, q, &rt, g->stream);
+            } else {
+                launch_apply<T>(op, d, q, nullptr, g->stream);
+            }
         };
         float best_ms = 1e30f;
-        for (int c = 0; c < nc; c++) {
-            op.tile_tr = cand[c][0]; op.tile_tc = cand[c][1]; op.tile_lz = cand[c][2];
-            run();
-            hipEventRecord(g->ev0, g->stream);
-            for (int k = 0; k < 3; k++) run();
-            hipEventRecord(g->ev1, g->stream);
-            if (hipEventSynchronize(g->ev1) != hipSuccess) return;
-            float ms = 0;
-            hipEventElapsedTime(&ms, g->ev0, g->ev1);
-            if (ms < best_ms) {
-                best_ms = ms;
-                best[0] = cand[c][0]; best[1] = cand[c][1]; best[2] = cand[c][2];
+        for (int mode = m0; mode <= m1; mode++) {
+            if (mode > 0 && !fz) continue;
+            for (int c = 0; c < nc; c++) {
+                op.tile_tr = cand[c][0]; op.tile_tc = cand[c][1]; op.tile_lz = cand[c][2];
+                run(mode);
+                hipEventRecord(g->ev0, g->stream);
+                for (int k = 0; k < 3; k++) run(mode);
+                hipEventRecord(g->ev1, g->stream);
+                if (hipEventSynchronize(g->ev1) != hipSuccess) return;
+                float ms = 0;
+                hipEventElapsedTime(&ms, g->ev0, g->ev1);
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best_mode = mode > 0 ? mode : 1;
+                    best[0] = cand[c][0]; best[1] = cand[c][1]; best[2] = cand[c][2];
+                }
             }
         }
         op.tile_tr = best[0]; op.tile_tc = best[1]; op.tile_lz = best[2];
     };
-    // r64 holds the right-hand side here and is only read
     launch_fill<double>(g->d64, 1.0, n, g->stream);
-    tune(g->dm.op64, g->d64, g->q64, g->r64, g->s64, g->tile64);
+    tune(g->dm.op64, g->d64, g->q64, g->s64, g->tile64, g->mode64);
     if (g->precision == CHFEM_PREC_MIXED && g->d32) {
         launch_fill<float>(g->d32, 1.0, n, g->stream);
-        launch_fill<float>(g->r32, 1.0, n, g->stream);
-        tune(g->dm.op32, g->d32, g->q32, g->r32, g->s32, g->tile32);
+        tune(g->dm.op32, g->d32, g->q32, g->s32, g->tile32, g->mode32);
     } else {
         memcpy(g->tile32, g->tile64, sizeof(g->tile32));
+        g->mode32 = g->mode64;
     }
     g->tuned = true;
+}
+
+template <typename T> static int loop_mode()
+{
+    const int env = fuse_mode_env();
+    if (env >= 0) return env;
+    return std::is_same<T, double>::value ? g->mode64 : g->mode32;
 }
 
 static bool above(double delta, double delta0, double tol) { return fabs(delta) > tol * tol * fabs(delta0); }
@@ -440,6 +499,12 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
     const bool kj = op.kinv != nullptr;
     const bool fused = kj && d2 && fuse_dir_enabled() && apply_dir_ok(op);
     T *dbuf[2] = {d, d2};
+    // single-kernel iterations: iteration j reads z, q from buffer (j + 1) % 2
+    // and writes buffer j % 2; the loop's first z is r's buffer
+    T *z2 = std::is_same<T, double>::value ? (T *)g->z2_64 : (T *)g->z2_32;
+    T *q2 = std::is_same<T, double>::value ? (T *)g->q2_64 : (T *)g->q2_32;
+    const bool cg1 = fused && loop_mode<T>() == 2 && z2 && q2;
+    T *zb[2] = {z2, r}, *qb[2] = {q, q2};
     if (!x0set) launch_fill<T>(x, 0.0, n, st);
     Reduce r0 = make_reduce(FIN_STORE, S_DELTA0);
     if (kj) launch_kj_precond<T>(op, r, d, &r0, st);
@@ -481,7 +546,7 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
     // tol^2 delta0, after which every queued launch returns at once, so the
     // iterate and the iteration count are those of a loop that synchronises
     // every iteration.
-    Reduce ra = make_guarded(FIN_ALPHA, 0), rdl = make_guarded(FIN_DELTA, 0);
+    Reduce ra = make_guarded(FIN_ALPHA, 0), rdl = make_guarded(FIN_DELTA, 0), rc = make_guarded(FIN_CG1, 0);
     if (fused) {
         Reduce rz = make_reduce(FIN_STORE, S_TMP0);
         launch_kj_to_z<T>(op, r, &rz, st);
@@ -493,6 +558,11 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
         const unsigned nb = std::min(cg_batch(), maxit - it);
         for (unsigned b = 0; b < nb; b++) {
             const unsigned j = it + b;
+            if (cg1) {
+                timed_apply_cg1<T>(op, zb[(j + 1) & 1], qb[(j + 1) & 1], j > 0 ? dbuf[(j + 1) & 1] : nullptr, zb[j & 1],
+                                   dbuf[j & 1], qb[j & 1], x, j > 0 ? 1 : 0, &rc, b == 0);
+                continue;
+            }
             if (fused) {
                 T *dn = dbuf[j & 1];
                 const T *dold = j > 0 ? dbuf[(j - 1) & 1] : nullptr;
@@ -646,6 +716,12 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
     const bool fused = kj && g->s32 && fuse_dir_enabled() && apply_dir_ok(o32);
     float *dbuf[2] = {d, g->s32};
     Reduce rz = make_reduce(FIN_STORE, S_TMP0);
+    // single-kernel iterations: iteration j reads z, q from buffer (j + 1) % 2
+    // and writes buffer j % 2; a segment's fresh z goes to the buffer its
+    // first iteration reads (r's buffer for the first segment)
+    const bool cg1 = fused && loop_mode<float>() == 2 && g->z2_32 && g->q2_32;
+    float *zb[2] = {g->z2_32, r}, *qb[2] = {q, g->q2_32};
+    Reduce rc = make_guarded(FIN_CG1, 0);
     Reduce r0 = make_reduce(FIN_STORE, S_DELTA0), rd = make_reduce(FIN_STORE, S_DELTA);
     launch_precond<double>(g->dinv64, g->r64, nullptr, n, &r0, st);
     if (!sync_scalars()) return -1;
@@ -698,6 +774,11 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
             const unsigned nb = std::min(cg_batch(), maxit - base - seg);
             for (unsigned b = 0; b < nb; b++) {
                 const unsigned j = base + seg + b;
+                if (cg1) {
+                    timed_apply_cg1<float>(o32, zb[(j + 1) & 1], qb[(j + 1) & 1], j > 0 ? dbuf[(j + 1) & 1] : nullptr,
+                                           zb[j & 1], dbuf[j & 1], qb[j & 1], x, j > base ? 1 : 0, &rc, b == 0);
+                    continue;
+                }
                 if (fused) {
                     // x += alpha d of iteration j - 1 inside the operator, when
                     // that iteration belongs to this segment (x was flushed
@@ -743,8 +824,9 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
         }
         if (last || !(fabs(delta) < fabs(prev))) break;  // no progress left in FP32
         delta_rel = delta;
-        launch_convert<double, float>(g->r64, r, n, st);
-        if (fused) launch_kj_to_z<float>(o32, r, &rz, st);
+        float *rs = cg1 ? zb[(it_total + 1) & 1] : r;  // the next segment's first z
+        launch_convert<double, float>(g->r64, rs, n, st);
+        if (fused) launch_kj_to_z<float>(o32, rs, &rz, st);
         else if (!kj) launch_precond<float>(dinv, r, q, n, nullptr, st);  // stored-dinv direction reads z from q
         base = it_total;
         if (!arm_guard(thresh())) return -1;
@@ -1281,6 +1363,7 @@ cudapcgFlag_t cudapcgSetLclMtxs(cudapcgVar_t *L)
     // Walsh-basis kernel (per material {lambda, mu} or {k, 0})
     g->walsh_ok = false;
     g->tuned = false;
+    g->mode64 = g->mode32 = 1;
     if ((ph == PHY_ELASTIC || ph == PHY_THERMAL) && nk <= 4) {
         double lm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         float lmf[8];
@@ -1396,18 +1479,41 @@ double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps, cudapcgF
         launch_convert<double, float>(g->d64, g->d32, n, g->stream);
         launch_convert<double, float>(g->x64, g->x32, n, g->stream);
     }
-    Reduce red = make_reduce(FIN_STORE, S_TMP0);
+    Reduce red = make_reduce(FIN_STORE, S_TMP0), red3 = make_reduce(FIN_STORE, S_TMP0);
     const Reduce *rp = fused_dot ? &red : nullptr;
-    const bool dir = fused_dot == 2;
+    const bool dir = fused_dot == 2 || fused_dot == 3, cg1 = fused_dot == 3;
     if (dir) {
         // CG-loop form: r = x, directions alternate between d and s
         if (!g->precond || !apply_dir_ok(precision == CHFEM_PREC_MIXED ? g->dm.op32 : g->dm.op64)) return -1.0;
         if (!set_scalar(S_DELTA_OLD, 1.0) || !set_scalar(S_DELTA, 0.5) || !set_scalar(S_ALPHA, 0.0)) return -1.0;
         if (precision == CHFEM_PREC_MIXED && !g->dinv32) return -1.0;
         launch_fill<double>(g->s64, 1.0, n, g->stream);
-        if (precision == CHFEM_PREC_MIXED) launch_fill<float>(g->s32, 1.0, n, g->stream);
+        launch_fill<double>(g->q64, 0.25, n, g->stream);
+        launch_fill<double>(g->q2_64, 0.25, n, g->stream);
+        launch_fill<double>(g->z2_64, 0.5, n, g->stream);
+        if (precision == CHFEM_PREC_MIXED) {
+            launch_fill<float>(g->s32, 1.0, n, g->stream);
+            launch_fill<float>(g->q32, 0.25, n, g->stream);
+            launch_fill<float>(g->q2_32, 0.25, n, g->stream);
+            launch_fill<float>(g->z2_32, 0.5, n, g->stream);
+        }
     }
     auto once = [&](unsigned k) {
+        if (cg1) {
+            // z, q and d alternate between two buffers each (x / z2, q / q2,
+            // d / s); the deferred updates run with S[ALPHA] = 0 on the
+            // stored Jacobi diagonal, which they rewrite unchanged
+            if (precision == CHFEM_PREC_MIXED) {
+                float *zb[2] = {g->x32, g->z2_32}, *qb[2] = {g->q32, g->q2_32}, *db[2] = {g->d32, g->s32};
+                launch_apply_cg1<float>(g->dm.op32, zb[k & 1], qb[k & 1], db[k & 1], zb[(k + 1) & 1], db[(k + 1) & 1],
+                                        qb[(k + 1) & 1], g->dinv32, 1, &red3, g->stream);
+            } else {
+                double *zb[2] = {g->x64, g->z2_64}, *qb[2] = {g->q64, g->q2_64}, *db[2] = {g->d64, g->s64};
+                launch_apply_cg1<double>(g->dm.op64, zb[k & 1], qb[k & 1], db[k & 1], zb[(k + 1) & 1],
+                                         db[(k + 1) & 1], qb[(k + 1) & 1], g->dinv64, 1, &red3, g->stream);
+            }
+            return;
+        }
         if (dir) {
             // r = x; the deferred iterate update runs on the stored Jacobi
             // diagonal with S[ALPHA] = 0, which rewrites it unchanged
@@ -1431,6 +1537,12 @@ double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps, cudapcgF
     float ms = 0;
     hipEventElapsedTime(&ms, g->ev0, g->ev1);
     return reps ? ms / reps : 0.0;
+}
+
+int cudapcgGetLoopForm(cudapcgFlag_t precision)
+{
+    if (!g) return -1;
+    return precision == CHFEM_PREC_MIXED ? loop_mode<float>() : loop_mode<double>();
 }
 
 cudapcgFlag_t cudapcgGetOperatorTile(cudapcgFlag_t precision, int *tile)

@@ -28,6 +28,12 @@ enum Finalize {
     FIN_STORE = 0,      // S[slot] = sum
     FIN_ALPHA = 1,      // S[S_DQ] = sum; S[S_ALPHA] = S[S_DELTA] / sum
     FIN_DELTA = 2,      // S[S_DELTA_OLD] = S[S_DELTA]; S[S_DELTA] = sum
+    // single-kernel CG, sums (d.q, z.q, q.dinv q, z.diag z): delta = z.diag z
+    // (r.z of this iteration's z), alpha = delta / d.q, delta_new = delta -
+    // 2 alpha z.q + alpha^2 q.dinv q (r.z of the next z), S[DELTA_OLD] =
+    // delta, S[DELTA] = delta_new; guarded: counts the iteration and sets
+    // S[DONE] when |delta_new| <= S[THRESH]
+    FIN_CG1 = 3,
 };
 
 struct Reduce {
@@ -188,6 +194,14 @@ bool apply_dir_ok(const OpDesc &op);
 template <typename T>
 void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q, T *x, int xpend,
                       const Reduce *red, hipStream_t s);
+// Single-kernel Jacobi-PCG iteration (same operators as launch_apply_dir):
+// z = zold - S[ALPHA] dinv qold (skipped unless pend), d = z + beta dold
+// (dold == nullptr: d = z), znew <- z, dnew <- d, qnew = A d, deferred
+// x += S[ALPHA] dold (pend), red (FIN_CG1) <- d.q, z.q, q.dinv q, z.diag z.  The
+// new buffers must not alias the old ones, x or each other.
+template <typename T>
+void launch_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *dold, T *znew, T *dnew, T *qnew, T *x,
+                      int pend, const Reduce *red, hipStream_t s);
 // x += S[ALPHA] d (the last deferred iterate update of a loop)
 template <typename T> void launch_axpy_alpha(T *x, const T *d, const double *S, size_t n, hipStream_t s);
 }  // namespace chfem

@@ -1079,6 +1079,20 @@ __global__ void __launch_bounds__(1024) k_reduce_finish(Reduce red, unsigned nb)
                 S[S_ITER] += 1.0;
                 if (fabs(s[0]) <= S[S_THRESH]) S[S_DONE] = 1.0;
             }
+        } else if (red.fin == FIN_CG1 && NV == 4) {
+            // delta of this iteration from its own z (s[3]), not carried
+            // over from the previous recurrence step: rounding then reaches
+            // beta through one step only
+            const double delta = s[3 % NV], alpha = delta / s[0];
+            const double dnew = delta - 2.0 * alpha * s[1 % NV] + alpha * alpha * s[2 % NV];
+            S[S_DQ] = s[0];
+            S[S_ALPHA] = alpha;
+            S[S_DELTA_OLD] = delta;
+            S[S_DELTA] = dnew;
+            if (red.guard) {
+                S[S_ITER] += 1.0;
+                if (fabs(dnew) <= S[S_THRESH]) S[S_DONE] = 1.0;
+            }
         } else {
 #pragma unroll
             for (int k = 0; k < NV; k++) S[red.slot + k] = s[k];
@@ -1090,6 +1104,7 @@ void launch_reduce_finish(const Reduce &red, unsigned nb, int nv, hipStream_t s)
 {
     if (nv == 1) k_reduce_finish<1><<<1, 1024, 0, s>>>(red, nb);
     else if (nv == 3) k_reduce_finish<3><<<1, 1024, 0, s>>>(red, nb);
+    else if (nv == 4) k_reduce_finish<4><<<1, 1024, 0, s>>>(red, nb);
     else k_reduce_finish<6><<<1, 1024, 0, s>>>(red, nb);
 }
 

@@ -184,6 +184,21 @@ struct DirFuse {
     T alpha;
     int first;
     int xpend;  // apply the deferred x update with alpha = S[ALPHA]
+    // single-kernel CG (FM 2): the z update of the previous iteration is
+    // deferred here too, z = z_old - alpha dinv q_old (dinv from the packed
+    // keys, kinv[key * ND + a]; the diagonal itself at kinv[65536 * ND +
+    // key * ND + a]), written to znew on the owned nodes; with xpend clear
+    // the update is skipped (q_old reads z)
+    const T *qold;
+    T *znew;
+    const T *kinv;
+};
+
+// Sums a tile contributes to the launch's reductions: d.q, and for the
+// single-kernel CG z.q, q.(dinv q) and z.(diag z) (the r.z of the z it
+// formed)
+struct TileSums {
+    double dq, zq, qmq, zdz;
 };
 
 // Work decomposition: a block of TR x TC work-items evaluates a TR x TC
@@ -204,13 +219,16 @@ struct DirFuse {
 // one to one, the batched solver (batch.hip) through a tile table.  FUSE: the
 // input is the fused direction update of DirFuse (the dot is then d.q with
 // the updated d).
-template <typename T, int ND, int TR, int TC, bool DOT, bool FUSE = false>
-__device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restrict__ q,
-                                             const uint16_t *__restrict__ keys, const T *__restrict__ lamu, int ebe,
-                                             int ny, int nx, int nz, int LZ, int bx, int by, int bz,
-                                             DirFuse<T> fz = DirFuse<T>{})
+template <typename T, int ND, int TR, int TC, bool DOT, int FM = 0>
+__device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__restrict__ q,
+                                               const uint16_t *__restrict__ keys, const T *__restrict__ lamu, int ebe,
+                                               int ny, int nx, int nz, int LZ, int bx, int by, int bz,
+                                               DirFuse<T> fz = DirFuse<T>{})
 {
     static_assert(ND == 1 || ND == 3, "1 (conduction) or 3 (elastic) values per node");
+    // FM 0: plain apply; 1: direction update inside (FUSE); 2: single-kernel
+    // CG, the z update inside as well (CG1)
+    constexpr bool FUSE = FM >= 1, CG1 = FM == 2;
     constexpr int NT = TR * TC, NR = TR + 1, NC = TC + 1, NV = NR * NC * ND;
     constexpr int SLOTS = (NV + NT - 1) / NT;
     constexpr int SB = 2 * ND + 1;  // padded stride of the row-combined forces in LDS (no bank conflicts)
@@ -221,6 +239,8 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
     __shared__ T su[2][SLOTS * NT];
     // FUSE: the fetched d_old planes, for the owned nodes' deferred x update
     __shared__ T sdo[FUSE ? 2 : 1][FUSE ? SLOTS * NT : 1];
+    // CG1: the updated z planes, for the owned nodes' z output and z.q
+    __shared__ T szp[CG1 ? 2 : 1][CG1 ? SLOTS * NT : 1];
     __shared__ T sb[NT * SB];
     __shared__ T smat[8];
     const int tid = threadIdx.x;
@@ -260,18 +280,41 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
         dsrc = fz.first ? fz.z : v;
         beta = fz.first ? T(0) : fz.beta;
     }
+    // CG1: q_old and dinv of every plane slot; the slot nodes' keys are
+    // loaded one layer ahead (kn) so the dependent dinv read does not wait
+    T pq[CG1 ? SLOTS : 1], pm[CG1 ? SLOTS : 1];
+    uint32_t kn[CG1 ? SLOTS : 1];
+    const T *qsrc = CG1 && fz.xpend ? fz.qold : fz.z;
+    auto load_keys = [&](int lg) {
+        if constexpr (CG1) {
+            const uint16_t *kb = keys + (size_t)wrapl(lg) * nxy;
+#pragma unroll
+            for (int k = 0; k < SLOTS; k++) kn[k] = kb[goff[k] / ND];
+        }
+    };
     auto fetch = [&](int lg) {
         const size_t lo = ND * (size_t)wrapl(lg) * nxy;
 #pragma unroll
         for (int k = 0; k < SLOTS; k++) {
+            if constexpr (CG1) {
+                const uint32_t o = goff[k];
+                pm[k] = fz.kinv[ND * kn[k] + o % ND];
+                pq[k] = qsrc[lo + o];
+            }
             if constexpr (FUSE) pz[k] = fz.z[lo + goff[k]];
             pre[k] = dsrc[lo + goff[k]];
         }
+        load_keys(lg + 1);
     };
-    auto store = [&](T *dst, T *ddst) {
+    auto store = [&](T *dst, T *ddst, T *zdst) {
 #pragma unroll
         for (int k = 0; k < SLOTS; k++) {
-            if constexpr (FUSE) {
+            if constexpr (CG1) {
+                const T zk = pz[k] - fz.alpha * (pm[k] * pq[k]);
+                dst[tid + k * NT] = zk + beta * pre[k];
+                ddst[tid + k * NT] = pre[k];
+                zdst[tid + k * NT] = zk;
+            } else if constexpr (FUSE) {
                 dst[tid + k * NT] = pz[k] + beta * pre[k];
                 ddst[tid + k * NT] = pre[k];
             } else {
@@ -345,11 +388,13 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
 
     T *cur = su[0], *nxt = su[1];
     T *dcur = sdo[0], *dnxt = sdo[FUSE ? 1 : 0];  // d_old planes, swapped with cur / nxt
+    T *zcur = szp[0], *znxt = szp[CG1 ? 1 : 0];   // z planes (CG1), swapped with cur / nxt
+    load_keys(l0 - 1);
     fetch(l0 - 1);
     uint32_t mk = mat_key(l0 - 1);
-    store(cur, dcur);
+    store(cur, dcur, zcur);
     fetch(l0);
-    store(nxt, dnxt);
+    store(nxt, dnxt, znxt);
     int m = material(mk);
     mk = mat_key(l0);
     __syncthreads();
@@ -360,14 +405,16 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
     T acc[ND];
 #pragma unroll
     for (int a = 0; a < ND; a++) acc[a] = own ? Af[a] + bleft[ND + a] : T(0);
-    store(cur, dcur);  // layer l0 + 1 replaces layer l0 - 1
+    store(cur, dcur, zcur);  // layer l0 + 1 replaces layer l0 - 1
     m = material(mk);
+    uint32_t kraw = mk;  // packed key of the owned node at the current layer (CG1 dinv)
     {
         T *t = cur; cur = nxt; nxt = t;  // cur = layer l0, nxt = layer l0 + 1
         t = dcur; dcur = dnxt; dnxt = t;
+        t = zcur; zcur = znxt; znxt = t;
     }
     __syncthreads();
-    double dsum = 0.0;
+    double dsum = 0.0, zq = 0.0, qmq = 0.0, zdz = 0.0;
     // this work-item's node offset in a layer (work-items that own no node
     // use node 0 for their unconditional loads; only owners store)
     const size_t own_off = own ? (size_t)(c0 + tc - 1) * ny + (r0 + tr - 1) : 0;
@@ -389,14 +436,34 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
         // wait is left for after this layer's stores).  The last step issues
         // them too (in bounds, unused): every step then has the same loads
         // and none is pending across the loop's back edge.
+        const uint32_t kcur = kraw;
         mk = mat_key(l + 1);
+        // CG1: dinv and diag of the owned node (its key is the element's,
+        // loaded a step ahead), read now and used after this layer's elements
+        T mo[CG1 ? ND : 1], dgo[CG1 ? ND : 1];
+        if constexpr (CG1) {
+#pragma unroll
+            for (int a = 0; a < ND; a++) {
+                mo[a] = fz.kinv[ND * kcur + a];
+                dgo[a] = fz.kinv[(size_t)65536 * ND + ND * kcur + a];
+            }
+        }
         T own_u[ND];
 #pragma unroll
         for (int a = 0; a < ND; a++) own_u[a] = T(0);
+        // owned node's d (and, CG1, its updated z), read from the planes of
+        // layer l here, before any work-item replaces them (after the
+        // barrier that follows this layer's elements)
+        T own_z[CG1 ? ND : 1];
         if ((DOT || FUSE) && own) {
             const T *uu = cur + (tc * NR + tr) * ND;
 #pragma unroll
             for (int a = 0; a < ND; a++) own_u[a] = uu[a];
+            if constexpr (CG1) {
+                const T *zz = zcur + (tc * NR + tr) * ND;
+#pragma unroll
+                for (int a = 0; a < ND; a++) own_z[a] = zz[a];
+            }
         }
         if constexpr (FUSE) {
             // deferred x += alpha d_old of layer l (d_old from the LDS plane)
@@ -415,8 +482,9 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
         __syncthreads();
         // the prefetched plane goes to LDS before this layer's node results
         // are stored, so waiting for the loads does not wait for the stores
-        store(cur, dcur);  // layer l + 2 replaces layer l
+        store(cur, dcur, zcur);  // layer l + 2 replaces layer l
         m = material(mk);
+        kraw = mk;
         pin_here(m);
         if (own) {
             T o[ND];
@@ -433,6 +501,16 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
 #pragma unroll
                 for (int a = 0; a < ND; a++) dn[a] = own_u[a];
             }
+            if constexpr (CG1) {
+                T *zn = fz.znew + on;
+#pragma unroll
+                for (int a = 0; a < ND; a++) {
+                    zn[a] = own_z[a];
+                    zq += (double)own_z[a] * o[a];
+                    qmq += (double)o[a] * (double)(mo[a] * o[a]);
+                    zdz += (double)own_z[a] * (double)(dgo[a] * own_z[a]);
+                }
+            }
             if (DOT) {
 #pragma unroll
                 for (int a = 0; a < ND; a++) dsum += (double)own_u[a] * o[a];
@@ -441,10 +519,11 @@ __device__ __forceinline__ double walsh_tile(const T *__restrict__ v, T *__restr
         {
             T *t = cur; cur = nxt; nxt = t;
             t = dcur; dcur = dnxt; dnxt = t;
+            t = zcur; zcur = znxt; znxt = t;
         }
         __syncthreads();
     }
-    return dsum;
+    return TileSums{dsum, zq, qmq, zdz};
 }
 
 }  // namespace chfem

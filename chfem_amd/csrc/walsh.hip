@@ -5,11 +5,12 @@
 
 namespace chfem {
 
-template <typename T, int ND, int TR, int TC, bool DOT, bool FUSE>
+template <typename T, int ND, int TR, int TC, bool DOT, int FM>
 __global__ void __launch_bounds__(TR *TC) k_walsh(const T *__restrict__ v, T *__restrict__ q,
                                                    const uint16_t *__restrict__ keys, const T *__restrict__ lamu,
                                                    int ebe, int ny, int nx, int nz, int LZ, Reduce red, DirFuse<T> fz)
 {
+    constexpr bool FUSE = FM >= 1;
     if (DOT && stopped(red)) return;
     // beta of the fused direction update, as k_kj_pcg_dir forms it
     if (FUSE && !fz.first) fz.beta = (T)(red.S[S_DELTA] / red.S[S_DELTA_OLD]);
@@ -23,10 +24,13 @@ __global__ void __launch_bounds__(TR *TC) k_walsh(const T *__restrict__ v, T *__
     const unsigned gx = (unsigned)(ny + TR - 2) / (TR - 1), gy = (unsigned)(nx + TC - 2) / (TC - 1);
     const unsigned b = blockIdx.x, nb = gridDim.x, per = nb / 8u, rem = nb % 8u, xcd = b % 8u, k = b / 8u;
     const unsigned t = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + k;
-    const double dsum = walsh_tile<T, ND, TR, TC, DOT, FUSE>(v, q, keys, lamu, ebe, ny, nx, nz, LZ, (int)(t % gx),
-                                                              (int)((t / gx) % gy), (int)(t / (gx * gy)), fz);
-    if (DOT) {
-        double vv[1] = {dsum};
+    const TileSums ts = walsh_tile<T, ND, TR, TC, DOT, FM>(v, q, keys, lamu, ebe, ny, nx, nz, LZ, (int)(t % gx),
+                                                           (int)((t / gx) % gy), (int)(t / (gx * gy)), fz);
+    if (DOT && FM == 2) {
+        double vv[4] = {ts.dq, ts.zq, ts.qmq, ts.zdz};
+        store_partials<4>(red, vv);
+    } else if (DOT) {
+        double vv[1] = {ts.dq};
         store_partials<1>(red, vv);
     }
 }
@@ -37,8 +41,10 @@ static void launch_walsh_nd(const OpDesc &op, const T *lamu, const T *v, T *q, c
 {
     Reduce rr = {nullptr, nullptr, 0, 0, 0};
     if (red) rr = *red;
-    DirFuse<T> f = {nullptr, nullptr, nullptr, T(0), T(0), 1, 0};
+    DirFuse<T> f = {nullptr, nullptr, nullptr, T(0), T(0), 1, 0, nullptr, nullptr, nullptr};
     if (fz) f = *fz;
+    // single-kernel CG form: q_old, z_new and the dinv table given
+    const bool cg1 = fz && fz->znew;
     int LZ = t.LZ > 0 ? t.LZ : 16;
     {
         // keep the block count within the reduction partials buffer
@@ -50,15 +56,19 @@ static void launch_walsh_nd(const OpDesc &op, const T *lamu, const T *v, T *q, c
     {                                                                                                   \
         const unsigned grid = (unsigned)((op.ny + TRv - 2) / (TRv - 1)) * ((op.nx + TCv - 2) / (TCv - 1)) *    \
                               ((op.nz + LZ - 1) / LZ);                                                  \
-        if (fz) {                                                                                       \
-            k_walsh<T, ND, TRv, TCv, true, true><<<grid, TRv * TCv, 0, s>>>(                            \
+        if (cg1) {                                                                                      \
+            k_walsh<T, ND, TRv, TCv, true, 2><<<grid, TRv * TCv, 0, s>>>(                               \
+                v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);                           \
+            launch_reduce_finish(rr, grid, 4, s);                                                       \
+        } else if (fz) {                                                                                \
+            k_walsh<T, ND, TRv, TCv, true, 1><<<grid, TRv * TCv, 0, s>>>(                               \
                 v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);                           \
             launch_reduce_finish(rr, grid, 1, s);                                                       \
         } else if (red) {                                                                               \
-            k_walsh<T, ND, TRv, TCv, true, false><<<grid, TRv * TCv, 0, s>>>(                           \
+            k_walsh<T, ND, TRv, TCv, true, 0><<<grid, TRv * TCv, 0, s>>>(                               \
                 v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);                           \
             launch_reduce_finish(rr, grid, 1, s);                                                       \
-        } else k_walsh<T, ND, TRv, TCv, false, false><<<grid, TRv * TCv, 0, s>>>(                       \
+        } else k_walsh<T, ND, TRv, TCv, false, 0><<<grid, TRv * TCv, 0, s>>>(                           \
                 v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);                           \
         return;                                                                                         \
     }
@@ -87,10 +97,25 @@ template <typename T>
 void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q, T *x, int xpend,
                       const Reduce *red, hipStream_t s)
 {
-    const DirFuse<T> f = {z, dnew, x, T(0), T(0), dold ? 0 : 1, dold ? xpend : 0};
+    const DirFuse<T> f = {z, dnew, x, T(0), T(0), dold ? 0 : 1, dold ? xpend : 0, nullptr, nullptr, nullptr};
     const WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
     launch_walsh<T>(op, (const T *)op.lamu, dold, q, red, t, s, &f);
 }
+
+template <typename T>
+void launch_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *dold, T *znew, T *dnew, T *qnew, T *x,
+                      int pend, const Reduce *red, hipStream_t s)
+{
+    const int first = dold ? 0 : 1, p = dold && pend ? 1 : 0;
+    const DirFuse<T> f = {zold, dnew, x, T(0), T(0), first, p, p ? qold : zold, znew, (const T *)op.kinv};
+    const WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
+    launch_walsh<T>(op, (const T *)op.lamu, dold, qnew, red, t, s, &f);
+}
+
+template void launch_apply_cg1<float>(const OpDesc &, const float *, const float *, const float *, float *, float *,
+                                      float *, float *, int, const Reduce *, hipStream_t);
+template void launch_apply_cg1<double>(const OpDesc &, const double *, const double *, const double *, double *,
+                                       double *, double *, double *, int, const Reduce *, hipStream_t);
 
 template void launch_apply_dir<float>(const OpDesc &, const float *, const float *, float *, float *, float *, int,
                                       const Reduce *, hipStream_t);

@@ -127,6 +127,7 @@ SIGNATURES = [
     ("chfemGuessMfmaCheck", C.c_int, [vp, vp, vp]),
     ("chfemMfmaCheckF", C.c_int, [vp, vp, vp, vp, vp, vp]),
     ("cudapcgGetOperatorTile", u8, [u8, vp]),
+    ("cudapcgGetLoopForm", C.c_int, [u8]),
     ("chfemBatchCreate", vp, []),
     ("chfemBatchDestroy", None, [vp]),
     ("chfemBatchAddSample", C.c_int, [vp, cp, vp]),

@@ -40,6 +40,8 @@ def main():
     ap.add_argument("--dir", action="store_true",
                     help="the CG loop's form: direction update d = z + beta d inside the operator (Walsh NBN)")
     ap.add_argument("--both", action="store_true", help="time the --dot form, then the --dir form")
+    ap.add_argument("--cg1", action="store_true",
+                    help="also time the single-kernel CG form (z -= alpha M^-1 q inside the operator too)")
     a = ap.parse_args()
     n = a.size
     if a.physics == "fluid":
@@ -70,14 +72,19 @@ def main():
         fluid_bytes = nv.value * (4 + 8 * w) + nb.value * (5 + 2 * w)
         out.update({"pore_nodes": nv.value, "border_nodes": nb.value, "dofs": nd_.value})
     forms = ["dot", "dir"] if a.both else ["dir" if a.dir else ("dot" if a.dot else "plain")]
+    if a.cg1:
+        forms.append("cg1")
     for form in forms:
-        ms = L.cudapcgOperatorBench(prec, a.reps, {"plain": 0, "dot": 1, "dir": 2}[form])
+        ms = L.cudapcgOperatorBench(prec, a.reps, {"plain": 0, "dot": 1, "dir": 2, "cg1": 3}[form])
         if ms < 0:
             raise SystemExit("operator bench failed (the direction form needs the Walsh-basis NBN operator)")
         if a.physics == "fluid":
             nbytes = fluid_bytes
         else:
-            nbytes = (BYTES_DIR[(a.physics, prec)] if form == "dir" else BYTES[(a.physics, prec)]) * N
+            w = 1 if a.physics == "thermal" else 3
+            vb = (4 if prec == 1 else 8) * w
+            per = {"dir": BYTES_DIR[(a.physics, prec)], "cg1": 1.0 + 8 * vb}.get(form, BYTES[(a.physics, prec)])
+            nbytes = per * N
         out[form] = {"avg_launch_ms": ms, "algorithmic_bytes": nbytes, "GBps": nbytes / (ms * 1e-3) / 1e9}
     L.hmgRelease()
     L.hmgEnd()

@@ -168,10 +168,15 @@ int chfemMfmaCheckF(const float *A, const float *B, float *D, const double *A2, 
  * solver's stream and the durations are summed. */
 cudapcgFlag_t cudapcgSetKernelTiming(cudapcgFlag_t on);
 cudapcgFlag_t cudapcgGetKernelTiming(double *op_ms_total, unsigned int *op_launches);
-/* Elastic operator tile in use for a precision (rows, columns and node
+/* Walsh-basis operator tile in use for a precision (rows, columns and node
  * layers per block), as fixed by CHFEM_WALSH_TILE or chosen by the
  * autotuner before the first solve. */
 cudapcgFlag_t cudapcgGetOperatorTile(cudapcgFlag_t precision, int *tile);
+/* Form of the Jacobi-PCG loop on the Walsh-basis operators for a precision:
+ * 0 separate kernels, 1 direction and iterate updates inside the operator
+ * plus a z update kernel, 2 single-kernel iterations; fixed by
+ * CHFEM_FUSE_DIR or chosen by the autotuner (-1 without a solver). */
+int cudapcgGetLoopForm(cudapcgFlag_t precision);
 /* Same counters for the last hmgSolveResident / hmgSolveHomogenization. */
 cudapcgFlag_t hmgGetKernelTiming(double *op_ms_total, unsigned int *op_launches);
 
@@ -183,7 +188,8 @@ cudapcgFlag_t hmgGetKernelTiming(double *op_ms_total, unsigned int *op_launches)
  * inside the operator (isotropic elastic NBN with the Jacobi preconditioner;
  * reads r and the previous direction, writes d and q).  Returns the mean
  * launch time in ms from HIP events on the solver's stream, or a negative
- * value on failure. */
+ * value on failure.  fused_dot 3 times the single-kernel CG iteration (the
+ * z update z -= alpha M^-1 q inside as well; reduces d.q, z.q, q.M^-1 q). */
 double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps, cudapcgFlag_t fused_dot);
 
 /* Calibration kernel for hardware-counter traffic: a grid-stride FP64 copy of

@@ -340,11 +340,12 @@ def test_ebe_kernels(kernel, which, nphase, monkeypatch):
 
 @pytest.mark.parametrize("which", ["elastic", "thermal"])
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("fuse", ["1", "0"])
+@pytest.mark.parametrize("fuse", ["2", "1", "0"])
 def test_fused_direction_update(fuse, precision, which, monkeypatch):
-    """The CG direction update d = M^-1 r + beta d done inside the Walsh
-    operator (two alternating direction buffers; CHFEM_FUSE_DIR=1, the
-    default) and by the separate direction kernel (0), on a grid of several
+    """The CG loop forms on the Walsh operators: single-kernel iterations
+    with the z update (and the r.z recurrence) inside the operator
+    (CHFEM_FUSE_DIR=2, the default), the direction and x updates inside the
+    operator with a z update kernel (1), and separate kernels (0), on a grid of several
     blocks in every direction, from the zero guess and from a non-zero guess,
     in FP64 and mixed precision (FP64 residual replacements keep the
     direction), against the oracle."""
