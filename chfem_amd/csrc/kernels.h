@@ -195,10 +195,11 @@ template <typename T>
 void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q, T *x, int xpend,
                       const Reduce *red, hipStream_t s);
 // Single-kernel Jacobi-PCG iteration (same operators as launch_apply_dir):
-// z = zold - S[ALPHA] dinv qold (skipped unless pend), d = z + beta dold
-// (dold == nullptr: d = z), znew <- z, dnew <- d, qnew = A d, deferred
-// x += S[ALPHA] dold (pend), red (FIN_CG1) <- d.q, z.q, q.dinv q, z.diag z.  The
-// new buffers must not alias the old ones, x or each other.
+// z = zold - S[ALPHA] qold (qold holds dinv q of the previous iteration;
+// skipped unless pend), d = z + beta dold (dold == nullptr: d = z), znew <- z,
+// dnew <- d, q = A d with qnew <- dinv q, deferred x += S[ALPHA] dold (pend),
+// red (FIN_CG1) <- d.q, z.q, q.dinv q, z.diag z.  The new buffers must not
+// alias the old ones, x or each other.
 template <typename T>
 void launch_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *dold, T *znew, T *dnew, T *qnew, T *x,
                       int pend, const Reduce *red, hipStream_t s);

@@ -185,10 +185,11 @@ struct DirFuse {
     int first;
     int xpend;  // apply the deferred x update with alpha = S[ALPHA]
     // single-kernel CG (FM 2): the z update of the previous iteration is
-    // deferred here too, z = z_old - alpha dinv q_old (dinv from the packed
-    // keys, kinv[key * ND + a]; the diagonal itself at kinv[65536 * ND +
-    // key * ND + a]), written to znew on the owned nodes; with xpend clear
-    // the update is skipped (q_old reads z)
+    // deferred here too, z = z_old - alpha qt_old, written to znew on the
+    // owned nodes; this launch's q output is qt = dinv q (dinv from the
+    // packed keys of the owned nodes, kinv[key * ND + a]; the diagonal
+    // itself at kinv[65536 * ND + key * ND + a]), the only form of q the
+    // loop reads.  With xpend clear the z update is skipped (qt_old reads z)
     const T *qold;
     T *znew;
     const T *kinv;
@@ -280,37 +281,23 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
         dsrc = fz.first ? fz.z : v;
         beta = fz.first ? T(0) : fz.beta;
     }
-    // CG1: q_old and dinv of every plane slot; the slot nodes' keys are
-    // loaded one layer ahead (kn) so the dependent dinv read does not wait
-    T pq[CG1 ? SLOTS : 1], pm[CG1 ? SLOTS : 1];
-    uint32_t kn[CG1 ? SLOTS : 1];
+    // CG1: qt_old (= dinv q_old) of every plane slot
+    T pq[CG1 ? SLOTS : 1];
     const T *qsrc = CG1 && fz.xpend ? fz.qold : fz.z;
-    auto load_keys = [&](int lg) {
-        if constexpr (CG1) {
-            const uint16_t *kb = keys + (size_t)wrapl(lg) * nxy;
-#pragma unroll
-            for (int k = 0; k < SLOTS; k++) kn[k] = kb[goff[k] / ND];
-        }
-    };
     auto fetch = [&](int lg) {
         const size_t lo = ND * (size_t)wrapl(lg) * nxy;
 #pragma unroll
         for (int k = 0; k < SLOTS; k++) {
-            if constexpr (CG1) {
-                const uint32_t o = goff[k];
-                pm[k] = fz.kinv[ND * kn[k] + o % ND];
-                pq[k] = qsrc[lo + o];
-            }
+            if constexpr (CG1) pq[k] = qsrc[lo + goff[k]];
             if constexpr (FUSE) pz[k] = fz.z[lo + goff[k]];
             pre[k] = dsrc[lo + goff[k]];
         }
-        load_keys(lg + 1);
     };
     auto store = [&](T *dst, T *ddst, T *zdst) {
 #pragma unroll
         for (int k = 0; k < SLOTS; k++) {
             if constexpr (CG1) {
-                const T zk = pz[k] - fz.alpha * (pm[k] * pq[k]);
+                const T zk = pz[k] - fz.alpha * pq[k];
                 dst[tid + k * NT] = zk + beta * pre[k];
                 ddst[tid + k * NT] = pre[k];
                 zdst[tid + k * NT] = zk;
@@ -389,7 +376,6 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     T *cur = su[0], *nxt = su[1];
     T *dcur = sdo[0], *dnxt = sdo[FUSE ? 1 : 0];  // d_old planes, swapped with cur / nxt
     T *zcur = szp[0], *znxt = szp[CG1 ? 1 : 0];   // z planes (CG1), swapped with cur / nxt
-    load_keys(l0 - 1);
     fetch(l0 - 1);
     uint32_t mk = mat_key(l0 - 1);
     store(cur, dcur, zcur);
@@ -494,8 +480,13 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
                 acc[a] = Af[a] + bleft[ND + a];
             }
             T *qo = q + on;
+            if constexpr (CG1) {
 #pragma unroll
-            for (int a = 0; a < ND; a++) qo[a] = o[a];
+                for (int a = 0; a < ND; a++) qo[a] = mo[a] * o[a];
+            } else {
+#pragma unroll
+                for (int a = 0; a < ND; a++) qo[a] = o[a];
+            }
             if constexpr (FUSE) {
                 T *dn = fz.dnew + on;
 #pragma unroll
