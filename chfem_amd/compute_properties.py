@@ -20,12 +20,17 @@ def compute_property(property, array, mat_props=None, voxel_size=1e-6, solver=No
 
     :param property: 'conductivity', 'elasticity' or 'permeability'
     :param array: 3D uint8 numpy array (z, y, x) of phase ids
-    :param mat_props: {phase: k} (conductivity), {phase: (E, nu)} (elasticity), None (permeability)
+    :param mat_props: {phase: k} (conductivity), {phase: (E, nu)} or {phase: (E, nu, alpha)}
+        (elasticity; alpha, the thermal expansion coefficient, adds the thermal
+        expansion analysis and needs direction 'all'), None (permeability)
     :param solver: 'cg', 'minres', 'cg3', 'minres3', 'cg2', 'minres2' (default cg; minres for permeability)
     :param direction: 'x', 'y', 'z', 'yz', 'xz', 'xy' or 'all'
     :param precision: 'fp64' (default) or 'mixed' (FP32 inner Krylov solve with FP64
         iterative refinement to solver_tolerance)
-    :return: the effective matrix as a list of row lists
+    :param output_fields: path prefix: writes <prefix>.nf and, per load case d,
+        <prefix>_{temperature,flux|displacement,stress|velocity,pressure}_<d>.bin
+    :return: the effective matrix as a list of row lists, followed by the
+        effective thermal expansion (xx, yy, zz, yz, xz, xy) when alpha is given
     """
     properties = {"conductivity": 0, "elasticity": 1, "permeability": 2}
     if property not in properties:

@@ -54,6 +54,8 @@ struct Model {
     unsigned C_dim = 9, lclMtx_dim = 40, lclCB_dim = 0;
     double props[512];
     double alpha[256];
+    double texp[6];                   // effective thermal expansion (Voigt xx,yy,zz,yz,xz,xy)
+    bool texp_ok = false;
     uint8_t props_keys[256];
     unsigned nmat = 0;
     unsigned ex = 0, ey = 0, ez = 0;   // element counts (cols, rows, layers)
@@ -521,6 +523,10 @@ LoadCase load_case(const Model &m, unsigned dir)
 
 struct DeviceAux {
     double *CB = nullptr, *seam = nullptr, *K = nullptr;
+    double *F = nullptr;      // corner field matrices (field export)
+    double *Bc = nullptr;     // centre strain matrix per material (thermal expansion)
+    double *da = nullptr;     // per material (D alpha)_xx (thermal expansion)
+    double *fout = nullptr;   // nodal field output
 };
 
 bool upload_aux(const Model &m, DeviceAux &a)
@@ -537,6 +543,10 @@ bool upload_aux(const Model &m, DeviceAux &a)
 
 void free_aux(DeviceAux &a)
 {
+    if (a.F) hipFree(a.F);
+    if (a.Bc) hipFree(a.Bc);
+    if (a.da) hipFree(a.da);
+    if (a.fout) hipFree(a.fout);
     if (a.CB) hipFree(a.CB);
     if (a.seam) hipFree(a.seam);
     if (a.K) hipFree(a.K);
@@ -586,6 +596,206 @@ bool device_updateC(Model &m, const DeviceAux &a, unsigned dir, const double *x)
     const double *S = scalars_host();
     const int nr = m.analysis == HMG_ELASTIC ? 6 : 3;
     for (int row = 0; row < nr; row++) m.C[row * nr + dir] = S[S_TMP0 + row] / (double)m.nelem;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// field export (femhmg_thermal_3D.c:694-1105, femhmg_elastic_3D.c:1800-2645,
+// femhmg_fluid_3D.c:312-360)
+// ---------------------------------------------------------------------------
+// grad N_j at corner k of the unit voxel (element frame, z = 1 - layer offset):
+// only corners sharing the other two coordinates with j contribute
+void corner_grad(int k, int j, double g[3])
+{
+    int xk, yk, zk, xj, yj, zj;
+    frame(k, false, xk, yk, zk);
+    frame(j, false, xj, yj, zj);
+    g[0] = (yj == yk && zj == zk) ? (xj ? 1.0 : -1.0) : 0.0;
+    g[1] = (xj == xk && zj == zk) ? (yj ? 1.0 : -1.0) : 0.0;
+    g[2] = (xj == xk && yj == yk) ? (zj ? 1.0 : -1.0) : 0.0;
+}
+
+// strain (Voigt, engineering shear) of displacement component b of node j
+// for gradient g of N_j
+void strain_of(const double g[3], int b, double eps[6])
+{
+    for (int r = 0; r < 6; r++) eps[r] = 0;
+    eps[b] = g[b];
+    if (b == 0) { eps[4] = g[2]; eps[5] = g[1]; }
+    if (b == 1) { eps[3] = g[2]; eps[5] = g[0]; }
+    if (b == 2) { eps[3] = g[1]; eps[4] = g[0]; }
+}
+
+// per material, per corner k: thermal k grad N (3x8), elastic D B (6x24)
+std::vector<double> corner_field_matrices(const Model &m)
+{
+    const bool el = m.analysis == HMG_ELASTIC;
+    const int nd = el ? 3 : 1, ke = 8 * nd, nf = el ? 6 : 3, sz = nf * ke;
+    std::vector<double> F((size_t)m.nmat * 8 * sz, 0.0);
+    for (unsigned t = 0; t < m.nmat; t++)
+        for (int k = 0; k < 8; k++) {
+            double *Fk = &F[((size_t)t * 8 + k) * sz];
+            for (int j = 0; j < 8; j++) {
+                double g[3];
+                corner_grad(k, j, g);
+                if (!el) {
+                    for (int a = 0; a < 3; a++) Fk[a * ke + j] = m.props[t] * g[a];
+                    continue;
+                }
+                const double E = m.props[2 * t], nu = m.props[2 * t + 1];
+                const double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
+                for (int b = 0; b < 3; b++) {
+                    double eps[6];
+                    strain_of(g, b, eps);
+                    const double tr = eps[0] + eps[1] + eps[2];
+                    for (int r = 0; r < 3; r++) Fk[r * ke + 3 * j + b] = lam * tr + 2 * mu * eps[r];
+                    for (int r = 3; r < 6; r++) Fk[r * ke + 3 * j + b] = mu * eps[r];
+                }
+            }
+        }
+    return F;
+}
+
+bool write_bin(const std::string &name, const double *v, size_t n)
+{
+    FILE *f = fopen(name.c_str(), "wb");
+    if (!f) {
+        printf("WARNING: could not write %s\n", name.c_str());
+        return false;
+    }
+    const bool ok = fwrite(v, sizeof(double), n, f) == n;
+    fclose(f);
+    return ok;
+}
+
+// writes <nf>_{temperature,flux|displacement,stress|velocity,pressure}_<dir>.bin
+// for the solution in the solver's x buffer
+bool export_fields(const Model &m, DeviceAux &a, unsigned dir)
+{
+    const double t0 = omp_get_wtime();
+    printf("    Exporting simulation results...");
+    std::vector<double> x(m.ndof);
+    if (hipMemcpy(x.data(), solution_buffer(), m.ndof * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return false;
+    const std::string base = m.nf_noext + "_";
+    const std::string sfx = "_" + std::to_string(dir) + ".bin";
+    bool ok;
+    if (m.analysis == HMG_FLUID) {
+        std::vector<double> vel(3 * m.nelem, 0.0), pres(m.nelem, 0.0);
+#pragma omp parallel for schedule(static)
+        for (long n = 0; n < (long)m.nelem; n++) {
+            const uint32_t id = m.dof_id[n];
+            if (m.fkeys[n] == 255)
+                for (int c = 0; c < 3; c++) vel[3 * n + c] = x[3 * (size_t)id + c];
+            if (m.fkeys[n]) pres[n] = x[3 * (size_t)m.nvel + id];
+        }
+        ok = write_bin(base + "velocity" + sfx, vel.data(), vel.size()) &&
+             write_bin(base + "pressure" + sfx, pres.data(), pres.size());
+    } else {
+        const bool el = m.analysis == HMG_ELASTIC;
+        const size_t nf = el ? 6 : 3;
+        if (!a.F) {
+            const std::vector<double> F = corner_field_matrices(m);
+            if (hipMalloc(&a.F, F.size() * sizeof(double)) != hipSuccess ||
+                hipMemcpy(a.F, F.data(), F.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+                return false;
+        }
+        if (!a.fout && hipMalloc(&a.fout, nf * m.nelem * sizeof(double)) != hipSuccess) return false;
+        DeviceModel *dm = active_model();
+        launch_fields(dm->op64, a.F, load_case(m, dir), 1, solution_buffer(), a.fout, active_stream());
+        std::vector<double> fld(nf * m.nelem);
+        if (hipStreamSynchronize(active_stream()) != hipSuccess ||
+            hipMemcpy(fld.data(), a.fout, fld.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            return false;
+        ok = write_bin(base + (el ? "displacement" : "temperature") + sfx, x.data(), x.size()) &&
+             write_bin(base + (el ? "stress" : "flux") + sfx, fld.data(), fld.size());
+    }
+    printf("Done.(%.2e s)\n", omp_get_wtime() - t0);
+    return ok;
+}
+
+// ---------------------------------------------------------------------------
+// thermal expansion (femhmg.c:1121-1165, femhmg_elastic_3D.c:464-508,
+// 863-1073): solve K u = sum_e B_c^T D_e a_e (no seam jump) and
+// alpha* = C^-1 beta with beta = <D_e (a_e - B_c u_e)> + C <B_c u_e>.
+// ---------------------------------------------------------------------------
+bool thermal_expansion_pass(Model &m, DeviceAux &a)
+{
+    appendf("    ------------------------------------------------------\n    Thermal expansion analysis:\n");
+    const size_t nm = m.nmat;
+    std::vector<double> da(nm), Bc(144 * nm);
+    for (unsigned t = 0; t < nm; t++) {
+        const double E = m.props[2 * t], nu = m.props[2 * t + 1];
+        const double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
+        da[t] = (3 * lam + 2 * mu) * m.alpha[t];  // (D a)_xx for a = alpha (1,1,1,0,0,0)
+    }
+    // centre strain matrix (6x24), the same for every material
+    for (unsigned t = 0; t < nm; t++)
+        for (int k = 0; k < 8; k++) {
+            int x, y, z;
+            frame(k, false, x, y, z);
+            const double g[3] = {(x ? 0.25 : -0.25), (y ? 0.25 : -0.25), (z ? 0.25 : -0.25)};
+            for (int b = 0; b < 3; b++) {
+                double eps[6];
+                strain_of(g, b, eps);
+                for (int r = 0; r < 6; r++) Bc[144 * t + r * 24 + 3 * k + b] = eps[r];
+            }
+        }
+    if ((!a.da && hipMalloc(&a.da, nm * sizeof(double)) != hipSuccess) ||
+        (!a.Bc && hipMalloc(&a.Bc, Bc.size() * sizeof(double)) != hipSuccess))
+        return false;
+    hipMemcpy(a.da, da.data(), nm * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(a.Bc, Bc.data(), Bc.size() * sizeof(double), hipMemcpyHostToDevice);
+    hipMemset(a.seam, 0, 6 * nm * sizeof(double));
+    DeviceModel *dm = active_model();
+    launch_rhs_thermal_expansion(dm->op64, a.da, rhs_buffer(), active_stream());
+    if (hipStreamSynchronize(active_stream()) != hipSuccess) return false;
+    cudapcgSetX0(nullptr, CUDAPCG_FALSE);
+    const int rc = solve_device(nullptr);
+    chfemSolveStats_t st;
+    cudapcgGetSolveStats(&st);
+    m.stats.push_back(st);
+    appendf("    | Solver ran %u iterations (%u sweeps), residual %.3e, %.3e s\n", st.iterations, st.sweeps,
+            st.residual, st.solve_seconds);
+    if (rc < 0) return false;
+    // <D_e B_c u_e> and <B_c u_e>: the C-column kernel with a zero seam term
+    const LoadCase lc0 = load_case(m, 0);
+    double sum_s[6], sum_e[6];
+    for (int pass = 0; pass < 2; pass++) {
+        Reduce red = make_reduce(FIN_STORE, S_TMP0);
+        launch_updateC(dm->op64, pass ? a.Bc : a.CB, a.seam, lc0, solution_buffer(), &red, active_stream());
+        if (!sync_scalars()) return false;
+        const double *S = scalars_host();
+        for (int r = 0; r < 6; r++) (pass ? sum_e : sum_s)[r] = S[S_TMP0 + r];
+    }
+    std::vector<size_t> count(nm, 0);
+    for (size_t e = 0; e < m.nelem; e++) count[m.elem_map[e]]++;
+    double da_sum = 0;
+    for (unsigned t = 0; t < nm; t++) da_sum += (double)count[t] * da[t];
+    double stress[6], strain[6], beta[6], Mx[36];
+    for (int r = 0; r < 6; r++) {
+        stress[r] = (sum_s[r] - (r < 3 ? da_sum : 0.0)) / (double)m.nelem;
+        strain[r] = sum_e[r] / (double)m.nelem;
+    }
+    for (int i = 0; i < 6; i++) {
+        beta[i] = -stress[i];
+        for (int j = 0; j < 6; j++) beta[i] += m.C[6 * i + j] * strain[j];
+    }
+    // alpha* = C^-1 beta, Gaussian elimination without pivoting (C is SPD)
+    memcpy(Mx, m.C, sizeof(Mx));
+    for (int i = 0; i < 5; i++)
+        for (int j = i + 1; j < 6; j++) {
+            const double f = Mx[6 * j + i] / Mx[7 * i];
+            for (int k = i + 1; k < 6; k++) Mx[6 * j + k] -= Mx[6 * i + k] * f;
+            beta[j] -= beta[i] * f;
+            Mx[6 * j + i] = 0;
+        }
+    for (int i = 5; i >= 0; i--) {
+        double v = beta[i];
+        for (int j = 5; j > i; j--) v -= Mx[6 * i + j] * m.texp[j];
+        m.texp[i] = v / Mx[7 * i];
+    }
+    m.texp_ok = true;
     return true;
 }
 
@@ -693,8 +903,6 @@ logical hmgInit(char *data_filename, char *elem_filename, char *sdf_filename, ui
     if (m.analysis == HMG_FLUID) build_fluid_maps(m);
     else build_keys(m);
     assemble_local(m);
-    if (m.thermal_expansion)
-        printf("WARNING: thermal expansion analysis is not provided by this library; alpha is ignored.\n");
     printf("Done.(%.2e s)\n", omp_get_wtime() - g_time_total);
     return HMG_TRUE;
 }
@@ -930,11 +1138,24 @@ logical hmgSolveResident(var *Cout)
         if (g_solver >= CUDAPCG_CG2_SOLVER && m.analysis == HMG_FLUID) {
             printf("    Effective fields:\n");
             for (unsigned jj = 0; jj < 3; jj++) printf("        %.8e\n", m.C[jj * 3 + d]);
+        } else if (m.save_fields && !export_fields(m, g_aux, d)) {
+            ok = false;
+            break;
         }
     }
     if (x0dev) (void)hipFree(x0dev);
+    m.texp_ok = false;
+    if (ok && m.thermal_expansion) {
+        if (m.analysis != HMG_ELASTIC)
+            printf("    WARNING: Thermal expansion analysis only available for ELASTIC.\n");
+        else if (m.have_sdf)
+            printf("    WARNING: Thermal expansion analysis is not available with a scalar density field.\n");
+        else if (dirs.size() < 6)
+            printf("    WARNING: Thermal expansion analysis needs the six load cases (direction 'all').\n");
+        else if (!thermal_expansion_pass(m, g_aux))
+            ok = false;
+    }
     cudapcgGetKernelTiming(&g_op_ms, &g_op_count);
-    if (m.save_fields) printf("WARNING: field export is not provided by this library.\n");
     appendf("    ------------------------------------------------------\n");
     if (Cout) memcpy(Cout, m.C, m.C_dim * sizeof(double));
     return ok ? HMG_TRUE : HMG_FALSE;
@@ -1014,12 +1235,20 @@ void hmgPrintConstitutiveMtx(void)
         printf("\n");
     }
     printf("-------------------------------------------------------\n");
+    if (const double *a = hmgGetThermalExpansion()) {
+        // symmetric 3x3 from Voigt xx, yy, zz, yz, xz, xy (femhmg_elastic_3D.c:1625-1630)
+        printf("Effective thermal expansion:\n");
+        printf("  %.8e  %.8e  %.8e\n", a[0], a[5], a[4]);
+        printf("  %.8e  %.8e  %.8e\n", a[5], a[1], a[3]);
+        printf("  %.8e  %.8e  %.8e\n", a[4], a[3], a[2]);
+        printf("-------------------------------------------------------\n");
+    }
 }
 
 var *hmgGetConstitutiveMtx(void) { return M ? M->C : nullptr; }
 unsigned int hmgGetConstitutiveMtxDim(void) { return M ? M->C_dim : 0; }
-var *hmgGetThermalExpansion(void) { return nullptr; }
-unsigned int hmgGetThermalExpansionDim(void) { return 0; }
+var *hmgGetThermalExpansion(void) { return M && M->thermal_expansion && M->texp_ok ? M->texp : nullptr; }
+unsigned int hmgGetThermalExpansionDim(void) { return hmgGetThermalExpansion() ? 6 : 0; }
 
 logical hmgSaveConstitutiveMtx(const char *filename)
 {
@@ -1027,6 +1256,7 @@ logical hmgSaveConstitutiveMtx(const char *filename)
     FILE *f = fopen(filename, "wb");
     if (!f) return HMG_FALSE;
     fwrite(M->C, sizeof(double), M->C_dim, f);
+    if (hmgGetThermalExpansion()) fwrite(M->texp, sizeof(double), 6, f);
     fclose(f);
     return HMG_TRUE;
 }
@@ -1154,6 +1384,10 @@ int runAnalysis(chfemgpuInput_t *u)
     const unsigned cd = hmgGetConstitutiveMtxDim();
     for (unsigned i = 0; i < cd; i++) u->eff_coeff[i] = C[i];
     u->num_of_thermal_expansion_coeffs = 0;
+    if (const double *a = hmgGetThermalExpansion()) {
+        u->num_of_thermal_expansion_coeffs = hmgGetThermalExpansionDim();
+        for (unsigned i = 0; i < u->num_of_thermal_expansion_coeffs; i++) u->eff_coeff[cd + i] = a[i];
+    }
     hmgEnd();
     printf("#######################################################\n");
     return 0;
@@ -1176,7 +1410,7 @@ int chfem_run(const uint8_t *img, size_t img_size, const char *nf_file, int anal
     }
     if (runAnalysis(&u) != 0) return -1;
     const unsigned n = (analysis == 0 || analysis == 2) ? 3 : 6;
-    for (unsigned i = 0; i < n * n; i++) eff_out[i] = u.eff_coeff[i];
+    for (unsigned i = 0; i < n * n + u.num_of_thermal_expansion_coeffs; i++) eff_out[i] = u.eff_coeff[i];
     if (n_out) *n_out = n;
     if (alpha_out) *alpha_out = u.num_of_thermal_expansion_coeffs;
     return 0;

@@ -161,6 +161,13 @@ void launch_rhs(const OpDesc &op, const double *K64, const LoadCase &lc, int dir
 void launch_updateC(const OpDesc &op, const double *CB64, const double *seamCB, const LoadCase &lc,
                     const double *x, const Reduce *red, hipStream_t s);
 template <typename T> void launch_fluid_velocity_sum(const OpDesc &op, const T *x, const Reduce *red, hipStream_t s);
+// thermal expansion load (elastic); da[m] = (D_m alpha_m (1,1,1,0,0,0))_xx per material
+void launch_rhs_thermal_expansion(const OpDesc &op, const double *da, double *b, hipStream_t s);
+// nodal flux (thermal, 3 per node) or stress (elastic, 6 per node) for field
+// export; F: per material, per corner, the field matrix (NF x 8*nd); jump = 0
+// leaves the seam jump out (thermal expansion load case)
+void launch_fields(const OpDesc &op, const double *F, const LoadCase &lc, int jump, const double *x, double *out,
+                   hipStream_t s);
 
 // x0 from a half-resolution solution (rows/cols/layers = fine element counts)
 void launch_interpl2(const double *coarse, double *fine, int rows, int cols, int layers, int stride, hipStream_t s);

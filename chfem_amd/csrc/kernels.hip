@@ -990,6 +990,99 @@ __global__ void __launch_bounds__(BS) k_fluid_vsum(const T *__restrict__ x, uint
 }
 
 // ---------------------------------------------------------------------------
+// thermal expansion load (femhmg_elastic_3D.c:464-508), gathered node by node:
+// node i receives, from each element e that has it as local node k,
+// B_c(k)^T D_e a_e with the eigenstrain a_e = alpha_e (1,1,1,0,0,0) and B_c
+// the strain matrix at the element centre.  Only the normal rows of B_c meet
+// column (k, a), so the term is +-1/4 (D_e a_e)_a; da[m] holds
+// (D_m a_m)_xx = (D_m a_m)_yy = (D_m a_m)_zz per material.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BS) k_rhs_texp(double *b, const uint16_t *__restrict__ keys,
+                                                 const double *__restrict__ da, int ebe, int ny, int nx, int nz,
+                                                 uint32_t N)
+{
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS) {
+        int r, c, l;
+        node_rcl(i, ny, nx, r, c, l);
+        const unsigned key = ebe ? 0u : keys[i];
+        double acc[3] = {0, 0, 0};
+        for (int k = 0; k < 8; k++) {
+            int m;
+            if (ebe) {
+                int re = r - cDR[k], ce = c - cDC[k], le = l - cDL[k];
+                re += re < 0 ? ny : 0; ce += ce < 0 ? nx : 0; le += le < 0 ? nz : 0;
+                m = (int)keys[(size_t)re + (size_t)ce * ny + (size_t)le * nx * ny];
+            } else {
+                m = (int)((key >> (2 * k)) & 3u);
+            }
+            const double f = 0.25 * da[m];
+            // element frame: x = column offset, y = 1 - row offset, z = 1 - layer offset
+            acc[0] += cDC[k] ? f : -f;
+            acc[1] += cDR[k] ? -f : f;
+            acc[2] += cDL[k] ? -f : f;
+        }
+        b[3 * (size_t)i] = acc[0];
+        b[3 * (size_t)i + 1] = acc[1];
+        b[3 * (size_t)i + 2] = acc[2];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// nodal fields for export (femhmg_thermal_3D.c:694-1105,
+// femhmg_elastic_3D.c:1800-2645): node i gets 1/8 of the flux k grad T
+// (thermal, NF = 3) or stress D B u (elastic, NF = 6) of each element that
+// has it as a corner, evaluated at that corner.  The element's values carry
+// the load case's seam jump (L on component comp of the seam-side corners S
+// of every seam element), which is the reference's seam compensation up to a
+// rigid translation.  F: per material, per corner k, the NF x 8*ND matrix
+// mapping element values to the corner field.
+// ---------------------------------------------------------------------------
+template <int ND>
+__global__ void __launch_bounds__(BS) k_fields(const double *__restrict__ x, const uint16_t *__restrict__ keys,
+                                               const double *__restrict__ F, int ebe, int ny, int nx, int nz,
+                                               uint32_t N, LoadCase lc, int jump, double *out)
+{
+    constexpr int KE = 8 * ND, NF = ND == 3 ? 6 : 3;
+    for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < N; i += gridDim.x * BS) {
+        int r, c, l;
+        node_rcl(i, ny, nx, r, c, l);
+        const unsigned key = ebe ? 0u : keys[i];
+        double acc[NF];
+#pragma unroll
+        for (int q = 0; q < NF; q++) acc[q] = 0;
+        for (int k = 0; k < 8; k++) {
+            int re = r - cDR[k], ce = c - cDC[k], le = l - cDL[k];
+            re += re < 0 ? ny : 0; ce += ce < 0 ? nx : 0; le += le < 0 ? nz : 0;
+            const size_t el = (size_t)re + (size_t)ce * ny + (size_t)le * nx * ny;
+            const int m = ebe ? (int)keys[el] : (int)((key >> (2 * k)) & 3u);
+            double ue[KE];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                int rj = re + cDR[j], cj = ce + cDC[j], lj = le + cDL[j];
+                rj -= rj >= ny ? ny : 0; cj -= cj >= nx ? nx : 0; lj -= lj >= nz ? nz : 0;
+                const size_t nj = (size_t)rj + (size_t)cj * ny + (size_t)lj * nx * ny;
+#pragma unroll
+                for (int a = 0; a < ND; a++) ue[ND * j + a] = x[ND * nj + a];
+            }
+            if (jump && (lc.axis == 0 ? ce == nx - 1 : (lc.axis == 1 ? re == 0 : le == 0))) {
+#pragma unroll
+                for (int t = 0; t < 4; t++) ue[ND * lc.S[t] + lc.comp] += lc.L;
+            }
+            const double *Fk = F + ((size_t)m * 8 + k) * NF * KE;
+#pragma unroll
+            for (int q = 0; q < NF; q++) {
+                double s = 0;
+#pragma unroll
+                for (int j = 0; j < KE; j++) s = fma(Fk[q * KE + j], ue[j], s);
+                acc[q] += 0.125 * s;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NF; q++) out[(size_t)NF * i + q] = acc[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // coarse -> fine initial guess (chfem/src/femhmg/cudapcg/kernels/image.cu:19-121)
 // ---------------------------------------------------------------------------
 __global__ void k_project2(double *res, const double *v, uint32_t nx, uint32_t ny, uint32_t nz, int stride)
@@ -1318,6 +1411,20 @@ void launch_updateC(const OpDesc &op, const double *CB64, const double *seamCB, 
         LAUNCH_RED(6, k_updateC<1>, grid_for(op.N), s, *red, x, op.keys, CB64, seamCB, op.ebe, op.ny, op.nx, op.nz, op.N, lc, *red, op.sdf);
     else
         LAUNCH_RED(6, k_updateC<3>, grid_for(op.N), s, *red, x, op.keys, CB64, seamCB, op.ebe, op.ny, op.nx, op.nz, op.N, lc, *red, op.sdf);
+}
+
+void launch_rhs_thermal_expansion(const OpDesc &op, const double *da, double *b, hipStream_t s)
+{
+    k_rhs_texp<<<grid_for(op.N), BS, 0, s>>>(b, op.keys, da, op.ebe, op.ny, op.nx, op.nz, op.N);
+}
+
+void launch_fields(const OpDesc &op, const double *F, const LoadCase &lc, int jump, const double *x, double *out,
+                   hipStream_t s)
+{
+    if (op.physics == PHY_THERMAL)
+        k_fields<1><<<grid_for(op.N), BS, 0, s>>>(x, op.keys, F, op.ebe, op.ny, op.nx, op.nz, op.N, lc, jump, out);
+    else
+        k_fields<3><<<grid_for(op.N), BS, 0, s>>>(x, op.keys, F, op.ebe, op.ny, op.nx, op.nz, op.N, lc, jump, out);
 }
 
 template <typename T>

@@ -1,5 +1,6 @@
-"""Input/output helpers with the reference's behaviour (chfem/io.py:3-109, :208-335):
-neutral-file (.nf) reading and writing and raw uint8 images."""
+"""Input/output helpers with the reference's behaviour (chfem/io.py:3-335):
+neutral-file (.nf) reading and writing, raw uint8 images and the exported
+fields."""
 import numpy as np
 
 
@@ -47,6 +48,41 @@ def import_raw(filename, shape=None, dtype=np.uint8):
             raise ValueError(f"File size ({len(raw)}) does not match expected size ({expected})")
     a = np.frombuffer(raw, dtype=dtype)
     return a.reshape(shape) if shape is not None else a
+
+
+def _read_field(filename, domain_shape, ncomp, rotate_domain, dtype):
+    """Exported fields are stored node by node in the solver's order
+    (row fastest, then column, then layer), ncomp values per node."""
+    data = np.fromfile(filename, dtype=dtype)
+    if len(domain_shape) == 2:
+        ny, nx = domain_shape
+        a = data.reshape((nx, ny) + ((ncomp,) if ncomp > 1 else ()))   # [x, y, (comp)]
+        if ncomp == 1:
+            return a if rotate_domain else a.T
+        return a.transpose(2, 0, 1) if rotate_domain else a.transpose(2, 1, 0)
+    nz, ny, nx = domain_shape
+    a = data.reshape((nz, nx, ny) + ((ncomp,) if ncomp > 1 else ()))  # [z, x, y, (comp)]
+    if ncomp == 1:
+        return a.transpose(1, 2, 0) if rotate_domain else a.transpose(0, 2, 1)
+    return a.transpose(3, 1, 2, 0) if rotate_domain else a.transpose(3, 0, 2, 1)
+
+
+def import_scalar_field_from_chfem(filename, domain_shape, rotate_domain=False, dtype=np.float64):
+    """Scalar field (temperature, pressure) of an exported analysis, shaped like
+    the image (z, y, x), or (x, y, z) with rotate_domain (chfem/io.py:111-140)."""
+    return _read_field(filename, domain_shape, 1, rotate_domain, dtype)
+
+
+def import_vector_field_from_chfem(filename, domain_shape, rotate_domain=False, dtype=np.float64):
+    """Vector field (flux, displacement, velocity): (3, z, y, x), or (3, x, y, z)
+    with rotate_domain; 2 components in 2D (chfem/io.py:142-173)."""
+    return _read_field(filename, domain_shape, 2 if len(domain_shape) == 2 else 3, rotate_domain, dtype)
+
+
+def import_stress_field_from_chfem(filename, domain_shape, rotate_domain=False, dtype=np.float64):
+    """Stress field, Voigt order xx, yy, zz, yz, xz, xy: (6, z, y, x), or
+    (6, x, y, z) with rotate_domain; 3 components in 2D (chfem/io.py:175-206)."""
+    return _read_field(filename, domain_shape, 3 if len(domain_shape) == 2 else 6, rotate_domain, dtype)
 
 
 def export_for_chfem(filename, array, analysis_type, mat_props=None, voxel_size=1, solver_type=0, rhs_type=0,

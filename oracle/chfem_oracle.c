@@ -45,6 +45,10 @@ static const int DR[8] = {1, 1, 0, 0, 1, 1, 0, 0};
 static const int DC[8] = {0, 1, 1, 0, 0, 1, 1, 0};
 static const int DL[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 
+/* load case index of the thermal expansion analysis (HOMOGENIZE_THERMAL_EXPANSION
+ * is 7 in chfem/src/femhmg/includes.h:45; here it follows the six seam cases) */
+#define ORC_DIR_TEXP 6
+
 /* periodic node/element index, PERIODICNUM_3D (utils.h:82) */
 static inline uint32_t pidx(long r, long c, long l, long ny, long nx, long nz)
 {
@@ -296,6 +300,7 @@ typedef struct {
     double *K;           /* dense per material: 64 (thermal), 576 (elastic); 1024 (fluid) */
     double *CB;          /* 24 per mat thermal (3x8), 144 per mat elastic, 80 fluid */
     double props[512];
+    double alpha[256];   /* thermal expansion coefficient per material (elastic) */
     double *esc;         /* scalar density field: per-element scale of K and CB (NULL: none) */
 } orc_model;
 
@@ -607,11 +612,18 @@ static size_t seam_elem(const orc_model *m, int axis, size_t i)
 }
 static double seam_L(const orc_model *m, int axis) { return axis == 0 ? m->nx : (axis == 1 ? m->ny : m->nz); }
 
+static void rhs_thermal_expansion(const orc_model *m, double *b);
+
+/* dir 0..5: seam-jump load cases; dir 6 (elastic): thermal expansion */
 void orc_rhs(void *h, int dir, double *b)
 {
     orc_model *m = (orc_model *)h;
     const long ny = m->ny, nx = m->nx, nz = m->nz;
     memset(b, 0, m->ndof * sizeof(double));
+    if (dir == ORC_DIR_TEXP) {
+        rhs_thermal_expansion(m, b);
+        return;
+    }
     if (m->analysis == ORC_FLUID) {
         for (size_t e = 0; e < m->N; e++) {
             if (m->elem[e] != 0) continue;
@@ -701,6 +713,194 @@ void orc_updateC(void *h, int dir, const double *x, double *C)
     }
     acc[0] += a0; acc[1] += a1; acc[2] += a2; acc[3] += a3; acc[4] += a4; acc[5] += a5;
     for (int row = 0; row < nrow; row++) C[row * nrow + dir] = acc[row] / (double)m->N;
+}
+
+/* =========================================================================
+ * Thermal expansion (elastic only; femhmg_elastic_3D.c:464-508, 863-1073,
+ * femhmg.c:1121-1165).  Every element carries the isotropic eigenstrain
+ * a_e (1,1,1,0,0,0) for a unit temperature rise.  Load: f_e = B(centre)^T D_e
+ * a_e scattered to the element's nodes, no seam jump.  With x the periodic
+ * solution, sigma = <D_e (B x_e - a_e)> and eps = <B x_e> (averages over
+ * elements, centre point), beta = -sigma + C eps and alpha* = C^-1 beta by
+ * Gaussian elimination without pivoting, as femhmg_elastic_3D.c:1045-1063.
+ * ========================================================================= */
+void orc_set_alpha(void *h, const double *alpha)
+{
+    orc_model *m = (orc_model *)h;
+    memcpy(m->alpha, alpha, sizeof(double) * m->nmat);
+}
+
+static void elastic_D(const orc_model *m, unsigned t, double D[6][6])
+{
+    const double E = m->props[2 * t], nu = m->props[2 * t + 1];
+    dmat(E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu)), D);
+}
+
+static void rhs_thermal_expansion(const orc_model *m, double *b)
+{
+    const long ny = m->ny, nx = m->nx, nz = m->nz;
+    double B[6][24];
+    bmat(0, 0.5, 0.5, 0.5, B);
+    for (size_t e = 0; e < m->N; e++) {
+        const unsigned t = m->elem[e];
+        double D[6][6], Da[6];
+        elastic_D(m, t, D);
+        for (int i = 0; i < 6; i++) Da[i] = (D[i][0] + D[i][1] + D[i][2]) * m->alpha[t];
+        long r = e % ny, c = (e / ny) % nx, l = e / (nx * ny);
+        for (int k = 0; k < 8; k++) {
+            size_t n = 3 * (size_t)pidx(r + DR[k], c + DC[k], l + DL[k], ny, nx, nz);
+            for (int j = 0; j < 3; j++) {
+                double f = 0;
+                for (int i = 0; i < 6; i++) f += B[i][3 * k + j] * Da[i];
+                b[n + j] += f;
+            }
+        }
+    }
+}
+
+/* C: the 6x6 effective matrix (row-major); out: alpha* in Voigt order
+ * xx, yy, zz, yz, xz, xy */
+void orc_thermal_expansion(void *h, const double *x, const double *C, double *out)
+{
+    orc_model *m = (orc_model *)h;
+    const long ny = m->ny, nx = m->nx, nz = m->nz;
+    double B[6][24], stress[6] = {0, 0, 0, 0, 0, 0}, strain[6] = {0, 0, 0, 0, 0, 0};
+    bmat(0, 0.5, 0.5, 0.5, B);
+    for (size_t e = 0; e < m->N; e++) {
+        const unsigned t = m->elem[e];
+        const double *CB = m->CB + 144 * (size_t)t;
+        long r = e % ny, c = (e / ny) % nx, l = e / (nx * ny);
+        double ue[24];
+        for (int k = 0; k < 8; k++) {
+            size_t n = 3 * (size_t)pidx(r + DR[k], c + DC[k], l + DL[k], ny, nx, nz);
+            for (int j = 0; j < 3; j++) ue[3 * k + j] = x[n + j];
+        }
+        double D[6][6];
+        elastic_D(m, t, D);
+        for (int i = 0; i < 6; i++) {
+            double s = 0, g = 0;
+            for (int j = 0; j < 24; j++) { s += CB[i * 24 + j] * ue[j]; g += B[i][j] * ue[j]; }
+            stress[i] += s - (D[i][0] + D[i][1] + D[i][2]) * m->alpha[t];
+            strain[i] += g;
+        }
+    }
+    double beta[6], M[36];
+    for (int i = 0; i < 6; i++) { stress[i] /= (double)m->N; strain[i] /= (double)m->N; }
+    for (int i = 0; i < 6; i++) {
+        beta[i] = -stress[i];
+        for (int j = 0; j < 6; j++) beta[i] += C[6 * i + j] * strain[j];
+    }
+    memcpy(M, C, sizeof(M));
+    for (int i = 0; i < 5; i++)
+        for (int j = i + 1; j < 6; j++) {
+            const double f = M[6 * j + i] / M[7 * i];
+            for (int k = i + 1; k < 6; k++) M[6 * j + k] -= M[6 * i + k] * f;
+            beta[j] -= beta[i] * f;
+            M[6 * j + i] = 0;
+        }
+    for (int i = 5; i >= 0; i--) {
+        double a = beta[i];
+        for (int j = 5; j > i; j--) a -= M[6 * i + j] * out[j];
+        out[i] = a / M[7 * i];
+    }
+}
+
+/* =========================================================================
+ * Nodal fields for export (femhmg_thermal_3D.c:694-1105,
+ * femhmg_elastic_3D.c:1800-2645, femhmg_fluid_3D.c:312-360).
+ *   thermal: per node, 1/8 of k grad T (3 values) of every element that has
+ *            the node as a corner, evaluated at that corner;
+ *   elastic: the same with the stress D B u (6 values, Voigt order);
+ *   then, for the seam elements of load case dir, the field of the seam
+ *   jump: corners off the jumped face take the field of the jump L on the
+ *   jumped nodes, corners on it the field of -L on the other nodes
+ *   (the reference's split; the two differ by a rigid translation).
+ * out: 3 (thermal) or 6 (elastic) values per periodic node.
+ * ========================================================================= */
+static void corner_field(const orc_model *m, unsigned t, int k, const double *ue, double *f)
+{
+    double xk, yk, zk;
+    node_xyz(k, 0, &xk, &yk, &zk);
+    if (m->analysis == ORC_THERMAL) {
+        f[0] = f[1] = f[2] = 0;
+        for (int j = 0; j < 8; j++) {
+            double g[3];
+            grad_shape(j, 0, xk, yk, zk, g, NULL);
+            for (int a = 0; a < 3; a++) f[a] += g[a] * ue[j];
+        }
+        for (int a = 0; a < 3; a++) f[a] *= m->props[t];
+        return;
+    }
+    double B[6][24], D[6][6], eps[6];
+    bmat(0, xk, yk, zk, B);
+    elastic_D(m, t, D);
+    for (int i = 0; i < 6; i++) {
+        eps[i] = 0;
+        for (int j = 0; j < 24; j++) eps[i] += B[i][j] * ue[j];
+    }
+    for (int i = 0; i < 6; i++) {
+        f[i] = 0;
+        for (int j = 0; j < 6; j++) f[i] += D[i][j] * eps[j];
+    }
+}
+
+void orc_fields(void *h, int dir, const double *x, double *out)
+{
+    orc_model *m = (orc_model *)h;
+    const long ny = m->ny, nx = m->nx, nz = m->nz;
+    const int nd = m->analysis == ORC_ELASTIC ? 3 : 1, nf = nd == 3 ? 6 : 3;
+    memset(out, 0, sizeof(double) * nf * m->N);
+    for (size_t e = 0; e < m->N; e++) {
+        long r = e % ny, c = (e / ny) % nx, l = e / (nx * ny);
+        size_t node[8];
+        double ue[24], f[6];
+        for (int k = 0; k < 8; k++) {
+            node[k] = pidx(r + DR[k], c + DC[k], l + DL[k], ny, nx, nz);
+            for (int a = 0; a < nd; a++) ue[nd * k + a] = x[nd * node[k] + a];
+        }
+        for (int k = 0; k < 8; k++) {
+            corner_field(m, m->elem[e], k, ue, f);
+            for (int i = 0; i < nf; i++) out[nf * node[k] + i] += 0.125 * f[i];
+        }
+    }
+    if (dir < 0 || dir > 5) return;
+    const loadcase_t lc = LOADCASES[dir];
+    const int comp = nd == 3 ? lc.comp : 0;
+    const double L = seam_L(m, lc.seam_axis);
+    const size_t ns = seam_count(m, lc.seam_axis);
+    for (size_t i = 0; i < ns; i++) {
+        const size_t e = seam_elem(m, lc.seam_axis, i);
+        long r = e % ny, c = (e / ny) % nx, l = e / (nx * ny);
+        int on[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int t = 0; t < 4; t++) on[lc.S[t]] = 1;
+        double jp[24], jm[24], f[6];
+        memset(jp, 0, sizeof(jp));
+        memset(jm, 0, sizeof(jm));
+        for (int k = 0; k < 8; k++) {
+            if (on[k]) jp[nd * k + comp] = L;
+            else jm[nd * k + comp] = -L;
+        }
+        for (int k = 0; k < 8; k++) {
+            const size_t n = pidx(r + DR[k], c + DC[k], l + DL[k], ny, nx, nz);
+            corner_field(m, m->elem[e], k, on[k] ? jm : jp, f);
+            for (int q = 0; q < nf; q++) out[nf * n + q] += 0.125 * f[q];
+        }
+    }
+}
+
+/* Stokes: velocity (3 per node, pore nodes only) and pressure (1 per node,
+ * every node with a fluid key) from the DOF vector */
+void orc_fluid_fields(void *h, const double *x, double *vel, double *pres)
+{
+    orc_model *m = (orc_model *)h;
+    for (size_t n = 0; n < m->N; n++) {
+        const uint32_t id = m->dof_id[n];
+        vel[3 * n] = vel[3 * n + 1] = vel[3 * n + 2] = 0;
+        pres[n] = 0;
+        if (m->fkeys[n] == 255)
+            for (int a = 0; a < 3; a++) vel[3 * n + a] = x[3 * (size_t)id + a];
+        if (m->fkeys[n]) pres[n] = x[3 * (size_t)m->nvel + id];
+    }
 }
 
 /* =========================================================================

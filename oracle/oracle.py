@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
 THERMAL, ELASTIC, FLUID = 0, 1, 2
+TEXP = 6  # load case index of the thermal expansion analysis (elastic)
 
 _lib = None
 
@@ -60,6 +61,10 @@ def lib():
         L.orc_set_element_scale.argtypes = [vp, vp]
         L.orc_interpl2.argtypes = [vp, C.c_uint, C.c_uint, C.c_uint, C.c_uint, vp]
         L.orc_coarse_elem_map.argtypes = [vp, C.c_uint, C.c_uint, C.c_uint, vp]
+        L.orc_set_alpha.argtypes = [vp, vp]
+        L.orc_thermal_expansion.argtypes = [vp, vp, vp, vp]
+        L.orc_fields.argtypes = [vp, i32, vp, vp]
+        L.orc_fluid_fields.argtypes = [vp, vp, vp, vp]
         L.orc_num_threads.restype = i32
         L.orc_set_threads.argtypes = [i32]
         L.orc_wtime.restype = dbl
@@ -183,6 +188,37 @@ class Model:
         ok = lib().orc_solve(self.h, direction, solver, precond, strategy, tol, maxit, x0p, _p(x),
                              C.byref(it), C.byref(res))
         return x, bool(ok), it.value, res.value
+
+    def set_alpha(self, alpha):
+        """Thermal expansion coefficient per material (elastic)."""
+        self._alpha = np.zeros(256)
+        a = np.asarray(alpha, dtype=np.float64).ravel()
+        self._alpha[:a.size] = a
+        lib().orc_set_alpha(self.h, _p(self._alpha))
+
+    def thermal_expansion(self, x, C):
+        """alpha* (Voigt xx, yy, zz, yz, xz, xy) from the solution of load case TEXP
+        and the 6x6 effective matrix C."""
+        out = np.zeros(6)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        Cm = np.ascontiguousarray(C, dtype=np.float64).reshape(36)
+        lib().orc_thermal_expansion(self.h, _p(x), _p(Cm), _p(out))
+        return out
+
+    def fields(self, direction, x):
+        """Nodal flux (thermal, 3 per node) or stress (elastic, 6 per node) as exported."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros((self.N, 6 if self.analysis == ELASTIC else 3))
+        lib().orc_fields(self.h, direction, _p(x), _p(out))
+        return out
+
+    def fluid_fields(self, x):
+        """Stokes: (velocity N x 3, pressure N) as exported."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        vel = np.zeros((self.N, 3))
+        pres = np.zeros(self.N)
+        lib().orc_fluid_fields(self.h, _p(x), _p(vel), _p(pres))
+        return vel, pres
 
     def homogenize(self, directions=None, **kw):
         """Per-direction RHS -> solve -> updateC, as hmgSolveHomogenization
