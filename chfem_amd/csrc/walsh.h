@@ -85,9 +85,34 @@ __device__ __forceinline__ void iwht(const Walsh<T> &F, T (&f)[8])
     }
 }
 
-// element forces from nodal displacements u[k][a]; lam, mu of the element
+// Per-material coefficients of the element force maps, with the Walsh
+// normalisations folded in (set up once per block by material_coefficients):
+// elastic  c = {lam/16, mu/16, mu/8, lam/48, (lam+3mu)/48, mu/48, (lam+4mu)/144}
+// conduction c = {k/16, k/24, k/48}
+constexpr int WCOEF = 8;
+
 template <typename T>
-__device__ __forceinline__ void element_forces(const T (&u)[8][3], T lam, T mu, T (&f)[8][3])
+__device__ __forceinline__ void material_coefficients(int nd, T p0, T p1, T *c)
+{
+    if (nd == 3) {
+        c[0] = p0 * T(1.0 / 16.0);
+        c[1] = p1 * T(1.0 / 16.0);
+        c[2] = p1 * T(1.0 / 8.0);
+        c[3] = p0 * T(1.0 / 48.0);
+        c[4] = (p0 + 3 * p1) * T(1.0 / 48.0);
+        c[5] = p1 * T(1.0 / 48.0);
+        c[6] = (p0 + 4 * p1) * T(1.0 / 144.0);
+    } else {
+        c[0] = p0 * T(1.0 / 16.0);
+        c[1] = p0 * T(1.0 / 24.0);
+        c[2] = p0 * T(1.0 / 48.0);
+    }
+}
+
+// element forces from nodal displacements u[k][a]; c: the element's material
+// coefficients (material_coefficients, elastic)
+template <typename T>
+__device__ __forceinline__ void element_forces(const T (&u)[8][3], const T *c_, T (&f)[8][3])
 {
     T c[8];
     Walsh<T> W[3];
@@ -97,36 +122,34 @@ __device__ __forceinline__ void element_forces(const T (&u)[8][3], T lam, T mu, 
         for (int k = 0; k < 8; k++) c[k] = u[k][a];
         W[a] = wht(c);
     }
-    const T l2 = lam + 2 * mu, l3 = lam + 3 * mu, l4 = lam + 4 * mu;
-    const T r16 = T(1.0 / 16.0), r48 = T(1.0 / 48.0), r144 = T(1.0 / 144.0);
-    const T tr = W[0].x + W[1].y + W[2].z;
+    const T l16 = c_[0], m16 = c_[1], m8 = c_[2], l48 = c_[3], l3 = c_[4], m48 = c_[5], l4 = c_[6];
+    const T ltr = l16 * (W[0].x + W[1].y + W[2].z);
     const T S = W[0].yz + W[1].xz + W[2].xy;
     Walsh<T> F[3];
     // component x
-    F[0].x = (lam * tr + 2 * mu * W[0].x) * r16;
-    F[0].y = mu * (W[0].y + W[1].x) * r16;
-    F[0].z = mu * (W[0].z + W[2].x) * r16;
-    F[0].xy = (l3 * W[0].xy + lam * W[2].yz) * r48;
-    F[0].xz = (l3 * W[0].xz + lam * W[1].yz) * r48;
-    F[0].yz = mu * (S + W[0].yz) * r48;
-    F[0].xyz = l4 * W[0].xyz * r144;
+    F[0].x = ltr + m8 * W[0].x;
+    F[0].y = m16 * (W[0].y + W[1].x);
+    F[0].z = m16 * (W[0].z + W[2].x);
+    F[0].xy = l3 * W[0].xy + l48 * W[2].yz;
+    F[0].xz = l3 * W[0].xz + l48 * W[1].yz;
+    F[0].yz = m48 * (S + W[0].yz);
+    F[0].xyz = l4 * W[0].xyz;
     // component y
     F[1].x = F[0].y;
-    F[1].y = (lam * tr + 2 * mu * W[1].y) * r16;
-    F[1].z = mu * (W[1].z + W[2].y) * r16;
-    F[1].xy = (l3 * W[1].xy + lam * W[2].xz) * r48;
-    F[1].yz = (l3 * W[1].yz + lam * W[0].xz) * r48;
-    F[1].xz = mu * (S + W[1].xz) * r48;
-    F[1].xyz = l4 * W[1].xyz * r144;
+    F[1].y = ltr + m8 * W[1].y;
+    F[1].z = m16 * (W[1].z + W[2].y);
+    F[1].xy = l3 * W[1].xy + l48 * W[2].xz;
+    F[1].yz = l3 * W[1].yz + l48 * W[0].xz;
+    F[1].xz = m48 * (S + W[1].xz);
+    F[1].xyz = l4 * W[1].xyz;
     // component z
     F[2].x = F[0].z;
     F[2].y = F[1].z;
-    F[2].z = (lam * tr + 2 * mu * W[2].z) * r16;
-    F[2].xz = (l3 * W[2].xz + lam * W[1].xy) * r48;
-    F[2].yz = (l3 * W[2].yz + lam * W[0].xy) * r48;
-    F[2].xy = mu * (S + W[2].xy) * r48;
-    F[2].xyz = l4 * W[2].xyz * r144;
-    (void)l2;
+    F[2].z = ltr + m8 * W[2].z;
+    F[2].xz = l3 * W[2].xz + l48 * W[1].xy;
+    F[2].yz = l3 * W[2].yz + l48 * W[0].xy;
+    F[2].xy = m48 * (S + W[2].xy);
+    F[2].xyz = l4 * W[2].xyz;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         iwht(F[a], c);
@@ -143,11 +166,12 @@ __device__ __forceinline__ void element_forces(const T (&u)[8][3], T lam, T mu, 
 // This reproduces k * {1/3 diagonal, 0 between nodes that differ in one
 // coordinate, -1/12 across face and body diagonals}
 // (chfem/src/femhmg/physical_phenomena/femhmg_thermal_3D.c:42-120).
+// (c: the material coefficients {k/16, k/24, k/48})
 template <typename T>
-__device__ __forceinline__ void conduction_forces(const T (&u)[8], T k, T (&f)[8])
+__device__ __forceinline__ void conduction_forces(const T (&u)[8], const T *c, T (&f)[8])
 {
     const Walsh<T> W = wht(u);
-    const T a1 = k * T(1.0 / 16.0), a2 = k * T(1.0 / 24.0), a3 = k * T(1.0 / 48.0);
+    const T a1 = c[0], a2 = c[1], a3 = c[2];
     Walsh<T> F;
     F.x = a1 * W.x;
     F.y = a1 * W.y;
@@ -160,6 +184,21 @@ __device__ __forceinline__ void conduction_forces(const T (&u)[8], T k, T (&f)[8
 }
 
 __device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// value of lane - 1 of the wavefront (a DPP whole-wave shift right by one
+// lane; lane 0 receives 0).  Same as __shfl_up(v, 1) for lanes 1..63, without
+// the LDS permute.
+__device__ __forceinline__ float lane_up1(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double lane_up1(double v)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
 
 // Keeps a value's computation at this point of the instruction stream (an
 // empty asm that consumes it; no instruction is emitted).  Used where the
@@ -243,13 +282,17 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     // CG1: the updated z planes, for the owned nodes' z output and z.q
     __shared__ T szp[CG1 ? 2 : 1][CG1 ? SLOTS * NT : 1];
     __shared__ T sb[NT * SB];
-    __shared__ T smat[8];
+    // material coefficients of materials 0..3, and material 4 (all zero) for
+    // the work-items without an element: they evaluate a zero element instead
+    // of branching around the evaluation
+    __shared__ T smat[5 * WCOEF];
     const int tid = threadIdx.x;
     const int tr = tid % TR, tc = tid / TR;
     const int r0 = bx * (TR - 1), c0 = by * (TC - 1), l0 = bz * LZ;
     const int vr = min(TR - 1, ny - r0), vc = min(TC - 1, nx - c0), vl = min(LZ, nz - l0);
     const size_t nxy = (size_t)nx * ny;
-    if (tid < 8) smat[tid] = lamu[tid];
+    if (tid < 5) material_coefficients<T>(ND, tid < 4 ? lamu[2 * tid] : T(0), tid < 4 ? lamu[2 * tid + 1] : T(0),
+                                          smat + WCOEF * tid);
 
     // node-layer load pattern: LDS index idx = (nc * NR + nr) * ND + comp holds node
     // (r0 - 1 + nr, c0 - 1 + nc); the in-layer global offset is the same for every
@@ -264,11 +307,13 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
         const int gr = wrapi(r0 - 1 + nr, ny), gc = wrapi(c0 - 1 + nc, nx);
         goff[k] = ok ? (uint32_t)ND * ((uint32_t)gc * (uint32_t)ny + (uint32_t)gr) + (uint32_t)comp : 0u;
     }
-    // layer index modulo nz for any lg (the prefetches run up to the last layer + 2)
+    // layer index modulo nz for any lg (the prologue; the loop steps its
+    // wrapped layer indices incrementally)
     auto wrapl = [&](int lg) {
         int lw = lg % nz;
         return lw < 0 ? lw + nz : lw;
     };
+    auto next_layer = [&](int lw) { return lw + 1 == nz ? 0 : lw + 1; };
     T pre[SLOTS];
     // FUSE: the direction update's operands (z and d_old) are fetched with
     // the plane and combined when the plane is stored to LDS; d_old is kept
@@ -284,13 +329,16 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     // CG1: qt_old (= dinv q_old) of every plane slot
     T pq[CG1 ? SLOTS : 1];
     const T *qsrc = CG1 && fz.xpend ? fz.qold : fz.z;
-    auto fetch = [&](int lg) {
-        const size_t lo = ND * (size_t)wrapl(lg) * nxy;
+    // fetch(lw): the plane of wrapped layer lw (uniform base pointer, 32-bit
+    // per-slot offsets)
+    auto fetch = [&](int lw) {
+        const size_t lo = ND * (size_t)lw * nxy;
+        const T *bq = qsrc + lo, *bz = fz.z + lo, *bd = dsrc + lo;
 #pragma unroll
         for (int k = 0; k < SLOTS; k++) {
-            if constexpr (CG1) pq[k] = qsrc[lo + goff[k]];
-            if constexpr (FUSE) pz[k] = fz.z[lo + goff[k]];
-            pre[k] = dsrc[lo + goff[k]];
+            if constexpr (CG1) pq[k] = bq[goff[k]];
+            if constexpr (FUSE) pz[k] = bz[goff[k]];
+            pre[k] = bd[goff[k]];
         }
     };
     auto store = [&](T *dst, T *ddst, T *zdst) {
@@ -317,10 +365,11 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     // is unconditional)
     const uint32_t eoff = elem_ok ? (uint32_t)wrapi(c0 - 1 + tc, nx) * (uint32_t)ny + (uint32_t)wrapi(r0 - 1 + tr, ny)
                                   : 0u;
-    auto mat_key = [&](int le) -> uint32_t { return keys[(size_t)wrapl(le) * nxy + eoff]; };
+    // (mat_key takes the wrapped layer index)
+    auto mat_key = [&](int lw) -> uint32_t { return (keys + (size_t)lw * nxy)[eoff]; };
     auto material = [&](uint32_t kv) -> int {
         // NBN keys: node (r,c,l) is local node 3 of element (r,c,l); EBE: element map
-        if (!elem_ok) return 0;
+        if (!elem_ok) return 4;
         return ebe ? (int)(kv & 3u) : (int)((kv >> 6) & 3u);
     };
     // Node (tr, tc) of a layer receives local node k's force from the element
@@ -333,40 +382,36 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     // are formed with cross-lane moves; B and Bf go through LDS (2 ND values
     // per element).
     static_assert(TR == 16 || TR == 32 || TR == 64, "rows per tile must tile a 64-wide wavefront");
+    // (work-items without an element read in-tile LDS entries, all written
+    // each step from finite vector values, and evaluate them with the zero
+    // material 4)
     auto element = [&](const T *cur, const T *nxt, int m, T (&A)[ND], T (&Af)[ND]) {
         T f[8][ND];
-        if (elem_ok) {
-            T u[8][ND];
+        T u[8][ND];
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const T *src = WDL(k) ? nxt : cur;
-                const int o = ((tc + WDC(k)) * NR + tr + WDR(k)) * ND;
+        for (int k = 0; k < 8; k++) {
+            const T *src = WDL(k) ? nxt : cur;
+            const int o = ((tc + WDC(k)) * NR + tr + WDR(k)) * ND;
 #pragma unroll
-                for (int a = 0; a < ND; a++) u[k][a] = src[o + a];
-            }
-            if constexpr (ND == 3) {
-                element_forces(u, smat[2 * m], smat[2 * m + 1], f);
-            } else {
-                T u1[8], f1[8];
-#pragma unroll
-                for (int k = 0; k < 8; k++) u1[k] = u[k][0];
-                conduction_forces(u1, smat[2 * m], f1);
-#pragma unroll
-                for (int k = 0; k < 8; k++) f[k][0] = f1[k];
-            }
+            for (int a = 0; a < ND; a++) u[k][a] = src[o + a];
+        }
+        if constexpr (ND == 3) {
+            element_forces(u, smat + WCOEF * m, f);
         } else {
+            T u1[8], f1[8];
 #pragma unroll
-            for (int k = 0; k < 8; k++)
+            for (int k = 0; k < 8; k++) u1[k] = u[k][0];
+            conduction_forces(u1, smat + WCOEF * m, f1);
 #pragma unroll
-                for (int a = 0; a < ND; a++) f[k][a] = T(0);
+            for (int k = 0; k < 8; k++) f[k][0] = f1[k];
         }
         T *dst = sb + tid * SB;
 #pragma unroll
         for (int a = 0; a < ND; a++) {
-            A[a] = f[3][a] + __shfl_up(f[0][a], 1);
-            Af[a] = f[7][a] + __shfl_up(f[4][a], 1);
-            dst[a] = f[2][a] + __shfl_up(f[1][a], 1);
-            dst[ND + a] = f[6][a] + __shfl_up(f[5][a], 1);
+            A[a] = f[3][a] + lane_up1(f[0][a]);
+            Af[a] = f[7][a] + lane_up1(f[4][a]);
+            dst[a] = f[2][a] + lane_up1(f[1][a]);
+            dst[ND + a] = f[6][a] + lane_up1(f[5][a]);
         }
     };
     const bool own = tr >= 1 && tc >= 1 && tr - 1 < vr && tc - 1 < vc;
@@ -376,15 +421,17 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     T *cur = su[0], *nxt = su[1];
     T *dcur = sdo[0], *dnxt = sdo[FUSE ? 1 : 0];  // d_old planes, swapped with cur / nxt
     T *zcur = szp[0], *znxt = szp[CG1 ? 1 : 0];   // z planes (CG1), swapped with cur / nxt
-    fetch(l0 - 1);
-    uint32_t mk = mat_key(l0 - 1);
+    const int lwm = wrapl(l0 - 1), lw0 = wrapl(l0);
+    int lw1 = next_layer(lw0), lw2 = next_layer(lw1);  // wrapped l + 1, l + 2 of the step
+    fetch(lwm);
+    uint32_t mk = mat_key(lwm);
     store(cur, dcur, zcur);
-    fetch(l0);
+    fetch(lw0);
     store(nxt, dnxt, znxt);
     int m = material(mk);
-    mk = mat_key(l0);
+    mk = mat_key(lw0);
     __syncthreads();
-    fetch(l0 + 1);
+    fetch(lw1);
     T A[ND], Af[ND];
     element(cur, nxt, m, A, Af);
     __syncthreads();
@@ -423,7 +470,7 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
         // them too (in bounds, unused): every step then has the same loads
         // and none is pending across the loop's back edge.
         const uint32_t kcur = kraw;
-        mk = mat_key(l + 1);
+        mk = mat_key(lw1);
         // CG1: dinv and diag of the owned node (its key is the element's,
         // loaded a step ahead), read now and used after this layer's elements
         T mo[CG1 ? ND : 1], dgo[CG1 ? ND : 1];
@@ -434,14 +481,12 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
                 dgo[a] = fz.kinv[(size_t)65536 * ND + ND * kcur + a];
             }
         }
-        T own_u[ND];
-#pragma unroll
-        for (int a = 0; a < ND; a++) own_u[a] = T(0);
         // owned node's d (and, CG1, its updated z), read from the planes of
         // layer l here, before any work-item replaces them (after the
-        // barrier that follows this layer's elements)
-        T own_z[CG1 ? ND : 1];
-        if ((DOT || FUSE) && own) {
+        // barrier that follows this layer's elements).  Every work-item reads
+        // its in-tile entry; only owners use it.
+        T own_u[ND], own_z[CG1 ? ND : 1];
+        {
             const T *uu = cur + (tc * NR + tr) * ND;
 #pragma unroll
             for (int a = 0; a < ND; a++) own_u[a] = uu[a];
@@ -459,11 +504,13 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
 #pragma unroll
                 for (int a = 0; a < ND; a++) xo[a] = pxn[a] + fz.alpha * dd[a];
             }
-            const size_t o1 = ND * ((size_t)wrapl(l + 1) * nxy + own_off);
+            const size_t o1 = ND * ((size_t)lw1 * nxy + own_off);
 #pragma unroll
             for (int a = 0; a < ND; a++) pxn[a] = fz.x[o1 + a];
         }
-        fetch(l + 2);
+        fetch(lw2);
+        lw1 = lw2;
+        lw2 = next_layer(lw2);
         element(cur, nxt, m, A, Af);
         __syncthreads();
         // the prefetched plane goes to LDS before this layer's node results
