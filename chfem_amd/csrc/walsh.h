@@ -366,7 +366,7 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     const uint32_t eoff = elem_ok ? (uint32_t)wrapi(c0 - 1 + tc, nx) * (uint32_t)ny + (uint32_t)wrapi(r0 - 1 + tr, ny)
                                   : 0u;
     // (mat_key takes the wrapped layer index)
-    auto mat_key = [&](int lw) -> uint32_t { return (keys + (size_t)lw * nxy)[eoff]; };
+    auto mat_key = [&](int lw) -> uint16_t { return (keys + (size_t)lw * nxy)[eoff]; };
     auto material = [&](uint32_t kv) -> int {
         // NBN keys: node (r,c,l) is local node 3 of element (r,c,l); EBE: element map
         if (!elem_ok) return 4;
@@ -424,7 +424,7 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     const int lwm = wrapl(l0 - 1), lw0 = wrapl(l0);
     int lw1 = next_layer(lw0), lw2 = next_layer(lw1);  // wrapped l + 1, l + 2 of the step
     fetch(lwm);
-    uint32_t mk = mat_key(lwm);
+    uint16_t mk = mat_key(lwm);
     store(cur, dcur, zcur);
     fetch(lw0);
     store(nxt, dnxt, znxt);
@@ -516,6 +516,10 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
         // the prefetched plane goes to LDS before this layer's node results
         // are stored, so waiting for the loads does not wait for the stores
         store(cur, dcur, zcur);  // layer l + 2 replaces layer l
+        // first use of the key loaded at the top of the step: without this
+        // the compiler consumes it (and reuses its register) right after the
+        // load, exposing the load's latency at the top of every step
+        asm volatile("" : "+v"(mk));
         m = material(mk);
         kraw = mk;
         pin_here(m);
