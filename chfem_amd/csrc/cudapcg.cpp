@@ -431,7 +431,7 @@ static void autotune_tiles()
                 op.tile_tr = cand[c][0]; op.tile_tc = cand[c][1]; op.tile_lz = cand[c][2];
                 run(mode);
                 hipEventRecord(g->ev0, g->stream);
-                for (int k = 0; k < 3; k++) run(mode);
+                for (int k = 0; k < 8; k++) run(mode);
                 hipEventRecord(g->ev1, g->stream);
                 if (hipEventSynchronize(g->ev1) != hipSuccess) return;
                 float ms = 0;

@@ -196,10 +196,10 @@ __device__ __forceinline__ double stokes_tile(const T *__restrict__ v, T *__rest
         T *dst = sb + tid * SB;
 #pragma unroll
         for (int a = 0; a < 4; a++) {
-            A[a] = f[3][a] + __shfl_up(f[0][a], 1);
-            Af[a] = f[7][a] + __shfl_up(f[4][a], 1);
-            dst[a] = f[2][a] + __shfl_up(f[1][a], 1);
-            dst[4 + a] = f[6][a] + __shfl_up(f[5][a], 1);
+            A[a] = f[3][a] + lane_up1(f[0][a]);
+            Af[a] = f[7][a] + lane_up1(f[4][a]);
+            dst[a] = f[2][a] + lane_up1(f[1][a]);
+            dst[4 + a] = f[6][a] + lane_up1(f[5][a]);
         }
     };
     const bool own = tr >= 1 && tc >= 1 && tr - 1 < vr && tc - 1 < vc;
