@@ -217,10 +217,13 @@ def main():
     # x written; form 1: z, d_old, x read + d, q, x written; form 0: v read +
     # q written.
     form = fast["form"]
-    nvec = {2: 8, 1: 6}.get(form, 2)
+    # vectors streamed per node: plain apply d, q; form 1 z, d_old, d, q;
+    # form 2 z_old, qt_old, d_old, z, d, qt (the iterate is updated per window
+    # of iterations by the ring flush, outside these launches)
+    nvec = {2: 6, 1: 4}.get(form, 2)
     per_node32 = 1.0 + nvec * 12.0
     plain_form = plain.get("form", form) if plain is not None else form
-    per_node64 = 1.0 + {2: 8, 1: 6}.get(plain_form, 2) * 24.0
+    per_node64 = 1.0 + {2: 6, 1: 4}.get(plain_form, 2) * 24.0
     bytes_per_launch = per_node32 * N
     achieved = bytes_per_launch / (op_avg_ms * 1e-3) / 1e9
     key = {2: "cg1_", 1: "dir_"}.get(form, "")
@@ -228,9 +231,9 @@ def main():
     tr_, tc_, lz_ = fast["tile"].split("x")
     kernel = {
         2: f"k_walsh<float,3,{tr_},{tc_},true,2> (FP32 elastic NBN single-kernel CG iteration: z -= alpha M^-1 q, "
-           f"d = z + beta d, q = A d, x += alpha d, d.q z.q q.M^-1q z.Dz), LZ {lz_}",
+           f"d = z + beta d, q = A d, d.q z.q q.M^-1q z.Dz), LZ {lz_}",
         1: f"k_walsh<float,3,{tr_},{tc_},true,1> (FP32 elastic NBN apply with the CG direction update "
-           f"d = z + beta d (z = M^-1 r), the deferred x += alpha d and d.Ad fused), LZ {lz_}",
+           f"d = z + beta d (z = M^-1 r) and d.Ad fused), LZ {lz_}",
     }.get(form, f"k_walsh<float,3,{tr_},{tc_},true,0> (FP32 elastic NBN apply + fused d.Ad), LZ {lz_}")
     cpu = None
     if not args.skip_cpu and plain is not None:

@@ -54,6 +54,11 @@ struct Solver {
     // second z and q buffers of the single-kernel CG loop
     double *z2_64 = nullptr, *q2_64 = nullptr;
     float *z2_32 = nullptr, *q2_32 = nullptr;
+    // direction ring of the fused CG loops beyond d and s (slots 2 .. w - 1;
+    // w = 0: not set up yet), per precision
+    double *ring64[RING_MAX] = {};
+    float *ring32[RING_MAX] = {};
+    int ring_w64 = 0, ring_w32 = 0;
     bool arrays = false, user_arrays = false, x0_set = false, dinv_valid = false;
     double *partials = nullptr, *S = nullptr, *Sh = nullptr;
     size_t partials_n = 0;
@@ -196,10 +201,10 @@ static bool fuse_dir_enabled() { return fuse_mode_env() != 0; }
 
 template <typename T>
 static void timed_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *dold, T *znew, T *dnew, T *qnew,
-                            T *x, int pend, const Reduce *red, bool timed)
+                            int pend, const Reduce *red, bool timed)
 {
     if (timed && g->timing) hipEventRecord(g->opA, g->stream);
-    launch_apply_cg1<T>(op, zold, qold, dold, znew, dnew, qnew, x, pend, red, g->stream);
+    launch_apply_cg1<T>(op, zold, qold, dold, znew, dnew, qnew, pend, red, g->stream);
     if (timed && g->timing) {
         hipEventRecord(g->opB, g->stream);
         op_pending = true;
@@ -207,11 +212,10 @@ static void timed_apply_cg1(const OpDesc &op, const T *zold, const T *qold, cons
 }
 
 template <typename T>
-static void timed_apply_dir(const OpDesc &op, const T *r, const T *dold, T *dnew, T *q, T *x, int xpend,
-                            const Reduce *red)
+static void timed_apply_dir(const OpDesc &op, const T *r, const T *dold, T *dnew, T *q, const Reduce *red)
 {
     if (g->timing) hipEventRecord(g->opA, g->stream);
-    launch_apply_dir<T>(op, r, dold, dnew, q, x, xpend, red, g->stream);
+    launch_apply_dir<T>(op, r, dold, dnew, q, red, g->stream);
     if (g->timing) {
         hipEventRecord(g->opB, g->stream);
         op_pending = true;
@@ -252,6 +256,8 @@ static cudapcgFlag_t free_solver_arrays()
     dfree(g->d64); dfree(g->q64); dfree(g->s64); dfree(g->dinv64);
     dfree(g->x32); dfree(g->r32); dfree(g->d32); dfree(g->q32); dfree(g->s32); dfree(g->dinv32);
     dfree(g->z2_64); dfree(g->q2_64); dfree(g->z2_32); dfree(g->q2_32);
+    for (int k = 0; k < RING_MAX; k++) { dfree(g->ring64[k]); dfree(g->ring32[k]); }
+    g->ring_w64 = g->ring_w32 = 0;
     dfree(g->partials); dfree(g->S);
     if (g->Sh) hipHostFree(g->Sh);
     g->Sh = nullptr;
@@ -397,15 +403,13 @@ static void autotune_tiles()
     const int env = fuse_mode_env();
     const int m0 = is_minres() ? 0 : (env >= 0 ? env : 1), m1 = is_minres() ? 0 : (env >= 0 ? env : 2);
     Reduce rt = make_reduce(FIN_STORE, S_TMP0);
-    g->Sh[S_ALPHA] = 0.0;  // the deferred updates of the timed launches change nothing
+    g->Sh[S_ALPHA] = 0.0;  // the deferred z updates of the timed launches change nothing
     if (hipMemcpyAsync(g->S + S_ALPHA, g->Sh + S_ALPHA, sizeof(double), hipMemcpyHostToDevice, g->stream) != hipSuccess)
         return;
     auto tune = [&](OpDesc &op, auto *d, auto *q, auto *d2, int *best, int &best_mode) {
         using T = typename std::remove_pointer<decltype(d)>::type;
-        // x: the deferred iterate updates run with alpha = 0, so x is
-        // rewritten unchanged (it may hold an initial guess); z2, q2 scratch
+        // z2, q2: scratch
         const bool dbl = std::is_same<T, double>::value;
-        T *xs = dbl ? (T *)g->x64 : (T *)g->x32;
         T *z2 = dbl ? (T *)g->z2_64 : (T *)g->z2_32;
         T *q2 = dbl ? (T *)g->q2_64 : (T *)g->q2_32;
         const bool fz = d2 && z2 && q2 && apply_dir_ok(op);
@@ -416,9 +420,9 @@ static void autotune_tiles()
                 // with the deferred updates pending (alpha = 0): the launch
                 // streams q_old as in the loop; z is rewritten in place with
                 // the values it holds
-                launch_apply_cg1<T>(op, z2, q2, d, z2, d2, q, xs, 1, &rt, g->stream);
+                launch_apply_cg1<T>(op, z2, q2, d, z2, d2, q, 1, &rt, g->stream);
             } else if (fz && mode == 1) {
-                launch_apply_dir<T>(op, z2, d, d2, q, xs, 0, &rt, g->stream);
+                launch_apply_dir<T>(op, z2, d, d2, q, &rt, g->stream);
                 launch_kj_zupdate<T>(op, z2, q, &rt, g->stream);
             } else {
                 launch_apply<T>(op, d, q, nullptr, g->stream);
@@ -480,6 +484,43 @@ static void note_x0(double delta, double delta0)
     g->x0_used = !(fallback && fabs(delta) > fabs(delta0));
 }
 
+// Direction ring of the fused CG loops: d, s and up to ring_width() - 2 more
+// buffers, allocated on first use; if they do not fit, the ring is d and s
+// only (the iterate is then flushed every two iterations).
+static unsigned ring_width()
+{
+    const char *e = getenv("CHFEM_RING");
+    const int v = e ? atoi(e) : 16;
+    return (unsigned)std::min(std::max(v, 2), (int)RING_MAX);
+}
+
+template <typename T>
+static Ring<T> dir_ring(T *d, T *s)
+{
+    const bool dbl = std::is_same<T, double>::value;
+    int &w = dbl ? g->ring_w64 : g->ring_w32;
+    T **extra = dbl ? (T **)g->ring64 : (T **)g->ring32;
+    if (w == 0) {
+        const unsigned want = ring_width();
+        w = 2;
+        for (unsigned k = 2; k < want; k++) {
+            if (hipMalloc(&extra[k], nvars() * sizeof(T)) != hipSuccess) {
+                (void)hipGetLastError();
+                for (unsigned j = 2; j < k; j++) dfree(extra[j]);
+                w = 2;
+                break;
+            }
+            w = (int)k + 1;
+        }
+    }
+    Ring<T> r{};
+    r.w = w;
+    r.p[0] = d;
+    r.p[1] = s;
+    for (int k = 2; k < w; k++) r.p[k] = extra[k];
+    return r;
+}
+
 // Jacobi-PCG in precision T (pcg.cu:28-243).  r holds b on entry; returns 1 if converged.
 // With op.kinv (thermal/elastic NBN) the Jacobi diagonal comes from the keys
 // and z = M^-1 r is folded into the direction update (d = M^-1 r + beta d);
@@ -487,9 +528,9 @@ static void note_x0(double delta, double delta0)
 // With d2 and an operator that supports it (apply_dir_ok), the loop runs on
 // the preconditioned residual z = M^-1 r (held in r's buffer from the loop's
 // start) and the direction update runs inside the operator: iteration j
-// writes its direction to dbuf[j % 2] from z and dbuf[(j - 1) % 2], and
-// applies the previous iteration's x += alpha d there too; the update kernel
-// only updates z.  The last iteration's x update is applied after the loop.
+// writes its direction to ring slot j % w from z and slot (j - 1) % w; the
+// update kernel only updates z.  x += alpha_j d_j is applied once per w
+// iterations for the whole window (launch_ring_flush) and after the loop.
 template <typename T>
 static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool x0set, double tol, unsigned maxit,
                    unsigned &it, double &residual, double *delta0_out, T *d2)
@@ -498,7 +539,9 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
     hipStream_t st = g->stream;
     const bool kj = op.kinv != nullptr;
     const bool fused = kj && d2 && fuse_dir_enabled() && apply_dir_ok(op);
-    T *dbuf[2] = {d, d2};
+    const Ring<T> ring = fused ? dir_ring<T>(d, d2) : Ring<T>{};
+    const unsigned W = fused ? (unsigned)ring.w : 1u;
+    auto slot = [&](unsigned j) { return (T *)ring.p[j % W]; };
     // single-kernel iterations: iteration j reads z, q from buffer (j + 1) % 2
     // and writes buffer j % 2; the loop's first z is r's buffer
     T *z2 = std::is_same<T, double>::value ? (T *)g->z2_64 : (T *)g->z2_32;
@@ -554,21 +597,26 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
     if (!arm_guard(tol * tol * fabs(delta0))) return -1;
     it = 0;
     bool done = false;
+    unsigned queued = 0, flushed = 0;  // fused: iterations queued, and folded into x
     while (!done && it < maxit) {
         const unsigned nb = std::min(cg_batch(), maxit - it);
         for (unsigned b = 0; b < nb; b++) {
             const unsigned j = it + b;
-            if (cg1) {
-                timed_apply_cg1<T>(op, zb[(j + 1) & 1], qb[(j + 1) & 1], j > 0 ? dbuf[(j + 1) & 1] : nullptr, zb[j & 1],
-                                   dbuf[j & 1], qb[j & 1], x, j > 0 ? 1 : 0, &rc, b == 0);
-                continue;
-            }
             if (fused) {
-                T *dn = dbuf[j & 1];
-                const T *dold = j > 0 ? dbuf[(j - 1) & 1] : nullptr;
-                if (b == 0) timed_apply_dir<T>(op, r, dold, dn, q, x, 1, &ra);
-                else launch_apply_dir<T>(op, r, dold, dn, q, x, 1, &ra, st);
-                launch_kj_zupdate<T>(op, r, q, &rdl, st);
+                if (cg1) {
+                    timed_apply_cg1<T>(op, zb[(j + 1) & 1], qb[(j + 1) & 1], j > 0 ? slot(j - 1) : nullptr, zb[j & 1],
+                                       slot(j), qb[j & 1], j > 0 ? 1 : 0, &rc, b == 0);
+                } else {
+                    const T *dold = j > 0 ? slot(j - 1) : nullptr;
+                    if (b == 0) timed_apply_dir<T>(op, r, dold, slot(j), q, &ra);
+                    else launch_apply_dir<T>(op, r, dold, slot(j), q, &ra, st);
+                    launch_kj_zupdate<T>(op, r, q, &rdl, st);
+                }
+                queued = j + 1;
+                if (queued % W == 0) {
+                    launch_ring_flush<T>(x, ring, flushed, queued, 0, g->S, n, st);
+                    flushed = queued;
+                }
                 continue;
             }
             if (j > 0) {
@@ -584,8 +632,8 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
         it = (unsigned)g->Sh[S_ITER];
         done = g->Sh[S_DONE] != 0.0;
     }
-    // the last executed iteration's deferred x += alpha d
-    if (fused && it > 0) launch_axpy_alpha<T>(x, dbuf[(it - 1) & 1], g->S, n, st);
+    // the iterations since the last window flush
+    if (fused) launch_ring_flush<T>(x, ring, flushed, queued, 0, g->S, n, st);
     delta = g->Sh[S_DELTA];
     residual = sqrt(fabs(delta) / fabs(delta0));
     return !above(delta, delta0, tol);
@@ -711,10 +759,13 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
     const float *dinv = g->dinv32;
     // direction update inside the FP32 operator, on the preconditioned
     // residual z = M^-1 r (kept in r's buffer); iteration j (counted over the
-    // whole solve, the direction is kept across replacements) writes
-    // dbuf[j % 2]
+    // whole solve, the direction is kept across replacements) writes ring
+    // slot j % w, and x += alpha_j d_j is applied per window of w iterations
+    // and at each segment end
     const bool fused = kj && g->s32 && fuse_dir_enabled() && apply_dir_ok(o32);
-    float *dbuf[2] = {d, g->s32};
+    const Ring<float> ring = fused ? dir_ring<float>(d, g->s32) : Ring<float>{};
+    const unsigned W = fused ? (unsigned)ring.w : 1u;
+    auto slot = [&](unsigned j) { return (float *)ring.p[j % W]; };
     Reduce rz = make_reduce(FIN_STORE, S_TMP0);
     // single-kernel iterations: iteration j reads z, q from buffer (j + 1) % 2
     // and writes buffer j % 2; a segment's fresh z goes to the buffer its
@@ -767,6 +818,7 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
     if (!arm_guard(thresh())) return -1;
     bool converged = false;
     unsigned base = 0;  // iterations before the current guarded segment
+    unsigned queued = 0, flushed = 0;  // fused: iterations queued, and folded into x
     while (true) {
         unsigned seg = 0;
         bool done = false;
@@ -774,21 +826,21 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
             const unsigned nb = std::min(cg_batch(), maxit - base - seg);
             for (unsigned b = 0; b < nb; b++) {
                 const unsigned j = base + seg + b;
-                if (cg1) {
-                    timed_apply_cg1<float>(o32, zb[(j + 1) & 1], qb[(j + 1) & 1], j > 0 ? dbuf[(j + 1) & 1] : nullptr,
-                                           zb[j & 1], dbuf[j & 1], qb[j & 1], x, j > base ? 1 : 0, &rc, b == 0);
-                    continue;
-                }
                 if (fused) {
-                    // x += alpha d of iteration j - 1 inside the operator, when
-                    // that iteration belongs to this segment (x was flushed
-                    // and zeroed at the segment start)
-                    float *dn = dbuf[j & 1];
-                    const float *dold = j > 0 ? dbuf[(j - 1) & 1] : nullptr;
-                    const int xpend = j > base ? 1 : 0;
-                    if (b == 0) timed_apply_dir<float>(o32, r, dold, dn, q, x, xpend, &ra);
-                    else launch_apply_dir<float>(o32, r, dold, dn, q, x, xpend, &ra, st);
-                    launch_kj_zupdate<float>(o32, r, q, &rdl, st);
+                    if (cg1) {
+                        timed_apply_cg1<float>(o32, zb[(j + 1) & 1], qb[(j + 1) & 1], j > 0 ? slot(j - 1) : nullptr,
+                                               zb[j & 1], slot(j), qb[j & 1], j > base ? 1 : 0, &rc, b == 0);
+                    } else {
+                        const float *dold = j > 0 ? slot(j - 1) : nullptr;
+                        if (b == 0) timed_apply_dir<float>(o32, r, dold, slot(j), q, &ra);
+                        else launch_apply_dir<float>(o32, r, dold, slot(j), q, &ra, st);
+                        launch_kj_zupdate<float>(o32, r, q, &rdl, st);
+                    }
+                    queued = j + 1;
+                    if (queued % W == 0) {
+                        launch_ring_flush<float>(x, ring, flushed, queued, base, g->S, n, st);
+                        flushed = queued;
+                    }
                     continue;
                 }
                 if (j > 0) {
@@ -806,8 +858,12 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
         }
         it_total = base + seg;
         const bool last = it_total >= maxit;
-        // the segment's last deferred x += alpha d
-        if (fused && seg > 0) launch_axpy_alpha<float>(x, dbuf[(it_total - 1) & 1], g->S, n, st);
+        // the segment's iterations since the last window flush; the next
+        // segment continues the ring from slot it_total % w
+        if (fused) {
+            launch_ring_flush<float>(x, ring, flushed, queued, base, g->S, n, st);
+            flushed = queued = it_total;
+        }
         // reliable update: x64 += x ; r64 -= A64 x ; x = 0 ; r = (float) r64
         launch_add_correction(g->x64, x, n, st);
         launch_convert<float, double>(x, g->d64, n, st);
@@ -1501,28 +1557,26 @@ double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps, cudapcgF
     auto once = [&](unsigned k) {
         if (cg1) {
             // z, q and d alternate between two buffers each (x / z2, q / q2,
-            // d / s); the deferred updates run with S[ALPHA] = 0 on the
-            // stored Jacobi diagonal, which they rewrite unchanged
+            // d / s); the deferred z update runs with S[ALPHA] = 0
             if (precision == CHFEM_PREC_MIXED) {
                 float *zb[2] = {g->x32, g->z2_32}, *qb[2] = {g->q32, g->q2_32}, *db[2] = {g->d32, g->s32};
                 launch_apply_cg1<float>(g->dm.op32, zb[k & 1], qb[k & 1], db[k & 1], zb[(k + 1) & 1], db[(k + 1) & 1],
-                                        qb[(k + 1) & 1], g->dinv32, 1, &red3, g->stream);
+                                        qb[(k + 1) & 1], 1, &red3, g->stream);
             } else {
                 double *zb[2] = {g->x64, g->z2_64}, *qb[2] = {g->q64, g->q2_64}, *db[2] = {g->d64, g->s64};
                 launch_apply_cg1<double>(g->dm.op64, zb[k & 1], qb[k & 1], db[k & 1], zb[(k + 1) & 1],
-                                         db[(k + 1) & 1], qb[(k + 1) & 1], g->dinv64, 1, &red3, g->stream);
+                                         db[(k + 1) & 1], qb[(k + 1) & 1], 1, &red3, g->stream);
             }
             return;
         }
         if (dir) {
-            // r = x; the deferred iterate update runs on the stored Jacobi
-            // diagonal with S[ALPHA] = 0, which rewrites it unchanged
+            // z = x
             if (precision == CHFEM_PREC_MIXED)
                 launch_apply_dir<float>(g->dm.op32, g->x32, k & 1 ? g->s32 : g->d32, k & 1 ? g->d32 : g->s32, g->q32,
-                                        g->dinv32, 1, &red, g->stream);
+                                        &red, g->stream);
             else
                 launch_apply_dir<double>(g->dm.op64, g->x64, k & 1 ? g->s64 : g->d64, k & 1 ? g->d64 : g->s64, g->q64,
-                                         g->dinv64, 1, &red, g->stream);
+                                         &red, g->stream);
             return;
         }
         if (precision == CHFEM_PREC_MIXED) launch_apply<float>(g->dm.op32, k & 1 ? g->x32 : g->d32, g->q32, rp, g->stream);

@@ -20,7 +20,10 @@ enum ScalarSlot {
     S_TMP2, S_TMP3, S_TMP4, S_TMP5, S_TMP6, S_TMP7,
     // guarded Krylov loops: done flag, iterations counted on the device,
     // stop threshold on r.z
-    S_DONE = 16, S_ITER, S_THRESH, S_COUNT = 24
+    S_DONE = 16, S_ITER, S_THRESH,
+    // alpha of guarded iteration j (j counted in S[S_ITER]) at
+    // S[S_AHIST + j % S_AHIST_LEN], for the ring flush of the iterate
+    S_AHIST = 24, S_AHIST_LEN = 32, S_COUNT = S_AHIST + S_AHIST_LEN
 };
 
 // How the finishing kernel of a fused reduction publishes its total.
@@ -193,23 +196,32 @@ void launch_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduc
 // direction update inside the operator (Walsh-basis thermal or isotropic
 // elastic NBN with key-Jacobi, see apply_dir_ok): d = z + beta d_old (beta =
 // S[DELTA]/S[DELTA_OLD]; dold == nullptr: d = z), dnew <- d, q = A d,
-// red <- d.q, and the previous iteration's deferred iterate update
-// x += S[ALPHA] d_old when xpend (and dold) is set (x is rewritten
-// unchanged otherwise).  dnew must not alias dold, z, q or x.  The loop's
-// other kernels: launch_kj_to_z (r -> z), launch_kj_zupdate.
+// red <- d.q.  dnew must not alias dold, z or q.  The loop's other kernels:
+// launch_kj_to_z (r -> z), launch_kj_zupdate, and launch_ring_flush for the
+// iterate.
 bool apply_dir_ok(const OpDesc &op);
 template <typename T>
-void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q, T *x, int xpend,
-                      const Reduce *red, hipStream_t s);
+void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q, const Reduce *red, hipStream_t s);
 // Single-kernel Jacobi-PCG iteration (same operators as launch_apply_dir):
 // z = zold - S[ALPHA] qold (qold holds dinv q of the previous iteration;
 // skipped unless pend), d = z + beta dold (dold == nullptr: d = z), znew <- z,
-// dnew <- d, q = A d with qnew <- dinv q, deferred x += S[ALPHA] dold (pend),
-// red (FIN_CG1) <- d.q, z.q, q.dinv q, z.diag z.  The new buffers must not
-// alias the old ones, x or each other.
+// dnew <- d, q = A d with qnew <- dinv q, red (FIN_CG1) <- d.q, z.q,
+// q.dinv q, z.diag z.  The new buffers must not alias the old ones or each
+// other.
 template <typename T>
-void launch_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *dold, T *znew, T *dnew, T *qnew, T *x,
+void launch_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *dold, T *znew, T *dnew, T *qnew,
                       int pend, const Reduce *red, hipStream_t s);
-// x += S[ALPHA] d (the last deferred iterate update of a loop)
-template <typename T> void launch_axpy_alpha(T *x, const T *d, const double *S, size_t n, hipStream_t s);
+// The iterate of the fused CG loops: iteration j writes its direction to
+// ring slot j % w; x += sum of S[S_AHIST + (j - base) % S_AHIST_LEN] *
+// ring[j % w] over the iterations j in [j0, j1) that ran (j - base <
+// S[S_ITER], the loop's guarded count since base).  w <= S_AHIST_LEN.
+constexpr int RING_MAX = 32;
+template <typename T>
+struct Ring {
+    const T *p[RING_MAX];
+    int w;
+};
+template <typename T>
+void launch_ring_flush(T *x, const Ring<T> &ring, unsigned j0, unsigned j1, unsigned base, const double *S, size_t n,
+                       hipStream_t s);
 }  // namespace chfem

@@ -752,15 +752,6 @@ __global__ void __launch_bounds__(BS) k_kj_to_z(T *r, const uint16_t *__restrict
     store_partials<1>(red, vv);
 }
 
-// x += S[ALPHA] d
-template <typename T>
-__global__ void __launch_bounds__(BS) k_axpy_alpha(T *__restrict__ x, const T *__restrict__ d,
-                                                   const double *__restrict__ S, size_t n)
-{
-    const T a = (T)S[S_ALPHA];
-    for (size_t i = (size_t)blockIdx.x * BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) x[i] += a * d[i];
-}
-
 // r -= q ; z = dinv r ; reduce r.z
 template <typename T, int ND>
 __global__ void __launch_bounds__(BS) k_kj_sub_precond(T *__restrict__ r, const T *__restrict__ q, T *__restrict__ z,
@@ -1165,6 +1156,7 @@ __global__ void __launch_bounds__(1024) k_reduce_finish(Reduce red, unsigned nb)
         if (red.fin == FIN_ALPHA) {
             S[S_DQ] = s[0];
             S[S_ALPHA] = S[S_DELTA] / s[0];
+            if (red.guard) S[S_AHIST + (unsigned)S[S_ITER] % S_AHIST_LEN] = S[S_ALPHA];
         } else if (red.fin == FIN_DELTA) {
             S[S_DELTA_OLD] = S[S_DELTA];
             S[S_DELTA] = s[0];
@@ -1183,6 +1175,7 @@ __global__ void __launch_bounds__(1024) k_reduce_finish(Reduce red, unsigned nb)
             S[S_DELTA_OLD] = delta;
             S[S_DELTA] = dnew;
             if (red.guard) {
+                S[S_AHIST + (unsigned)S[S_ITER] % S_AHIST_LEN] = alpha;
                 S[S_ITER] += 1.0;
                 if (fabs(dnew) <= S[S_THRESH]) S[S_DONE] = 1.0;
             }
@@ -1361,10 +1354,41 @@ void launch_kj_to_z(const OpDesc &op, T *r, const Reduce *red, hipStream_t s)
     launch_reduce_finish(*red, gr, 1, s);
 }
 
+// x += sum_j alpha_j ring[j % w] over the window's iterations that ran; the
+// coefficients are read once per block into LDS
 template <typename T>
-void launch_axpy_alpha(T *x, const T *d, const double *S, size_t n, hipStream_t s)
+__global__ void __launch_bounds__(BS) k_ring_flush(T *x, Ring<T> ring, unsigned j0, unsigned j1, unsigned base,
+                                                  const double *__restrict__ S, size_t n)
 {
-    k_axpy_alpha<T><<<stream_grid(n, sizeof(T)), BS, 0, s>>>(x, d, S, n);
+    __shared__ T coef[RING_MAX];
+    __shared__ const T *src[RING_MAX];
+    __shared__ int cnt;
+    if (threadIdx.x == 0) {
+        const unsigned ran = base + (unsigned)S[S_ITER];
+        const unsigned end = j1 < ran ? j1 : ran;
+        int c = 0;
+        for (unsigned j = j0; j < end && c < RING_MAX; j++, c++) {
+            coef[c] = (T)S[S_AHIST + (j - base) % S_AHIST_LEN];
+            src[c] = ring.p[j % (unsigned)ring.w];
+        }
+        cnt = c;
+    }
+    __syncthreads();
+    const int c = cnt;
+    if (c == 0) return;
+    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) {
+        T acc = x[i];
+        for (int k = 0; k < c; k++) acc += coef[k] * src[k][i];
+        x[i] = acc;
+    }
+}
+
+template <typename T>
+void launch_ring_flush(T *x, const Ring<T> &ring, unsigned j0, unsigned j1, unsigned base, const double *S, size_t n,
+                       hipStream_t s)
+{
+    if (j1 <= j0) return;
+    k_ring_flush<T><<<stream_grid(n, sizeof(T)), BS, 0, s>>>(x, ring, j0, j1, base, S, n);
 }
 
 template <typename T>
@@ -1466,7 +1490,8 @@ void launch_interpl2(const double *coarse, double *fine, int rows, int cols, int
     template void launch_kj_pcg_update<T>(const OpDesc &, T *, T *, const T *, const T *, const Reduce *,     \
                                           hipStream_t);                                                       \
     template void launch_kj_sub_precond<T>(const OpDesc &, T *, const T *, T *, const Reduce *, hipStream_t); \
-    template void launch_axpy_alpha<T>(T *, const T *, const double *, size_t, hipStream_t);                 \
+    template void launch_ring_flush<T>(T *, const Ring<T> &, unsigned, unsigned, unsigned, const double *, size_t, \
+                                       hipStream_t);                                                                \
     template void launch_kj_zupdate<T>(const OpDesc &, T *, const T *, const Reduce *, hipStream_t);          \
     template void launch_kj_to_z<T>(const OpDesc &, T *, const Reduce *, hipStream_t);
 INST(float)

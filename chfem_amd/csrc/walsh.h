@@ -211,24 +211,24 @@ __device__ __forceinline__ void pin_here(int v) { asm volatile("" ::"v"(v)); }
 // packed keys in its update kernel), so the operator's input
 // d = z + beta d_old is formed from two streamed vectors while the node
 // planes are fetched; q = A d is computed from it, and d is written to dnew
-// for the nodes the block owns.  The previous iteration's iterate update is
-// deferred to here too: x += alpha d_old on the owned nodes (alpha = 0 leaves
-// x as it is).  v is d_old; first: d = z (v unused).
+// for the nodes the block owns.  The iterate is not touched: the loop keeps
+// its directions in a ring of buffers and adds sum alpha_j d_j to x once per
+// window (launch_ring_flush).  v is d_old; first: d = z (v unused).
 template <typename T>
 struct DirFuse {
     const T *z;
     T *dnew;
-    T *x;
     T beta;
     T alpha;
     int first;
-    int xpend;  // apply the deferred x update with alpha = S[ALPHA]
     // single-kernel CG (FM 2): the z update of the previous iteration is
-    // deferred here too, z = z_old - alpha qt_old, written to znew on the
-    // owned nodes; this launch's q output is qt = dinv q (dinv from the
-    // packed keys of the owned nodes, kinv[key * ND + a]; the diagonal
-    // itself at kinv[65536 * ND + key * ND + a]), the only form of q the
-    // loop reads.  With xpend clear the z update is skipped (qt_old reads z)
+    // deferred here, z = z_old - alpha qt_old (alpha = S[ALPHA]), written to
+    // znew on the owned nodes; this launch's q output is qt = dinv q (dinv
+    // from the packed keys of the owned nodes, kinv[key * ND + a]; the
+    // diagonal itself at kinv[65536 * ND + key * ND + a]), the only form of
+    // q the loop reads.  With zpend clear the z update is skipped (qt_old
+    // reads z)
+    int zpend;
     const T *qold;
     T *znew;
     const T *kinv;
@@ -277,8 +277,6 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     // branch around a load makes the compiler wait for all loads in flight
     // at its join)
     __shared__ T su[2][SLOTS * NT];
-    // FUSE: the fetched d_old planes, for the owned nodes' deferred x update
-    __shared__ T sdo[FUSE ? 2 : 1][FUSE ? SLOTS * NT : 1];
     // CG1: the updated z planes, for the owned nodes' z output and z.q
     __shared__ T szp[CG1 ? 2 : 1][CG1 ? SLOTS * NT : 1];
     __shared__ T sb[NT * SB];
@@ -316,9 +314,8 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     auto next_layer = [&](int lw) { return lw + 1 == nz ? 0 : lw + 1; };
     T pre[SLOTS];
     // FUSE: the direction update's operands (z and d_old) are fetched with
-    // the plane and combined when the plane is stored to LDS; d_old is kept
-    // in LDS as well, for the owned nodes' x update.  On the first iteration
-    // (no d_old) the d_old operand reads z and beta is 0.
+    // the plane and combined when the plane is stored to LDS.  On the first
+    // iteration (no d_old) the d_old operand reads z and beta is 0.
     T pz[FUSE ? SLOTS : 1];
     const T *dsrc = v;
     T beta = T(0);
@@ -328,7 +325,7 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     }
     // CG1: qt_old (= dinv q_old) of every plane slot
     T pq[CG1 ? SLOTS : 1];
-    const T *qsrc = CG1 && fz.xpend ? fz.qold : fz.z;
+    const T *qsrc = CG1 && fz.zpend ? fz.qold : fz.z;
     // fetch(lw): the plane of wrapped layer lw (uniform base pointer, 32-bit
     // per-slot offsets)
     auto fetch = [&](int lw) {
@@ -341,17 +338,15 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
             pre[k] = bd[goff[k]];
         }
     };
-    auto store = [&](T *dst, T *ddst, T *zdst) {
+    auto store = [&](T *dst, T *zdst) {
 #pragma unroll
         for (int k = 0; k < SLOTS; k++) {
             if constexpr (CG1) {
                 const T zk = pz[k] - fz.alpha * pq[k];
                 dst[tid + k * NT] = zk + beta * pre[k];
-                ddst[tid + k * NT] = pre[k];
                 zdst[tid + k * NT] = zk;
             } else if constexpr (FUSE) {
                 dst[tid + k * NT] = pz[k] + beta * pre[k];
-                ddst[tid + k * NT] = pre[k];
             } else {
                 dst[tid + k * NT] = pre[k];
             }
@@ -419,15 +414,14 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     const T *bleft = sb + ((tc >= 1 ? tc - 1 : 0) * TR + tr) * SB;
 
     T *cur = su[0], *nxt = su[1];
-    T *dcur = sdo[0], *dnxt = sdo[FUSE ? 1 : 0];  // d_old planes, swapped with cur / nxt
     T *zcur = szp[0], *znxt = szp[CG1 ? 1 : 0];   // z planes (CG1), swapped with cur / nxt
     const int lwm = wrapl(l0 - 1), lw0 = wrapl(l0);
     int lw1 = next_layer(lw0), lw2 = next_layer(lw1);  // wrapped l + 1, l + 2 of the step
     fetch(lwm);
     uint16_t mk = mat_key(lwm);
-    store(cur, dcur, zcur);
+    store(cur, zcur);
     fetch(lw0);
-    store(nxt, dnxt, znxt);
+    store(nxt, znxt);
     int m = material(mk);
     mk = mat_key(lw0);
     __syncthreads();
@@ -438,12 +432,11 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     T acc[ND];
 #pragma unroll
     for (int a = 0; a < ND; a++) acc[a] = own ? Af[a] + bleft[ND + a] : T(0);
-    store(cur, dcur, zcur);  // layer l0 + 1 replaces layer l0 - 1
+    store(cur, zcur);  // layer l0 + 1 replaces layer l0 - 1
     m = material(mk);
     uint32_t kraw = mk;  // packed key of the owned node at the current layer (CG1 dinv)
     {
         T *t = cur; cur = nxt; nxt = t;  // cur = layer l0, nxt = layer l0 + 1
-        t = dcur; dcur = dnxt; dnxt = t;
         t = zcur; zcur = znxt; znxt = t;
     }
     __syncthreads();
@@ -451,15 +444,6 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     // this work-item's node offset in a layer (work-items that own no node
     // use node 0 for their unconditional loads; only owners store)
     const size_t own_off = own ? (size_t)(c0 + tc - 1) * ny + (r0 + tr - 1) : 0;
-    // FUSE: x of the owned node, loaded one layer ahead for the deferred
-    // iterate update, which is stored at the top of its layer's step (its
-    // store then completes during the step instead of holding up the next)
-    T pxn[FUSE ? ND : 1];
-    if constexpr (FUSE) {
-        const size_t o0 = ND * ((size_t)l0 * nxy + own_off);
-#pragma unroll
-        for (int a = 0; a < ND; a++) pxn[a] = fz.x[o0 + a];
-    }
     for (int s = 0; s < vl; s++) {
         const int l = l0 + s;
         const size_t on = ND * ((size_t)l * nxy + own_off);
@@ -496,18 +480,6 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
                 for (int a = 0; a < ND; a++) own_z[a] = zz[a];
             }
         }
-        if constexpr (FUSE) {
-            // deferred x += alpha d_old of layer l (d_old from the LDS plane)
-            if (own) {
-                const T *dd = dcur + (tc * NR + tr) * ND;
-                T *xo = fz.x + on;
-#pragma unroll
-                for (int a = 0; a < ND; a++) xo[a] = pxn[a] + fz.alpha * dd[a];
-            }
-            const size_t o1 = ND * ((size_t)lw1 * nxy + own_off);
-#pragma unroll
-            for (int a = 0; a < ND; a++) pxn[a] = fz.x[o1 + a];
-        }
         fetch(lw2);
         lw1 = lw2;
         lw2 = next_layer(lw2);
@@ -515,7 +487,7 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
         __syncthreads();
         // the prefetched plane goes to LDS before this layer's node results
         // are stored, so waiting for the loads does not wait for the stores
-        store(cur, dcur, zcur);  // layer l + 2 replaces layer l
+        store(cur, zcur);  // layer l + 2 replaces layer l
         // first use of the key loaded at the top of the step: without this
         // the compiler consumes it (and reuses its register) right after the
         // load, exposing the load's latency at the top of every step
@@ -560,8 +532,7 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
         }
         {
             T *t = cur; cur = nxt; nxt = t;
-            t = dcur; dcur = dnxt; dnxt = t;
-            t = zcur; zcur = znxt; znxt = t;
+                t = zcur; zcur = znxt; znxt = t;
         }
         __syncthreads();
     }

@@ -14,9 +14,9 @@ __global__ void __launch_bounds__(TR *TC) k_walsh(const T *__restrict__ v, T *__
     if (DOT && stopped(red)) return;
     // beta of the fused direction update, as k_kj_pcg_dir forms it
     if (FUSE && !fz.first) fz.beta = (T)(red.S[S_DELTA] / red.S[S_DELTA_OLD]);
-    // alpha of the previous iteration for the deferred x update, as
-    // k_kj_pcg_update forms it
-    if (FUSE) fz.alpha = fz.xpend ? (T)red.S[S_ALPHA] : T(0);
+    // single-kernel CG: alpha of the previous iteration for the deferred z
+    // update
+    if (FM == 2) fz.alpha = fz.zpend ? (T)red.S[S_ALPHA] : T(0);
     // 1-D grid; blocks are dispatched to the 8 XCDs round-robin (block b to
     // XCD b % 8), so each XCD is given one contiguous range of tiles (rows of
     // tiles fastest): tiles that share halo nodes then run on the same XCD at
@@ -41,7 +41,7 @@ static void launch_walsh_nd(const OpDesc &op, const T *lamu, const T *v, T *q, c
 {
     Reduce rr = {nullptr, nullptr, 0, 0, 0};
     if (red) rr = *red;
-    DirFuse<T> f = {nullptr, nullptr, nullptr, T(0), T(0), 1, 0, nullptr, nullptr, nullptr};
+    DirFuse<T> f = {nullptr, nullptr, T(0), T(0), 1, 0, nullptr, nullptr, nullptr};
     if (fz) f = *fz;
     // single-kernel CG form: q_old, z_new and the dinv table given
     const bool cg1 = fz && fz->znew;
@@ -94,33 +94,32 @@ bool apply_dir_ok(const OpDesc &op)
 }
 
 template <typename T>
-void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q, T *x, int xpend,
-                      const Reduce *red, hipStream_t s)
+void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q, const Reduce *red, hipStream_t s)
 {
-    const DirFuse<T> f = {z, dnew, x, T(0), T(0), dold ? 0 : 1, dold ? xpend : 0, nullptr, nullptr, nullptr};
+    const DirFuse<T> f = {z, dnew, T(0), T(0), dold ? 0 : 1, 0, nullptr, nullptr, nullptr};
     const WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
     launch_walsh<T>(op, (const T *)op.lamu, dold, q, red, t, s, &f);
 }
 
 template <typename T>
-void launch_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *dold, T *znew, T *dnew, T *qnew, T *x,
+void launch_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *dold, T *znew, T *dnew, T *qnew,
                       int pend, const Reduce *red, hipStream_t s)
 {
     const int first = dold ? 0 : 1, p = dold && pend ? 1 : 0;
-    const DirFuse<T> f = {zold, dnew, x, T(0), T(0), first, p, p ? qold : zold, znew, (const T *)op.kinv};
+    const DirFuse<T> f = {zold, dnew, T(0), T(0), first, p, p ? qold : zold, znew, (const T *)op.kinv};
     const WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
     launch_walsh<T>(op, (const T *)op.lamu, dold, qnew, red, t, s, &f);
 }
 
 template void launch_apply_cg1<float>(const OpDesc &, const float *, const float *, const float *, float *, float *,
-                                      float *, float *, int, const Reduce *, hipStream_t);
+                                      float *, int, const Reduce *, hipStream_t);
 template void launch_apply_cg1<double>(const OpDesc &, const double *, const double *, const double *, double *,
-                                       double *, double *, double *, int, const Reduce *, hipStream_t);
+                                       double *, double *, int, const Reduce *, hipStream_t);
 
-template void launch_apply_dir<float>(const OpDesc &, const float *, const float *, float *, float *, float *, int,
-                                      const Reduce *, hipStream_t);
-template void launch_apply_dir<double>(const OpDesc &, const double *, const double *, double *, double *, double *,
-                                       int, const Reduce *, hipStream_t);
+template void launch_apply_dir<float>(const OpDesc &, const float *, const float *, float *, float *, const Reduce *,
+                                      hipStream_t);
+template void launch_apply_dir<double>(const OpDesc &, const double *, const double *, double *, double *,
+                                       const Reduce *, hipStream_t);
 template void launch_walsh<float>(const OpDesc &, const float *, const float *, float *, const Reduce *, WalshTiles,
                                   hipStream_t, const DirFuse<float> *);
 template void launch_walsh<double>(const OpDesc &, const double *, const double *, double *, const Reduce *,

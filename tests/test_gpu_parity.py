@@ -375,3 +375,33 @@ def test_fused_direction_update(fuse, precision, which, monkeypatch):
         assert st1.iterations < st.iterations + 2
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("fuse", ["2", "1"])
+@pytest.mark.parametrize("ring", ["2", "5", "16"])
+def test_direction_ring_widths(ring, fuse, precision, monkeypatch):
+    """The fused CG loops keep their directions in a ring of CHFEM_RING
+    buffers and add sum alpha_j d_j to the iterate once per ring window and
+    at every loop or FP64-replacement end.  Windows that end inside a batch
+    of queued iterations, at a replacement, or after the loop has converged
+    (queued iterations that did not run) all give the oracle's solution
+    (widths 2 and 5 end windows inside the 16-iteration batches)."""
+    monkeypatch.setenv("CHFEM_FUSE_DIR", fuse)
+    monkeypatch.setenv("CHFEM_RING", ring)
+    monkeypatch.setenv("CHFEM_WALSH_TILE", "16x16x4")
+    img = three_phase((21, 34, 19), 53)
+    m = H.model(H.ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25], colors=(0, 100, 255))
+    xr, okr, itr, _ = m.solve(1, tol=1e-10)
+    assert okr
+    Cr = m.update_c(1, xr)
+    s = H.GpuSolver(m, precision=precision, tol=1e-10 if precision == 0 else 1e-9)
+    try:
+        x, ok, st = s.solve(m.rhs(1))
+        assert ok
+        assert rel(m.update_c(1, x), Cr) < 1e-7
+        if precision == 0:
+            assert rel(x, xr) < 1e-7
+            assert abs(int(st.iterations) - itr) <= 2
+    finally:
+        s.close()
