@@ -40,7 +40,10 @@ enum {
     SB_FRESH,   // next iteration is the first of a segment: direction = dinv r
     SB_REPL,    // residual replacement selected by the host
     SB_START,   // fresh FP32 start selected by the host
-    SB_STRIDE = 16
+    // alpha of each step of the current ring window (fused loop), zeroed
+    // after every window flush
+    SB_AHIST = 16, SB_RING_MAX = 16,
+    SB_STRIDE = 32
 };
 
 enum { OP_ITER = 0, OP_REPL = 1 };
@@ -65,6 +68,23 @@ void start(const Batch &b, const double *r64, float *r32, float *d32, float *x32
 void delta0(const Batch &b, const double *r64, hipStream_t s);
 void flush(const Batch &b, double *x64, double *e64, float *x32, hipStream_t s);
 void resid(const Batch &b, double *r64, const double *q64, float *r32, hipStream_t s);
+
+// Fused loop on z = M^-1 r (FP32): directions in a ring of w <= SB_RING_MAX
+// slots of `total` floats each, indexed by each system's own iteration count
+// j (SB_ITER).  A step: d = z + beta d_old (d = z on a system's fresh step;
+// d_old in slot (j - 1) % w) into slot j % w, q = A d and partial d.q
+// (apply_dir); alpha into SB_AHIST + j % w (finish_alpha); z -= alpha M^-1 q
+// and partial z.(D z) (zupdate).  Every w steps ring_flush adds
+// sum alpha_k d_k to x32 and zeroes the alphas (slots a system did not fill
+// in the window have alpha 0).
+void apply_dir(const Batch &b, const float *z, float *ring, size_t total, int w, float *q, hipStream_t s);
+void finish_alpha(const Batch &b, int w, hipStream_t s);
+void zupdate(const Batch &b, float *z, const float *q, hipStream_t s);
+void ring_flush(const Batch &b, float *x32, const float *ring, size_t total, int w, hipStream_t s);
+// z-form start and replacement: as start / resid with z32 = M^-1 r in place
+// of r32 (and no d32)
+void start_z(const Batch &b, const double *r64, float *z32, float *x32, hipStream_t s);
+void resid_z(const Batch &b, double *r64, const double *q64, float *z32, hipStream_t s);
 
 }  // namespace batch
 }  // namespace chfem

@@ -61,11 +61,40 @@ __global__ void __launch_bounds__(TR *TC) k_apply_walsh(const Sys *__restrict__ 
     if (threadIdx.x == 0) part[ti] = s;
 }
 
+// Fused step of the z-form loop: d = z + beta d_old (or d = z on a fresh
+// step) formed while the planes are fetched, q = A d and the partial d.q of
+// each tile.  A system's iteration j (its own count, SB_ITER) writes ring
+// slot j % w and reads slot (j - 1) % w: the direction carries over the
+// steps a system sits out and its residual replacements.
+template <int ND, int TR, int TC>
+__global__ void __launch_bounds__(TR *TC) k_apply_dir(const Sys *__restrict__ sys, const Tile *__restrict__ tiles,
+                                                       const float *z, float *ring, size_t total, int w, float *q,
+                                                       const double *__restrict__ Sb, double *__restrict__ part)
+{
+    const unsigned b = blockIdx.x, nb = gridDim.x, per = nb / 8u, rem = nb % 8u, xcd = b % 8u, k = b / 8u;
+    const unsigned ti = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + k;
+    const Tile t = tiles[ti];
+    if (!selected<float>(Sb, t.sys, OP_ITER)) return;
+    const Sys &S = sys[t.sys];
+    const double *R = Sb + t.sys * SB_STRIDE;
+    const unsigned j = (unsigned)R[SB_ITER], uw = (unsigned)w;
+    const float *dold = ring + (size_t)((j + uw - 1) % uw) * total;
+    DirFuse<float> fz{};
+    fz.z = z + S.off;
+    fz.dnew = ring + (size_t)(j % uw) * total + S.off;
+    fz.first = R[SB_FRESH] != 0.0 ? 1 : 0;
+    fz.beta = fz.first ? 0.f : (float)(R[SB_DELTA] / R[SB_DELTA_OLD]);
+    const double d = walsh_tile<float, ND, TR, TC, true, 1>(dold + S.off, q + S.off, S.keys, (const float *)S.mat32, 0,
+                                                            S.ny, S.nx, S.nz, S.LZ, t.bx, t.by, t.bz, fz).dq;
+    const double s = bsum(d);
+    if (threadIdx.x == 0) part[ti] = s;
+}
+
 // ---------------------------------------------------------------------------
 // finishing kernels: one block per system sums its partial range
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(BB) k_finish(const Sys *__restrict__ sys, double *__restrict__ Sb,
-                                               const double *__restrict__ part, int what)
+                                               const double *__restrict__ part, int what, int w)
 {
     const int s = blockIdx.x;
     double *R = Sb + s * SB_STRIDE;
@@ -85,6 +114,7 @@ __global__ void __launch_bounds__(BB) k_finish(const Sys *__restrict__ sys, doub
     if (what == FIN_B_ALPHA) {
         R[SB_DQ] = tot;
         R[SB_ALPHA] = R[SB_DELTA] / tot;
+        if (w > 0) R[SB_AHIST + (unsigned)R[SB_ITER] % (unsigned)w] = R[SB_ALPHA];
     } else if (what == FIN_B_DELTA) {
         R[SB_DELTA_OLD] = R[SB_DELTA];
         R[SB_DELTA] = tot;
@@ -154,6 +184,92 @@ __global__ void __launch_bounds__(BB) k_update(const Sys *__restrict__ sys, cons
             const T rn = r[o + a] - alpha * q[o + a];
             r[o + a] = rn;
             acc += (double)rn * (double)(dv[a] * rn);
+        }
+    }
+    const double s = bsum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// z -= alpha dinv q ; partial z.(diag z) (= r.z)
+template <int ND>
+__global__ void __launch_bounds__(BB) k_zupdate(const Sys *__restrict__ sys, const Chunk *__restrict__ chunks,
+                                                const double *__restrict__ Sb, float *z, const float *__restrict__ q,
+                                                double *__restrict__ part)
+{
+    const Chunk ch = chunks[blockIdx.x];
+    const double *R = Sb + ch.sys * SB_STRIDE;
+    if (R[SB_ACTIVE] == 0.0 || R[SB_DONE] != 0.0) return;
+    const Sys &S = sys[ch.sys];
+    const uint32_t i = ch.i0 + threadIdx.x;
+    double acc = 0;
+    if (i < ch.i1) {
+        const float alpha = (float)R[SB_ALPHA];
+        const float *kinv = (const float *)S.kinv32;
+        const unsigned key = S.keys[i];
+        const size_t o = S.off + (size_t)ND * i;
+#pragma unroll
+        for (int a = 0; a < ND; a++) {
+            const float m = kinv[(size_t)key * ND + a], dg = kinv[(size_t)65536 * ND + (size_t)key * ND + a];
+            const float zn = z[o + a] - alpha * (m * q[o + a]);
+            z[o + a] = zn;
+            acc += (double)zn * (double)(dg * zn);
+        }
+    }
+    const double s = bsum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// x32 += sum_k alpha_k ring[k] over the window (every system; alpha 0 for
+// steps a system did not run)
+__global__ void __launch_bounds__(BB) k_ring_flush(const Sys *__restrict__ sys, const Chunk *__restrict__ chunks,
+                                                   const double *__restrict__ Sb, float *x32,
+                                                   const float *__restrict__ ring, size_t total, int w, int nd)
+{
+    __shared__ float coef[SB_RING_MAX];
+    const Chunk ch = chunks[blockIdx.x];
+    const double *R = Sb + ch.sys * SB_STRIDE;
+    if (R[SB_ACTIVE] == 0.0) return;  // ran no step since its alphas were cleared (block-uniform)
+    if (threadIdx.x < (unsigned)w) coef[threadIdx.x] = (float)R[SB_AHIST + threadIdx.x];
+    __syncthreads();
+    const Sys &S = sys[ch.sys];
+    const uint32_t i = ch.i0 + threadIdx.x;
+    if (i >= ch.i1) return;
+    for (int a = 0; a < nd; a++) {
+        const size_t o = S.off + (size_t)nd * i + a;
+        float acc = x32[o];
+        for (int k = 0; k < w; k++) acc += coef[k] * ring[(size_t)k * total + o];
+        x32[o] = acc;
+    }
+}
+
+// zeroes the window's alphas after its flush (one block per system)
+__global__ void k_clear_alpha(double *Sb, int w)
+{
+    if (threadIdx.x < (unsigned)w) Sb[blockIdx.x * SB_STRIDE + SB_AHIST + threadIdx.x] = 0.0;
+}
+
+// z-form start of an FP32 segment: z32 = dinv (float) r64, x32 = 0, partial
+// r.z (systems with `start` set)
+__global__ void __launch_bounds__(BB) k_start_z(const Sys *__restrict__ sys, const Chunk *__restrict__ chunks,
+                                                const double *__restrict__ Sb, const double *__restrict__ r64,
+                                                float *z32, float *x32, double *__restrict__ part, int nd)
+{
+    const Chunk ch = chunks[blockIdx.x];
+    const double *R = Sb + ch.sys * SB_STRIDE;
+    if (R[SB_START] == 0.0) return;
+    const Sys &S = sys[ch.sys];
+    const uint32_t i = ch.i0 + threadIdx.x;
+    double acc = 0;
+    if (i < ch.i1) {
+        const float *kinv = (const float *)S.kinv32;
+        const unsigned key = S.keys[i];
+        for (int a = 0; a < nd; a++) {
+            const size_t o = S.off + (size_t)nd * i + a;
+            const float rv = (float)r64[o];
+            const float z = kinv[(size_t)key * nd + a] * rv;
+            z32[o] = z;
+            x32[o] = 0.f;
+            acc += (double)rv * (double)z;
         }
     }
     const double s = bsum(acc);
@@ -255,6 +371,34 @@ __global__ void __launch_bounds__(BB) k_resid(const Sys *__restrict__ sys, const
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// z-form replacement, part 2: r64 -= q64 ; partial r64.(dinv r64) ;
+// z32 = dinv32 (float) r64
+__global__ void __launch_bounds__(BB) k_resid_z(const Sys *__restrict__ sys, const Chunk *__restrict__ chunks,
+                                                const double *__restrict__ Sb, double *r64,
+                                                const double *__restrict__ q64, float *z32, double *__restrict__ part,
+                                                int nd)
+{
+    const Chunk ch = chunks[blockIdx.x];
+    if (Sb[ch.sys * SB_STRIDE + SB_REPL] == 0.0) return;
+    const Sys &S = sys[ch.sys];
+    const uint32_t i = ch.i0 + threadIdx.x;
+    double acc = 0;
+    if (i < ch.i1) {
+        const double *kinv = (const double *)S.kinv64;
+        const float *kinv32 = (const float *)S.kinv32;
+        const unsigned key = S.keys[i];
+        for (int a = 0; a < nd; a++) {
+            const size_t o = S.off + (size_t)nd * i + a;
+            const double rn = r64[o] - q64[o];
+            r64[o] = rn;
+            z32[o] = kinv32[(size_t)key * nd + a] * (float)rn;
+            acc += rn * (kinv[(size_t)key * nd + a] * rn);
+        }
+    }
+    const double s = bsum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -272,7 +416,45 @@ template void apply<double>(const Batch &, const double *, double *, int, hipStr
 
 void finish(const Batch &b, int what, hipStream_t s)
 {
-    k_finish<<<b.nsys, BB, 0, s>>>(b.sys, b.Sb, b.part, what);
+    k_finish<<<b.nsys, BB, 0, s>>>(b.sys, b.Sb, b.part, what, 0);
+}
+
+void finish_alpha(const Batch &b, int w, hipStream_t s)
+{
+    k_finish<<<b.nsys, BB, 0, s>>>(b.sys, b.Sb, b.part, FIN_B_ALPHA, w);
+}
+
+void apply_dir(const Batch &b, const float *z, float *ring, size_t total, int w, float *q, hipStream_t s)
+{
+    if (b.ntiles == 0) return;
+    if (b.nd == 3)
+        k_apply_dir<3, TILE_R, TILE_C><<<b.ntiles, TILE_R * TILE_C, 0, s>>>(b.sys, b.tiles, z, ring, total, w, q, b.Sb,
+                                                                          b.part);
+    else
+        k_apply_dir<1, TILE_R, TILE_C><<<b.ntiles, TILE_R * TILE_C, 0, s>>>(b.sys, b.tiles, z, ring, total, w, q, b.Sb,
+                                                                          b.part);
+}
+
+void zupdate(const Batch &b, float *z, const float *q, hipStream_t s)
+{
+    if (b.nd == 3) k_zupdate<3><<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, z, q, b.part);
+    else k_zupdate<1><<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, z, q, b.part);
+}
+
+void ring_flush(const Batch &b, float *x32, const float *ring, size_t total, int w, hipStream_t s)
+{
+    k_ring_flush<<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, x32, ring, total, w, b.nd);
+    k_clear_alpha<<<b.nsys, SB_RING_MAX, 0, s>>>(b.Sb, w);
+}
+
+void start_z(const Batch &b, const double *r64, float *z32, float *x32, hipStream_t s)
+{
+    k_start_z<<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, r64, z32, x32, b.part, b.nd);
+}
+
+void resid_z(const Batch &b, double *r64, const double *q64, float *z32, hipStream_t s)
+{
+    k_resid_z<<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, r64, q64, z32, b.part, b.nd);
 }
 
 void dir(const Batch &b, float *d, const float *r, hipStream_t s)

@@ -1638,7 +1638,7 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
     Chunk *dchunks = nullptr;
     double *Sb = nullptr, *part = nullptr, *x64 = nullptr, *r64 = nullptr, *e64 = nullptr, *q64 = nullptr;
     double *rpart = nullptr, *rS = nullptr;
-    float *x32 = nullptr, *r32 = nullptr, *d32 = nullptr, *q32 = nullptr;
+    float *x32 = nullptr, *r32 = nullptr, *d32 = nullptr, *q32 = nullptr, *ring = nullptr;
     bool ok = bupload(dsys, sys.data(), sys.size()) && bupload(dtiles, tiles.data(), tiles.size()) &&
               bupload(dchunks, chunks.data(), chunks.size()) &&
               hipMalloc(&Sb, sizeof(double) * nsys * SB_STRIDE) == hipSuccess &&
@@ -1646,9 +1646,20 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
               hipMalloc(&x64, total * 8) == hipSuccess && hipMalloc(&r64, total * 8) == hipSuccess &&
               hipMalloc(&e64, total * 8) == hipSuccess && hipMalloc(&q64, total * 8) == hipSuccess &&
               hipMalloc(&x32, total * 4) == hipSuccess && hipMalloc(&r32, total * 4) == hipSuccess &&
-              hipMalloc(&d32, total * 4) == hipSuccess && hipMalloc(&q32, total * 4) == hipSuccess &&
+              hipMalloc(&q32, total * 4) == hipSuccess &&
               hipMalloc(&rpart, sizeof(double) * MAX_RED_GRID * 8) == hipSuccess &&
               hipMalloc(&rS, sizeof(double) * S_COUNT) == hipSuccess;
+    // Fused loop on z (r32 holds z): directions in a ring of w slots (16, or
+    // fewer if they do not fit; w divides the 16 steps between host
+    // synchronisations).  Without room for two slots: the separate
+    // direction, operator and update kernels on r.
+    int w = 0;
+    for (int cand = SB_RING_MAX; ok && cand >= 2 && !w; cand /= 2)
+        if (hipMalloc(&ring, total * 4 * (size_t)cand) == hipSuccess) w = cand;
+        else (void)hipGetLastError();
+    const bool zform = w >= 2;
+    if (zform) ok = ok && hipMemsetAsync(ring, 0, total * 4 * (size_t)w, st) == hipSuccess;
+    else ok = ok && hipMalloc(&d32, total * 4) == hipSuccess;
     Batch bt;
     bt.nsys = nsys; bt.nd = nd; bt.sys = dsys; bt.tiles = dtiles; bt.ntiles = (int)tiles.size();
     bt.chunks = dchunks; bt.nchunks = (int)chunks.size(); bt.Sb = Sb; bt.part = part;
@@ -1696,7 +1707,8 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
             R(s, SB_THRESH) = std::max(tol * tol * fabs(d0v[s]), drop * fabs(d0v[s]));
         }
         ok = ok && up();
-        start(bt, r64, r32, d32, x32, st);
+        if (zform) start_z(bt, r64, r32, x32, st);
+        else start(bt, r64, r32, d32, x32, st);
         finish(bt, FIN_B_START, st);
         ok = ok && down();
         for (int s = 0; s < nsys; s++) R(s, SB_START) = 0.0;
@@ -1708,6 +1720,14 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
         for (int s = 0; s < nsys; s++) live += R(s, SB_ACTIVE) != 0.0;
         if (!live) break;
         for (unsigned k = 0; k < nb; k++) {
+            if (zform) {
+                apply_dir(bt, r32, ring, total, w, q32, st);
+                finish_alpha(bt, w, st);
+                zupdate(bt, r32, q32, st);
+                finish(bt, FIN_B_DELTA, st);
+                if ((k + 1) % (unsigned)w == 0) ring_flush(bt, x32, ring, total, w, st);
+                continue;
+            }
             dir(bt, d32, r32, st);
             apply<float>(bt, d32, q32, OP_ITER, st);
             finish(bt, FIN_B_ALPHA, st);
@@ -1729,7 +1749,8 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
         // residual replacement of the selected systems
         flush(bt, x64, e64, x32, st);
         apply<double>(bt, e64, q64, OP_REPL, st);
-        resid(bt, r64, q64, r32, st);
+        if (zform) resid_z(bt, r64, q64, r32, st);
+        else resid(bt, r64, q64, r32, st);
         finish(bt, FIN_B_TRUE, st);
         if (!(ok = down())) break;
         for (int s = 0; s < nsys; s++) {
@@ -1789,7 +1810,7 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     bfree(dsys); bfree(dtiles); bfree(dchunks); bfree(Sb); bfree(part); bfree(x64); bfree(r64); bfree(e64);
-    bfree(q64); bfree(x32); bfree(r32); bfree(d32); bfree(q32); bfree(rpart); bfree(rS);
+    bfree(q64); bfree(x32); bfree(r32); bfree(d32); bfree(q32); bfree(ring); bfree(rpart); bfree(rS);
     (void)hipStreamDestroy(st);
     if (!ok) return 0;
     return all ? 1 : 2;
