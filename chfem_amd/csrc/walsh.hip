@@ -74,6 +74,7 @@ static void launch_walsh_nd(const OpDesc &op, const T *lamu, const T *v, T *q, c
     }
     if (t.TR == 16 && t.TC == 16) WALSH_LAUNCH(16, 16)
     if (t.TR == 64 && t.TC == 4) WALSH_LAUNCH(64, 4)
+    if (t.TR == 64 && t.TC == 8) WALSH_LAUNCH(64, 8)
     if (t.TR == 32 && t.TC == 16) WALSH_LAUNCH(32, 16)
     WALSH_LAUNCH(32, 8)
 #undef WALSH_LAUNCH

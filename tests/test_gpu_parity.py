@@ -206,7 +206,7 @@ def test_mixed_precision_elastic(ragged_elastic):
         s.close()
 
 
-@pytest.mark.parametrize("tile", ["32x8x16", "16x16x4", "64x4x7", "32x16x3"])
+@pytest.mark.parametrize("tile", ["32x8x16", "16x16x4", "64x4x7", "32x16x3", "64x8x5"])
 @pytest.mark.parametrize("shape", [(40, 70, 9), (3, 1, 2), (17, 33, 65)])
 @pytest.mark.parametrize("which", ["elastic", "thermal"])
 def test_walsh_operator_tiles(which, tile, shape, monkeypatch):
