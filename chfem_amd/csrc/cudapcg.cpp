@@ -170,7 +170,9 @@ static int set_scalar(int slot, double v)
 }
 
 // q = A v inside a Krylov loop; with timing on the launch is bracketed by
-// events whose interval is collected at the next host synchronisation.
+// events whose interval is collected at the next host synchronisation (the
+// fused CG forms below bind the events to the operator kernel's dispatch
+// instead, so their interval is that kernel's run time alone).
 static bool op_pending = false;
 template <typename T>
 static void timed_apply(const OpDesc &op, const T *v, T *q, const Reduce *red)
@@ -203,23 +205,18 @@ template <typename T>
 static void timed_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *dold, T *znew, T *dnew, T *qnew,
                             int pend, const Reduce *red, bool timed)
 {
-    if (timed && g->timing) hipEventRecord(g->opA, g->stream);
-    launch_apply_cg1<T>(op, zold, qold, dold, znew, dnew, qnew, pend, red, g->stream);
-    if (timed && g->timing) {
-        hipEventRecord(g->opB, g->stream);
-        op_pending = true;
-    }
+    const hipEvent_t ev[2] = {g->opA, g->opB};
+    const bool on = timed && g->timing;
+    launch_apply_cg1<T>(op, zold, qold, dold, znew, dnew, qnew, pend, red, g->stream, on ? ev : nullptr);
+    if (on) op_pending = true;
 }
 
 template <typename T>
 static void timed_apply_dir(const OpDesc &op, const T *r, const T *dold, T *dnew, T *q, const Reduce *red)
 {
-    if (g->timing) hipEventRecord(g->opA, g->stream);
-    launch_apply_dir<T>(op, r, dold, dnew, q, red, g->stream);
-    if (g->timing) {
-        hipEventRecord(g->opB, g->stream);
-        op_pending = true;
-    }
+    const hipEvent_t ev[2] = {g->opA, g->opB};
+    launch_apply_dir<T>(op, r, dold, dnew, q, red, g->stream, g->timing ? ev : nullptr);
+    if (g->timing) op_pending = true;
 }
 
 static int sync_loop()

@@ -187,10 +187,12 @@ namespace chfem {
 struct WalshTiles { int TR, TC, LZ; };
 template <typename T> struct DirFuse;
 // fz non-null: v is d_old and the launch forms d = z + beta d_old
-// (walsh.h DirFuse), writes it to fz->dnew and reduces d.q (red required)
+// (walsh.h DirFuse), writes it to fz->dnew and reduces d.q (red required).
+// ev non-null: {start, stop} events bound to the operator kernel's dispatch
+// (hipExtLaunchKernel), timing that kernel alone.
 template <typename T>
 void launch_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduce *red, WalshTiles t, hipStream_t s,
-                  const DirFuse<T> *fz = nullptr);
+                  const DirFuse<T> *fz = nullptr, const hipEvent_t *ev = nullptr);
 
 // Jacobi-PCG step on the preconditioned residual z = dinv r with the
 // direction update inside the operator (Walsh-basis thermal or isotropic
@@ -201,7 +203,8 @@ void launch_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduc
 // iterate.
 bool apply_dir_ok(const OpDesc &op);
 template <typename T>
-void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q, const Reduce *red, hipStream_t s);
+void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q, const Reduce *red, hipStream_t s,
+                      const hipEvent_t *ev = nullptr);
 // Single-kernel Jacobi-PCG iteration (same operators as launch_apply_dir):
 // z = zold - S[ALPHA] qold (qold holds dinv q of the previous iteration;
 // skipped unless pend), d = z + beta dold (dold == nullptr: d = z), znew <- z,
@@ -210,7 +213,7 @@ void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q
 // other.
 template <typename T>
 void launch_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *dold, T *znew, T *dnew, T *qnew,
-                      int pend, const Reduce *red, hipStream_t s);
+                      int pend, const Reduce *red, hipStream_t s, const hipEvent_t *ev = nullptr);
 // The iterate of the fused CG loops: iteration j writes its direction to
 // ring slot j % w; x += sum of S[S_AHIST + (j - base) % S_AHIST_LEN] *
 // ring[j % w] over the iterations j in [j0, j1) that ran (j - base <

@@ -1,6 +1,7 @@
 // walsh.hip -- kernel and launcher of the Walsh-basis elastic and conduction operators
 // (device code in walsh.h).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include "walsh.h"
 
 namespace chfem {
@@ -37,7 +38,7 @@ __global__ void __launch_bounds__(TR *TC) k_walsh(const T *__restrict__ v, T *__
 
 template <typename T, int ND>
 static void launch_walsh_nd(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduce *red, WalshTiles t,
-                            hipStream_t s, const DirFuse<T> *fz)
+                            hipStream_t s, const DirFuse<T> *fz, const hipEvent_t *ev)
 {
     Reduce rr = {nullptr, nullptr, 0, 0, 0};
     if (red) rr = *red;
@@ -52,24 +53,29 @@ static void launch_walsh_nd(const OpDesc &op, const T *lamu, const T *v, T *q, c
         const long cols = (long)((op.ny + tr - 2) / (tr - 1)) * ((op.nx + tc - 2) / (tc - 1));
         while (cols * ((op.nz + LZ - 1) / LZ) > MAX_PARTIAL_BLOCKS) LZ *= 2;
     }
+    // ev: a start and a stop event bound to the k_walsh dispatch itself
+    // (hipExtLaunchKernel), so their interval is the kernel's own run time
+#define WALSH_GO(TRv, TCv, DOTv, FMv)                                                                   \
+    if (ev) hipExtLaunchKernelGGL(k_walsh<T, ND, TRv, TCv, DOTv, FMv>, dim3(grid), dim3(TRv * TCv), 0, s, ev[0], \
+                                  ev[1], 0, v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);        \
+    else k_walsh<T, ND, TRv, TCv, DOTv, FMv><<<grid, TRv * TCv, 0, s>>>(v, q, op.keys, lamu, op.ebe, op.ny, op.nx, \
+                                                                         op.nz, LZ, rr, f);
 #define WALSH_LAUNCH(TRv, TCv)                                                                          \
     {                                                                                                   \
         const unsigned grid = (unsigned)((op.ny + TRv - 2) / (TRv - 1)) * ((op.nx + TCv - 2) / (TCv - 1)) *    \
                               ((op.nz + LZ - 1) / LZ);                                                  \
         if (cg1) {                                                                                      \
-            k_walsh<T, ND, TRv, TCv, true, 2><<<grid, TRv * TCv, 0, s>>>(                               \
-                v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);                           \
+            WALSH_GO(TRv, TCv, true, 2)                                                                 \
             launch_reduce_finish(rr, grid, 4, s);                                                       \
         } else if (fz) {                                                                                \
-            k_walsh<T, ND, TRv, TCv, true, 1><<<grid, TRv * TCv, 0, s>>>(                               \
-                v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);                           \
+            WALSH_GO(TRv, TCv, true, 1)                                                                 \
             launch_reduce_finish(rr, grid, 1, s);                                                       \
         } else if (red) {                                                                               \
-            k_walsh<T, ND, TRv, TCv, true, 0><<<grid, TRv * TCv, 0, s>>>(                               \
-                v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);                           \
+            WALSH_GO(TRv, TCv, true, 0)                                                                 \
             launch_reduce_finish(rr, grid, 1, s);                                                       \
-        } else k_walsh<T, ND, TRv, TCv, false, 0><<<grid, TRv * TCv, 0, s>>>(                           \
-                v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);                           \
+        } else {                                                                                        \
+            WALSH_GO(TRv, TCv, false, 0)                                                                \
+        }                                                                                               \
         return;                                                                                         \
     }
     if (t.TR == 16 && t.TC == 16) WALSH_LAUNCH(16, 16)
@@ -78,14 +84,15 @@ static void launch_walsh_nd(const OpDesc &op, const T *lamu, const T *v, T *q, c
     if (t.TR == 32 && t.TC == 16) WALSH_LAUNCH(32, 16)
     WALSH_LAUNCH(32, 8)
 #undef WALSH_LAUNCH
+#undef WALSH_GO
 }
 
 template <typename T>
 void launch_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduce *red, WalshTiles t, hipStream_t s,
-                  const DirFuse<T> *fz)
+                  const DirFuse<T> *fz, const hipEvent_t *ev)
 {
-    if (op.physics == PHY_ELASTIC) launch_walsh_nd<T, 3>(op, lamu, v, q, red, t, s, fz);
-    else launch_walsh_nd<T, 1>(op, lamu, v, q, red, t, s, fz);
+    if (op.physics == PHY_ELASTIC) launch_walsh_nd<T, 3>(op, lamu, v, q, red, t, s, fz, ev);
+    else launch_walsh_nd<T, 1>(op, lamu, v, q, red, t, s, fz, ev);
 }
 
 bool apply_dir_ok(const OpDesc &op)
@@ -95,35 +102,36 @@ bool apply_dir_ok(const OpDesc &op)
 }
 
 template <typename T>
-void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q, const Reduce *red, hipStream_t s)
+void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q, const Reduce *red, hipStream_t s,
+                      const hipEvent_t *ev)
 {
     const DirFuse<T> f = {z, dnew, T(0), T(0), dold ? 0 : 1, 0, nullptr, nullptr, nullptr};
     const WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
-    launch_walsh<T>(op, (const T *)op.lamu, dold, q, red, t, s, &f);
+    launch_walsh<T>(op, (const T *)op.lamu, dold, q, red, t, s, &f, ev);
 }
 
 template <typename T>
 void launch_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *dold, T *znew, T *dnew, T *qnew,
-                      int pend, const Reduce *red, hipStream_t s)
+                      int pend, const Reduce *red, hipStream_t s, const hipEvent_t *ev)
 {
     const int first = dold ? 0 : 1, p = dold && pend ? 1 : 0;
     const DirFuse<T> f = {zold, dnew, T(0), T(0), first, p, p ? qold : zold, znew, (const T *)op.kinv};
     const WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
-    launch_walsh<T>(op, (const T *)op.lamu, dold, qnew, red, t, s, &f);
+    launch_walsh<T>(op, (const T *)op.lamu, dold, qnew, red, t, s, &f, ev);
 }
 
 template void launch_apply_cg1<float>(const OpDesc &, const float *, const float *, const float *, float *, float *,
-                                      float *, int, const Reduce *, hipStream_t);
+                                      float *, int, const Reduce *, hipStream_t, const hipEvent_t *);
 template void launch_apply_cg1<double>(const OpDesc &, const double *, const double *, const double *, double *,
-                                       double *, double *, int, const Reduce *, hipStream_t);
+                                       double *, double *, int, const Reduce *, hipStream_t, const hipEvent_t *);
 
 template void launch_apply_dir<float>(const OpDesc &, const float *, const float *, float *, float *, const Reduce *,
-                                      hipStream_t);
+                                      hipStream_t, const hipEvent_t *);
 template void launch_apply_dir<double>(const OpDesc &, const double *, const double *, double *, double *,
-                                       const Reduce *, hipStream_t);
+                                       const Reduce *, hipStream_t, const hipEvent_t *);
 template void launch_walsh<float>(const OpDesc &, const float *, const float *, float *, const Reduce *, WalshTiles,
-                                  hipStream_t, const DirFuse<float> *);
+                                  hipStream_t, const DirFuse<float> *, const hipEvent_t *);
 template void launch_walsh<double>(const OpDesc &, const double *, const double *, double *, const Reduce *,
-                                   WalshTiles, hipStream_t, const DirFuse<double> *);
+                                   WalshTiles, hipStream_t, const DirFuse<double> *, const hipEvent_t *);
 
 }  // namespace chfem
