@@ -391,8 +391,9 @@ static bool is_minres();
 static void autotune_tiles()
 {
     if (!g->autotune || g->tuned) return;
+    // (tile rows 65: 64-row tiles with the row above on an extra wavefront)
     static const int cand[][3] = {{32, 8, 16}, {32, 8, 32}, {16, 16, 16}, {32, 16, 16}, {32, 16, 32}, {64, 4, 16},
-                                  {64, 8, 16}};
+                                  {64, 8, 16}, {65, 7, 16}, {65, 7, 32}, {65, 15, 16}};
     const int nc = (int)(sizeof(cand) / sizeof(cand[0]));
     const size_t n = nvars();
     // the CG loops' kernels per iteration: form 1 = operator with the

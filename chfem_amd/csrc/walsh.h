@@ -245,7 +245,12 @@ struct TileSums {
 // patch of elements per element layer and outputs the (TR-1) x (TC-1) nodes
 // interior to the patch (work-item (tr, tc), tr, tc >= 1, owns node
 // (r0 + tr - 1, c0 + tc - 1)).  Every work-item evaluates exactly one element
-// per layer step.  The block marches through LZ node layers; node layers stream
+// per layer step.  HALO (TR = 64): the TR x TC work-items evaluate the
+// elements whose origin is a node of the tile's TR rows, and one more
+// wavefront evaluates the TC elements of the row above (origin row r0 - 1),
+// whose far-row forces reach row 0 of the tile through LDS; the block then
+// outputs TR x (TC-1) nodes (work-item (tr, tc), tc >= 1, owns node
+// (r0 + tr, c0 + tc - 1)), so tiles of 64 rows cover 256 rows exactly.  The block marches through LZ node layers; node layers stream
 // through two LDS buffers, the next layer's values are fetched into registers
 // while the current layer's elements are evaluated, and the far-face forces of
 // each node are carried to the next step in registers.
@@ -259,7 +264,7 @@ struct TileSums {
 // one to one, the batched solver (batch.hip) through a tile table.  FUSE: the
 // input is the fused direction update of DirFuse (the dot is then d.q with
 // the updated d).
-template <typename T, int ND, int TR, int TC, bool DOT, int FM = 0>
+template <typename T, int ND, int TR, int TC, bool DOT, int FM = 0, bool HALO = false>
 __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__restrict__ q,
                                                const uint16_t *__restrict__ keys, const T *__restrict__ lamu, int ebe,
                                                int ny, int nx, int nz, int LZ, int bx, int by, int bz,
@@ -269,7 +274,11 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     // FM 0: plain apply; 1: direction update inside (FUSE); 2: single-kernel
     // CG, the z update inside as well (CG1)
     constexpr bool FUSE = FM >= 1, CG1 = FM == 2;
-    constexpr int NT = TR * TC, NR = TR + 1, NC = TC + 1, NV = NR * NC * ND;
+    static_assert(!HALO || TR == 64, "the halo wavefront variant takes 64-row columns");
+    // NCOL column work-items (+ the halo wavefront); OR node rows owned per
+    // tile; the node planes hold rows r0 - 1 .. r0 + OR
+    constexpr int NCOL = TR * TC, NT = NCOL + (HALO ? 64 : 0), OR = HALO ? TR : TR - 1;
+    constexpr int NR = OR + 2, NC = TC + 1, NV = NR * NC * ND;
     constexpr int SLOTS = (NV + NT - 1) / NT;
     constexpr int SB = 2 * ND + 1;  // padded stride of the row-combined forces in LDS (no bank conflicts)
     // node planes padded to SLOTS * NT entries: every work-item stores every
@@ -279,15 +288,27 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     __shared__ T su[2][SLOTS * NT];
     // CG1: the updated z planes, for the owned nodes' z output and z.q
     __shared__ T szp[CG1 ? 2 : 1][CG1 ? SLOTS * NT : 1];
-    __shared__ T sb[NT * SB];
+    __shared__ T sb[NCOL * SB];
+    // HALO: far-row forces f0, f4, f1, f5 of the row-above elements, per
+    // column (one slot per work-item of the halo wavefront)
+    __shared__ T shal[HALO ? 64 * 4 * ND : 1];
     // material coefficients of materials 0..3, and material 4 (all zero) for
     // the work-items without an element: they evaluate a zero element instead
     // of branching around the evaluation
     __shared__ T smat[5 * WCOEF];
     const int tid = threadIdx.x;
-    const int tr = tid % TR, tc = tid / TR;
-    const int r0 = bx * (TR - 1), c0 = by * (TC - 1), l0 = bz * LZ;
-    const int vr = min(TR - 1, ny - r0), vc = min(TC - 1, nx - c0), vl = min(LZ, nz - l0);
+    // hl: work-item of the halo wavefront (wavefront-uniform), evaluating the
+    // element of column hc (hc < TC) of the row above the tile
+    const bool hl = HALO && tid >= NCOL;
+    const int hc = tid - NCOL;
+    const int tr = hl ? 0 : tid % TR, tc = hl ? 0 : tid / TR;
+    const int r0 = bx * OR, c0 = by * (TC - 1), l0 = bz * LZ;
+    const int vr = min(OR, ny - r0), vc = min(TC - 1, nx - c0), vl = min(LZ, nz - l0);
+    // this work-item's element: origin at plane row er, column ec (node
+    // (r0 - 1 + er, c0 - 1 + ec)); halo work-items past the tile's columns
+    // point at column 0 and evaluate the zero material
+    const int er = hl ? 0 : tr + (HALO ? 1 : 0);
+    const int ec = hl ? (hc < TC ? hc : 0) : tc;
     const size_t nxy = (size_t)nx * ny;
     if (tid < 5) material_coefficients<T>(ND, tid < 4 ? lamu[2 * tid] : T(0), tid < 4 ? lamu[2 * tid + 1] : T(0),
                                           smat + WCOEF * tid);
@@ -355,10 +376,10 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     // element of this work-item: origin (r0 - 1 + tr, c0 - 1 + tc).  Its
     // material is read from the packed keys one layer ahead (mat_key) and
     // decoded when needed (material).
-    const bool elem_ok = tr <= vr && tc <= vc;
+    const bool elem_ok = hl ? (hc < TC && hc <= vc) : ((HALO ? tr < vr : tr <= vr) && tc <= vc);
     // (work-items without an element read node 0 of the layer: the key load
     // is unconditional)
-    const uint32_t eoff = elem_ok ? (uint32_t)wrapi(c0 - 1 + tc, nx) * (uint32_t)ny + (uint32_t)wrapi(r0 - 1 + tr, ny)
+    const uint32_t eoff = elem_ok ? (uint32_t)wrapi(c0 - 1 + ec, nx) * (uint32_t)ny + (uint32_t)wrapi(r0 - 1 + er, ny)
                                   : 0u;
     // (mat_key takes the wrapped layer index)
     auto mat_key = [&](int lw) -> uint16_t { return (keys + (size_t)lw * nxy)[eoff]; };
@@ -386,7 +407,7 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             const T *src = WDL(k) ? nxt : cur;
-            const int o = ((tc + WDC(k)) * NR + tr + WDR(k)) * ND;
+            const int o = ((ec + WDC(k)) * NR + er + WDR(k)) * ND;
 #pragma unroll
             for (int a = 0; a < ND; a++) u[k][a] = src[o + a];
         }
@@ -400,6 +421,20 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
 #pragma unroll
             for (int k = 0; k < 8; k++) f[k][0] = f1[k];
         }
+        if (hl) {
+            // row-above element: its forces on the tile's row 0 (local
+            // nodes 0, 4 in its column, 1, 5 in the next)
+            T *h = shal + hc * 4 * ND;
+#pragma unroll
+            for (int a = 0; a < ND; a++) {
+                h[a] = f[0][a];
+                h[ND + a] = f[4][a];
+                h[2 * ND + a] = f[1][a];
+                h[3 * ND + a] = f[5][a];
+                A[a] = Af[a] = T(0);
+            }
+            return;
+        }
         T *dst = sb + tid * SB;
 #pragma unroll
         for (int a = 0; a < ND; a++) {
@@ -409,7 +444,22 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
             dst[ND + a] = f[6][a] + lane_up1(f[5][a]);
         }
     };
-    const bool own = tr >= 1 && tc >= 1 && tr - 1 < vr && tc - 1 < vc;
+    // HALO: row 0 of the tile (lane 0 of each column wavefront, where the
+    // cross-lane move brings 0) adds the row-above elements' forces, after
+    // the barrier that follows the elements
+    auto halo_row = [&](T (&A)[ND], T (&Af)[ND]) {
+        if (HALO && tr == 0) {
+            const T *h0 = shal + tc * 4 * ND, *h1 = shal + (tc - 1) * 4 * ND;
+#pragma unroll
+            for (int a = 0; a < ND; a++) {
+                A[a] += h0[a] + h1[2 * ND + a];
+                Af[a] += h0[ND + a] + h1[3 * ND + a];
+            }
+        }
+    };
+    // owned node (r0 + orow, c0 + tc - 1)
+    const int orow = HALO ? tr : tr - 1;
+    const bool own = !hl && orow >= 0 && tc >= 1 && orow < vr && tc - 1 < vc;
     // B and Bf of the element left of this node's column, same row
     const T *bleft = sb + ((tc >= 1 ? tc - 1 : 0) * TR + tr) * SB;
 
@@ -430,6 +480,7 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     element(cur, nxt, m, A, Af);
     __syncthreads();
     T acc[ND];
+    if (own) halo_row(A, Af);
 #pragma unroll
     for (int a = 0; a < ND; a++) acc[a] = own ? Af[a] + bleft[ND + a] : T(0);
     store(cur, zcur);  // layer l0 + 1 replaces layer l0 - 1
@@ -443,7 +494,7 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
     double dsum = 0.0, zq = 0.0, qmq = 0.0, zdz = 0.0;
     // this work-item's node offset in a layer (work-items that own no node
     // use node 0 for their unconditional loads; only owners store)
-    const size_t own_off = own ? (size_t)(c0 + tc - 1) * ny + (r0 + tr - 1) : 0;
+    const size_t own_off = own ? (size_t)(c0 + tc - 1) * ny + (r0 + orow) : 0;
     for (int s = 0; s < vl; s++) {
         const int l = l0 + s;
         const size_t on = ND * ((size_t)l * nxy + own_off);
@@ -471,11 +522,11 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
         // its in-tile entry; only owners use it.
         T own_u[ND], own_z[CG1 ? ND : 1];
         {
-            const T *uu = cur + (tc * NR + tr) * ND;
+            const T *uu = cur + (ec * NR + er) * ND;
 #pragma unroll
             for (int a = 0; a < ND; a++) own_u[a] = uu[a];
             if constexpr (CG1) {
-                const T *zz = zcur + (tc * NR + tr) * ND;
+                const T *zz = zcur + (ec * NR + er) * ND;
 #pragma unroll
                 for (int a = 0; a < ND; a++) own_z[a] = zz[a];
             }
@@ -496,6 +547,7 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
         kraw = mk;
         pin_here(m);
         if (own) {
+            halo_row(A, Af);
             T o[ND];
 #pragma unroll
             for (int a = 0; a < ND; a++) {

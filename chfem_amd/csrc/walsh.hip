@@ -6,8 +6,8 @@
 
 namespace chfem {
 
-template <typename T, int ND, int TR, int TC, bool DOT, int FM>
-__global__ void __launch_bounds__(TR *TC) k_walsh(const T *__restrict__ v, T *__restrict__ q,
+template <typename T, int ND, int TR, int TC, bool DOT, int FM, bool HALO>
+__global__ void __launch_bounds__(TR *TC + (HALO ? 64 : 0)) k_walsh(const T *__restrict__ v, T *__restrict__ q,
                                                    const uint16_t *__restrict__ keys, const T *__restrict__ lamu,
                                                    int ebe, int ny, int nx, int nz, int LZ, Reduce red, DirFuse<T> fz)
 {
@@ -22,10 +22,11 @@ __global__ void __launch_bounds__(TR *TC) k_walsh(const T *__restrict__ v, T *__
     // XCD b % 8), so each XCD is given one contiguous range of tiles (rows of
     // tiles fastest): tiles that share halo nodes then run on the same XCD at
     // about the same time and meet each other's halo lines in its L2.
-    const unsigned gx = (unsigned)(ny + TR - 2) / (TR - 1), gy = (unsigned)(nx + TC - 2) / (TC - 1);
+    constexpr int OR = HALO ? TR : TR - 1;  // node rows per tile
+    const unsigned gx = (unsigned)(ny + OR - 1) / OR, gy = (unsigned)(nx + TC - 2) / (TC - 1);
     const unsigned b = blockIdx.x, nb = gridDim.x, per = nb / 8u, rem = nb % 8u, xcd = b % 8u, k = b / 8u;
     const unsigned t = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + k;
-    const TileSums ts = walsh_tile<T, ND, TR, TC, DOT, FM>(v, q, keys, lamu, ebe, ny, nx, nz, LZ, (int)(t % gx),
+    const TileSums ts = walsh_tile<T, ND, TR, TC, DOT, FM, HALO>(v, q, keys, lamu, ebe, ny, nx, nz, LZ, (int)(t % gx),
                                                            (int)((t / gx) % gy), (int)(t / (gx * gy)), fz);
     if (DOT && FM == 2) {
         double vv[4] = {ts.dq, ts.zq, ts.qmq, ts.zdz};
@@ -55,34 +56,45 @@ static void launch_walsh_nd(const OpDesc &op, const T *lamu, const T *v, T *q, c
     }
     // ev: a start and a stop event bound to the k_walsh dispatch itself
     // (hipExtLaunchKernel), so their interval is the kernel's own run time
-#define WALSH_GO(TRv, TCv, DOTv, FMv)                                                                   \
-    if (ev) hipExtLaunchKernelGGL(k_walsh<T, ND, TRv, TCv, DOTv, FMv>, dim3(grid), dim3(TRv * TCv), 0, s, ev[0], \
-                                  ev[1], 0, v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);        \
-    else k_walsh<T, ND, TRv, TCv, DOTv, FMv><<<grid, TRv * TCv, 0, s>>>(v, q, op.keys, lamu, op.ebe, op.ny, op.nx, \
-                                                                         op.nz, LZ, rr, f);
-#define WALSH_LAUNCH(TRv, TCv)                                                                          \
+#define WALSH_GO(TRv, TCv, H, DOTv, FMv)                                                                \
+    if (ev) hipExtLaunchKernelGGL(k_walsh<T, ND, TRv, TCv, DOTv, FMv, H>, dim3(grid), dim3(nt), 0, s, ev[0], ev[1], \
+                                  0, v, q, op.keys, lamu, op.ebe, op.ny, op.nx, op.nz, LZ, rr, f);               \
+    else k_walsh<T, ND, TRv, TCv, DOTv, FMv, H><<<grid, nt, 0, s>>>(v, q, op.keys, lamu, op.ebe, op.ny, op.nx,   \
+                                                                     op.nz, LZ, rr, f);
+    // H: the halo-wavefront variant (tiles own all TRv rows; one more
+    // wavefront per block)
+#define WALSH_LAUNCH(TRv, TCv, H)                                                                       \
     {                                                                                                   \
-        const unsigned grid = (unsigned)((op.ny + TRv - 2) / (TRv - 1)) * ((op.nx + TCv - 2) / (TCv - 1)) *    \
+        constexpr int orows = H ? TRv : TRv - 1, nt = TRv * TCv + (H ? 64 : 0);                         \
+        const unsigned grid = (unsigned)((op.ny + orows - 1) / orows) * ((op.nx + TCv - 2) / (TCv - 1)) * \
                               ((op.nz + LZ - 1) / LZ);                                                  \
         if (cg1) {                                                                                      \
-            WALSH_GO(TRv, TCv, true, 2)                                                                 \
+            WALSH_GO(TRv, TCv, H, true, 2)                                                              \
             launch_reduce_finish(rr, grid, 4, s);                                                       \
         } else if (fz) {                                                                                \
-            WALSH_GO(TRv, TCv, true, 1)                                                                 \
+            WALSH_GO(TRv, TCv, H, true, 1)                                                              \
             launch_reduce_finish(rr, grid, 1, s);                                                       \
         } else if (red) {                                                                               \
-            WALSH_GO(TRv, TCv, true, 0)                                                                 \
+            WALSH_GO(TRv, TCv, H, true, 0)                                                              \
             launch_reduce_finish(rr, grid, 1, s);                                                       \
         } else {                                                                                        \
-            WALSH_GO(TRv, TCv, false, 0)                                                                \
+            WALSH_GO(TRv, TCv, H, false, 0)                                                             \
         }                                                                                               \
         return;                                                                                         \
     }
-    if (t.TR == 16 && t.TC == 16) WALSH_LAUNCH(16, 16)
-    if (t.TR == 64 && t.TC == 4) WALSH_LAUNCH(64, 4)
-    if (t.TR == 64 && t.TC == 8) WALSH_LAUNCH(64, 8)
-    if (t.TR == 32 && t.TC == 16) WALSH_LAUNCH(32, 16)
-    WALSH_LAUNCH(32, 8)
+    // tile rows 65: 65 element rows per column (64 owned + the row above),
+    // the halo-wavefront variant of 64-row tiles.  15 columns (1024
+    // work-items) only in FP32: the FP64 planes of the single-kernel CG
+    // iteration would not fit in LDS.
+    if constexpr (sizeof(T) == 4) {
+        if (t.TR == 65 && t.TC == 15) WALSH_LAUNCH(64, 15, true)
+    }
+    if (t.TR == 65) WALSH_LAUNCH(64, 7, true)
+    if (t.TR == 16 && t.TC == 16) WALSH_LAUNCH(16, 16, false)
+    if (t.TR == 64 && t.TC == 4) WALSH_LAUNCH(64, 4, false)
+    if (t.TR == 64 && t.TC == 8) WALSH_LAUNCH(64, 8, false)
+    if (t.TR == 32 && t.TC == 16) WALSH_LAUNCH(32, 16, false)
+    WALSH_LAUNCH(32, 8, false)
 #undef WALSH_LAUNCH
 #undef WALSH_GO
 }

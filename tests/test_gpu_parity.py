@@ -206,7 +206,7 @@ def test_mixed_precision_elastic(ragged_elastic):
         s.close()
 
 
-@pytest.mark.parametrize("tile", ["32x8x16", "16x16x4", "64x4x7", "32x16x3", "64x8x5"])
+@pytest.mark.parametrize("tile", ["32x8x16", "16x16x4", "64x4x7", "32x16x3", "64x8x5", "65x7x5", "65x15x3"])
 @pytest.mark.parametrize("shape", [(40, 70, 9), (3, 1, 2), (17, 33, 65)])
 @pytest.mark.parametrize("which", ["elastic", "thermal"])
 def test_walsh_operator_tiles(which, tile, shape, monkeypatch):
@@ -400,6 +400,36 @@ def test_direction_ring_widths(ring, fuse, precision, monkeypatch):
         x, ok, st = s.solve(m.rhs(1))
         assert ok
         assert rel(m.update_c(1, x), Cr) < 1e-7
+        if precision == 0:
+            assert rel(x, xr) < 1e-7
+            assert abs(int(st.iterations) - itr) <= 2
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("fuse", ["2", "1"])
+@pytest.mark.parametrize("tile", ["65x7x5", "65x15x3"])
+@pytest.mark.parametrize("which", ["elastic", "thermal"])
+def test_halo_tiles_fused_solve(which, tile, fuse, precision, monkeypatch):
+    """Tiles of 64 owned rows (65 element rows: the row above each tile is
+    evaluated by an extra wavefront) in the fused CG loops: two tiles along
+    the rows (70 = 64 + 6), ragged columns and layers; the solution's
+    effective column against the oracle's, and the FP64 iterate itself."""
+    monkeypatch.setenv("CHFEM_FUSE_DIR", fuse)
+    monkeypatch.setenv("CHFEM_WALSH_TILE", tile)
+    img = three_phase((11, 70, 23), 61)
+    if which == "elastic":
+        m = H.model(H.ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25], colors=(0, 100, 255))
+    else:
+        m = H.model(H.THERMAL, img, [1.0, 10.0, 3.5], colors=(0, 100, 255))
+    xr, okr, itr, _ = m.solve(0, tol=1e-10)
+    assert okr
+    s = H.GpuSolver(m, precision=precision, tol=1e-10 if precision == 0 else 1e-9)
+    try:
+        x, ok, st = s.solve(m.rhs(0))
+        assert ok
+        assert rel(m.update_c(0, x), m.update_c(0, xr)) < 1e-7
         if precision == 0:
             assert rel(x, xr) < 1e-7
             assert abs(int(st.iterations) - itr) <= 2
