@@ -229,12 +229,17 @@ def main():
     key = {2: "cg1_", 1: "dir_"}.get(form, "")
     traffic = read_traffic(f"elastic_walsh_{key}fp32_{n}_{fast['tile']}")
     tr_, tc_, lz_ = fast["tile"].split("x")
+    # tile rows 65: the kernel's 64-row tiles with the row above on an extra
+    # wavefront (its last template argument)
+    halo = tr_ == "65"
+    kt = f"k_walsh<float,3,{64 if halo else tr_},{tc_},true,{{}},{'true' if halo else 'false'}>"
+    geom = f"LZ {lz_}" + (", 64 owned rows per tile + the row above on an extra wavefront" if halo else "")
     kernel = {
-        2: f"k_walsh<float,3,{tr_},{tc_},true,2> (FP32 elastic NBN single-kernel CG iteration: z -= alpha M^-1 q, "
-           f"d = z + beta d, q = A d, d.q z.q q.M^-1q z.Dz), LZ {lz_}",
-        1: f"k_walsh<float,3,{tr_},{tc_},true,1> (FP32 elastic NBN apply with the CG direction update "
-           f"d = z + beta d (z = M^-1 r) and d.Ad fused), LZ {lz_}",
-    }.get(form, f"k_walsh<float,3,{tr_},{tc_},true,0> (FP32 elastic NBN apply + fused d.Ad), LZ {lz_}")
+        2: kt.format(2) + " (FP32 elastic NBN single-kernel CG iteration: z -= alpha M^-1 q, "
+           f"d = z + beta d, q = A d, d.q z.q q.M^-1q z.Dz), {geom}",
+        1: kt.format(1) + " (FP32 elastic NBN apply with the CG direction update "
+           f"d = z + beta d (z = M^-1 r) and d.Ad fused), {geom}",
+    }.get(form, kt.format(0) + f" (FP32 elastic NBN apply + fused d.Ad), {geom}")
     cpu = None
     if not args.skip_cpu and plain is not None:
         log(f"[bench] cpu baseline (oracle)")
