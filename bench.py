@@ -212,18 +212,17 @@ def main():
     N = n ** 3
     op_avg_ms = fast["op_ms_total"] / max(fast["op_launches"], 1)
     # Algorithmic bytes per node of the operator launch inside the CG loop,
-    # by the loop form the library used (1 B image + 3-vectors of 12 B):
-    # form 2 (single-kernel iteration): z_old, q_old, d_old, x read + z, d, q,
-    # x written; form 1: z, d_old, x read + d, q, x written; form 0: v read +
-    # q written.
+    # by the loop form the library used: 1 B image + the 3-vectors streamed
+    # per node (12 B in FP32, 24 B in FP64).  Plain apply: d read, q written;
+    # form 1: z, d_old read + d, q written; form 2 (single-kernel iteration,
+    # z recovered from the two previous directions): d_old2, qt_old, d_old
+    # read + d, qt written.  The iterate is updated per window of iterations
+    # by the ring flush, outside these launches.
     form = fast["form"]
-    # vectors streamed per node: plain apply d, q; form 1 z, d_old, d, q;
-    # form 2 z_old, qt_old, d_old, z, d, qt (the iterate is updated per window
-    # of iterations by the ring flush, outside these launches)
-    nvec = {2: 6, 1: 4}.get(form, 2)
+    nvec = {2: 5, 1: 4}.get(form, 2)
     per_node32 = 1.0 + nvec * 12.0
     plain_form = plain.get("form", form) if plain is not None else form
-    per_node64 = 1.0 + {2: 6, 1: 4}.get(plain_form, 2) * 24.0
+    per_node64 = 1.0 + {2: 5, 1: 4}.get(plain_form, 2) * 24.0
     bytes_per_launch = per_node32 * N
     achieved = bytes_per_launch / (op_avg_ms * 1e-3) / 1e9
     key = {2: "cg1_", 1: "dir_"}.get(form, "")
@@ -235,8 +234,8 @@ def main():
     kt = f"k_walsh<float,3,{64 if halo else tr_},{tc_},true,{{}},{'true' if halo else 'false'}>"
     geom = f"LZ {lz_}" + (", 64 owned rows per tile + the row above on an extra wavefront" if halo else "")
     kernel = {
-        2: kt.format(2) + " (FP32 elastic NBN single-kernel CG iteration: z -= alpha M^-1 q, "
-           f"d = z + beta d, q = A d, d.q z.q q.M^-1q z.Dz), {geom}",
+        2: kt.format(2) + " (FP32 elastic NBN single-kernel CG iteration: z = d_old - beta' d_old2 - "
+           f"alpha M^-1 q, d = z + beta d_old, q = A d, d.q z.q q.M^-1q z.Dz), {geom}",
         1: kt.format(1) + " (FP32 elastic NBN apply with the CG direction update "
            f"d = z + beta d (z = M^-1 r) and d.Ad fused), {geom}",
     }.get(form, kt.format(0) + f" (FP32 elastic NBN apply + fused d.Ad), {geom}")

@@ -203,11 +203,11 @@ static bool fuse_dir_enabled() { return fuse_mode_env() != 0; }
 
 template <typename T>
 static void timed_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *dold, T *znew, T *dnew, T *qnew,
-                            int pend, const Reduce *red, bool timed)
+                            int pend, const Reduce *red, bool timed, const T *dold2 = nullptr)
 {
     const hipEvent_t ev[2] = {g->opA, g->opB};
     const bool on = timed && g->timing;
-    launch_apply_cg1<T>(op, zold, qold, dold, znew, dnew, qnew, pend, red, g->stream, on ? ev : nullptr);
+    launch_apply_cg1<T>(op, zold, qold, dold, znew, dnew, qnew, pend, red, g->stream, on ? ev : nullptr, dold2);
     if (on) op_pending = true;
 }
 
@@ -603,8 +603,16 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
             const unsigned j = it + b;
             if (fused) {
                 if (cg1) {
-                    timed_apply_cg1<T>(op, zb[(j + 1) & 1], qb[(j + 1) & 1], j > 0 ? slot(j - 1) : nullptr, zb[j & 1],
-                                       slot(j), qb[j & 1], j > 0 ? 1 : 0, &rc, b == 0);
+                    // with 3 or more ring slots, iterations from the third on
+                    // recover z from the two previous directions, and only
+                    // the first writes z (the second reads it)
+                    const bool zr = W >= 3 && j >= 2;
+                    T *zw = W >= 3 && j >= 1 ? nullptr : zb[j & 1];
+                    // (the timed launch: the third of the batch, a steady
+                    // iteration)
+                    timed_apply_cg1<T>(op, zb[(j + 1) & 1], qb[(j + 1) & 1], j > 0 ? slot(j - 1) : nullptr, zw,
+                                       slot(j), qb[j & 1], j > 0 ? 1 : 0, &rc, b == (nb > 2 ? 2u : 0u),
+                                       zr ? slot(j - 2) : nullptr);
                 } else {
                     const T *dold = j > 0 ? slot(j - 1) : nullptr;
                     if (b == 0) timed_apply_dir<T>(op, r, dold, slot(j), q, &ra);
@@ -827,8 +835,14 @@ static int run_mixed_reliable(bool x0set, unsigned &it_total, unsigned &sweeps, 
                 const unsigned j = base + seg + b;
                 if (fused) {
                     if (cg1) {
+                        // z recovered from the directions from the segment's
+                        // third iteration on (3 or more ring slots); only the
+                        // segment's first iteration writes z
+                        const bool zr = W >= 3 && j >= base + 2;
+                        float *zw = W >= 3 && j >= base + 1 ? nullptr : zb[j & 1];
                         timed_apply_cg1<float>(o32, zb[(j + 1) & 1], qb[(j + 1) & 1], j > 0 ? slot(j - 1) : nullptr,
-                                               zb[j & 1], slot(j), qb[j & 1], j > base ? 1 : 0, &rc, b == 0);
+                                               zw, slot(j), qb[j & 1], j > base ? 1 : 0, &rc,
+                                               b == (nb > 2 ? 2u : 0u), zr ? slot(j - 2) : nullptr);
                     } else {
                         const float *dold = j > 0 ? slot(j - 1) : nullptr;
                         if (b == 0) timed_apply_dir<float>(o32, r, dold, slot(j), q, &ra);
@@ -1540,31 +1554,35 @@ double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps, cudapcgF
     if (dir) {
         // CG-loop form: r = x, directions alternate between d and s
         if (!g->precond || !apply_dir_ok(precision == CHFEM_PREC_MIXED ? g->dm.op32 : g->dm.op64)) return -1.0;
-        if (!set_scalar(S_DELTA_OLD, 1.0) || !set_scalar(S_DELTA, 0.5) || !set_scalar(S_ALPHA, 0.0)) return -1.0;
+        if (!set_scalar(S_DELTA_OLD, 1.0) || !set_scalar(S_DELTA, 0.5) || !set_scalar(S_ALPHA, 0.0) ||
+            !set_scalar(S_BPREV, 0.5))
+            return -1.0;
         if (precision == CHFEM_PREC_MIXED && !g->dinv32) return -1.0;
         launch_fill<double>(g->s64, 1.0, n, g->stream);
         launch_fill<double>(g->q64, 0.25, n, g->stream);
         launch_fill<double>(g->q2_64, 0.25, n, g->stream);
-        launch_fill<double>(g->z2_64, 0.5, n, g->stream);
+        launch_fill<double>(g->z2_64, 1.0, n, g->stream);
         if (precision == CHFEM_PREC_MIXED) {
             launch_fill<float>(g->s32, 1.0, n, g->stream);
             launch_fill<float>(g->q32, 0.25, n, g->stream);
             launch_fill<float>(g->q2_32, 0.25, n, g->stream);
-            launch_fill<float>(g->z2_32, 0.5, n, g->stream);
+            launch_fill<float>(g->z2_32, 1.0, n, g->stream);
         }
     }
     auto once = [&](unsigned k) {
         if (cg1) {
-            // z, q and d alternate between two buffers each (x / z2, q / q2,
-            // d / s); the deferred z update runs with S[ALPHA] = 0
+            // the solver's steady iteration: z recovered from the two
+            // previous directions (d rotates over d / s / z2), not written;
+            // q alternates between q / q2; the deferred z update runs with
+            // S[ALPHA] = 0
             if (precision == CHFEM_PREC_MIXED) {
-                float *zb[2] = {g->x32, g->z2_32}, *qb[2] = {g->q32, g->q2_32}, *db[2] = {g->d32, g->s32};
-                launch_apply_cg1<float>(g->dm.op32, zb[k & 1], qb[k & 1], db[k & 1], zb[(k + 1) & 1], db[(k + 1) & 1],
-                                        qb[(k + 1) & 1], 1, &red3, g->stream);
+                float *qb[2] = {g->q32, g->q2_32}, *db[3] = {g->d32, g->s32, g->z2_32};
+                launch_apply_cg1<float>(g->dm.op32, g->x32, qb[k & 1], db[(k + 1) % 3], nullptr, db[(k + 2) % 3],
+                                        qb[(k + 1) & 1], 1, &red3, g->stream, nullptr, db[k % 3]);
             } else {
-                double *zb[2] = {g->x64, g->z2_64}, *qb[2] = {g->q64, g->q2_64}, *db[2] = {g->d64, g->s64};
-                launch_apply_cg1<double>(g->dm.op64, zb[k & 1], qb[k & 1], db[k & 1], zb[(k + 1) & 1],
-                                         db[(k + 1) & 1], qb[(k + 1) & 1], 1, &red3, g->stream);
+                double *qb[2] = {g->q64, g->q2_64}, *db[3] = {g->d64, g->s64, g->z2_64};
+                launch_apply_cg1<double>(g->dm.op64, g->x64, qb[k & 1], db[(k + 1) % 3], nullptr, db[(k + 2) % 3],
+                                         qb[(k + 1) & 1], 1, &red3, g->stream, nullptr, db[k % 3]);
             }
             return;
         }

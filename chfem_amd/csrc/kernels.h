@@ -21,6 +21,9 @@ enum ScalarSlot {
     // guarded Krylov loops: done flag, iterations counted on the device,
     // stop threshold on r.z
     S_DONE = 16, S_ITER, S_THRESH,
+    // single-kernel CG: beta of the last finished iteration (the one that
+    // formed the newest direction)
+    S_BPREV,
     // alpha of guarded iteration j (j counted in S[S_ITER]) at
     // S[S_AHIST + j % S_AHIST_LEN], for the ring flush of the iterate
     S_AHIST = 24, S_AHIST_LEN = 32, S_COUNT = S_AHIST + S_AHIST_LEN
@@ -210,10 +213,14 @@ void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q
 // skipped unless pend), d = z + beta dold (dold == nullptr: d = z), znew <- z,
 // dnew <- d, q = A d with qnew <- dinv q, red (FIN_CG1) <- d.q, z.q,
 // q.dinv q, z.diag z.  The new buffers must not alias the old ones or each
-// other.
+// other.  znew == nullptr: z is not written.  dold2 non-null (with dold and
+// pend): zold is not read; z_old = dold - S[BPREV] dold2 (dold2: the
+// direction before dold, which the previous iteration updated to dold with
+// beta S[BPREV]).
 template <typename T>
 void launch_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *dold, T *znew, T *dnew, T *qnew,
-                      int pend, const Reduce *red, hipStream_t s, const hipEvent_t *ev = nullptr);
+                      int pend, const Reduce *red, hipStream_t s, const hipEvent_t *ev = nullptr,
+                      const T *dold2 = nullptr);
 // The iterate of the fused CG loops: iteration j writes its direction to
 // ring slot j % w; x += sum of S[S_AHIST + (j - base) % S_AHIST_LEN] *
 // ring[j % w] over the iterations j in [j0, j1) that ran (j - base <

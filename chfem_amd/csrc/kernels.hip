@@ -1170,6 +1170,10 @@ __global__ void __launch_bounds__(1024) k_reduce_finish(Reduce red, unsigned nb)
             // beta through one step only
             const double delta = s[3 % NV], alpha = delta / s[0];
             const double dnew = delta - 2.0 * alpha * s[1 % NV] + alpha * alpha * s[2 % NV];
+            // the beta this iteration's launch formed its direction with
+            // (the same quotient it computed), for the next launch that
+            // recovers z from the directions
+            S[S_BPREV] = S[S_DELTA] / S[S_DELTA_OLD];
             S[S_DQ] = s[0];
             S[S_ALPHA] = alpha;
             S[S_DELTA_OLD] = delta;

@@ -227,11 +227,17 @@ struct DirFuse {
     // from the packed keys of the owned nodes, kinv[key * ND + a]; the
     // diagonal itself at kinv[65536 * ND + key * ND + a]), the only form of
     // q the loop reads.  With zpend clear the z update is skipped (qt_old
-    // reads z)
+    // reads z).  znew == nullptr: z is not written.
     int zpend;
     const T *qold;
     T *znew;
     const T *kinv;
+    // zring: z_old is not read but recovered from the two previous
+    // directions, z_old = d_old - bprev d_old2 (the update that formed d_old,
+    // bprev = S[BPREV], recorded by that iteration's finishing kernel); z
+    // then points at d_old2
+    int zring;
+    T bprev;
 };
 
 // Sums a tile contributes to the launch's reductions: d.q, and for the
@@ -363,7 +369,8 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
 #pragma unroll
         for (int k = 0; k < SLOTS; k++) {
             if constexpr (CG1) {
-                const T zk = pz[k] - fz.alpha * pq[k];
+                const T zo = fz.zring ? pre[k] - fz.bprev * pz[k] : pz[k];
+                const T zk = zo - fz.alpha * pq[k];
                 dst[tid + k * NT] = zk + beta * pre[k];
                 zdst[tid + k * NT] = zk;
             } else if constexpr (FUSE) {
@@ -568,10 +575,13 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
                 for (int a = 0; a < ND; a++) dn[a] = own_u[a];
             }
             if constexpr (CG1) {
-                T *zn = fz.znew + on;
+                if (fz.znew) {
+                    T *zn = fz.znew + on;
+#pragma unroll
+                    for (int a = 0; a < ND; a++) zn[a] = own_z[a];
+                }
 #pragma unroll
                 for (int a = 0; a < ND; a++) {
-                    zn[a] = own_z[a];
                     zq += (double)own_z[a] * o[a];
                     qmq += (double)o[a] * (double)(mo[a] * o[a]);
                     zdz += (double)own_z[a] * (double)(dgo[a] * own_z[a]);

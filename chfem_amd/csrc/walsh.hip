@@ -18,6 +18,9 @@ __global__ void __launch_bounds__(TR *TC + (HALO ? 64 : 0)) k_walsh(const T *__r
     // single-kernel CG: alpha of the previous iteration for the deferred z
     // update
     if (FM == 2) fz.alpha = fz.zpend ? (T)red.S[S_ALPHA] : T(0);
+    // single-kernel CG with z recovered from the ring: the beta that formed
+    // d_old
+    if (FM == 2 && fz.zring) fz.bprev = (T)red.S[S_BPREV];
     // 1-D grid; blocks are dispatched to the 8 XCDs round-robin (block b to
     // XCD b % 8), so each XCD is given one contiguous range of tiles (rows of
     // tiles fastest): tiles that share halo nodes then run on the same XCD at
@@ -43,10 +46,11 @@ static void launch_walsh_nd(const OpDesc &op, const T *lamu, const T *v, T *q, c
 {
     Reduce rr = {nullptr, nullptr, 0, 0, 0};
     if (red) rr = *red;
-    DirFuse<T> f = {nullptr, nullptr, T(0), T(0), 1, 0, nullptr, nullptr, nullptr};
+    DirFuse<T> f = {nullptr, nullptr, T(0), T(0), 1, 0, nullptr, nullptr, nullptr, 0, T(0)};
     if (fz) f = *fz;
-    // single-kernel CG form: q_old, z_new and the dinv table given
-    const bool cg1 = fz && fz->znew;
+    // single-kernel CG form: the dinv table given (only launch_apply_cg1
+    // sets it; its z output may be absent)
+    const bool cg1 = fz && fz->kinv;
     int LZ = t.LZ > 0 ? t.LZ : 16;
     {
         // keep the block count within the reduction partials buffer
@@ -117,25 +121,30 @@ template <typename T>
 void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q, const Reduce *red, hipStream_t s,
                       const hipEvent_t *ev)
 {
-    const DirFuse<T> f = {z, dnew, T(0), T(0), dold ? 0 : 1, 0, nullptr, nullptr, nullptr};
+    const DirFuse<T> f = {z, dnew, T(0), T(0), dold ? 0 : 1, 0, nullptr, nullptr, nullptr, 0, T(0)};
     const WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
     launch_walsh<T>(op, (const T *)op.lamu, dold, q, red, t, s, &f, ev);
 }
 
 template <typename T>
 void launch_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *dold, T *znew, T *dnew, T *qnew,
-                      int pend, const Reduce *red, hipStream_t s, const hipEvent_t *ev)
+                      int pend, const Reduce *red, hipStream_t s, const hipEvent_t *ev, const T *dold2)
 {
     const int first = dold ? 0 : 1, p = dold && pend ? 1 : 0;
-    const DirFuse<T> f = {zold, dnew, T(0), T(0), first, p, p ? qold : zold, znew, (const T *)op.kinv};
+    // dold2: z_old recovered from the directions (needs d_old and the
+    // pending update)
+    const int zr = dold2 && p ? 1 : 0;
+    const DirFuse<T> f = {zr ? dold2 : zold, dnew, T(0), T(0), first, p, p ? qold : zold, znew, (const T *)op.kinv,
+                          zr, T(0)};
     const WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
     launch_walsh<T>(op, (const T *)op.lamu, dold, qnew, red, t, s, &f, ev);
 }
 
 template void launch_apply_cg1<float>(const OpDesc &, const float *, const float *, const float *, float *, float *,
-                                      float *, int, const Reduce *, hipStream_t, const hipEvent_t *);
+                                      float *, int, const Reduce *, hipStream_t, const hipEvent_t *, const float *);
 template void launch_apply_cg1<double>(const OpDesc &, const double *, const double *, const double *, double *,
-                                       double *, double *, int, const Reduce *, hipStream_t, const hipEvent_t *);
+                                       double *, double *, int, const Reduce *, hipStream_t, const hipEvent_t *,
+                                       const double *);
 
 template void launch_apply_dir<float>(const OpDesc &, const float *, const float *, float *, float *, const Reduce *,
                                       hipStream_t, const hipEvent_t *);

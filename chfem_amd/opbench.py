@@ -41,7 +41,8 @@ def main():
                     help="the CG loop's form: direction update d = z + beta d inside the operator (Walsh NBN)")
     ap.add_argument("--both", action="store_true", help="time the --dot form, then the --dir form")
     ap.add_argument("--cg1", action="store_true",
-                    help="also time the single-kernel CG form (z -= alpha M^-1 q inside the operator too)")
+                    help="also time the single-kernel CG form (z = d_old - beta' d_old2 - alpha M^-1 q inside the "
+                         "operator too)")
     a = ap.parse_args()
     n = a.size
     if a.physics == "fluid":
@@ -83,7 +84,9 @@ def main():
         else:
             w = 1 if a.physics == "thermal" else 3
             vb = (4 if prec == 1 else 8) * w
-            per = {"dir": BYTES_DIR[(a.physics, prec)], "cg1": 1.0 + 6 * vb}.get(form, BYTES[(a.physics, prec)])
+            # cg1: d_old2, qt_old, d_old read + d, qt written (z recovered
+            # from the two previous directions)
+            per = {"dir": BYTES_DIR[(a.physics, prec)], "cg1": 1.0 + 5 * vb}.get(form, BYTES[(a.physics, prec)])
             nbytes = per * N
         out[form] = {"avg_launch_ms": ms, "algorithmic_bytes": nbytes, "GBps": nbytes / (ms * 1e-3) / 1e9}
     L.hmgRelease()
