@@ -403,7 +403,9 @@ static void autotune_tiles()
     const int m0 = is_minres() ? 0 : (env >= 0 ? env : 1), m1 = is_minres() ? 0 : (env >= 0 ? env : 2);
     Reduce rt = make_reduce(FIN_STORE, S_TMP0);
     g->Sh[S_ALPHA] = 0.0;  // the deferred z updates of the timed launches change nothing
-    if (hipMemcpyAsync(g->S + S_ALPHA, g->Sh + S_ALPHA, sizeof(double), hipMemcpyHostToDevice, g->stream) != hipSuccess)
+    g->Sh[S_BPREV] = 0.0;
+    if (hipMemcpyAsync(g->S + S_ALPHA, g->Sh + S_ALPHA, sizeof(double), hipMemcpyHostToDevice, g->stream) != hipSuccess ||
+        hipMemcpyAsync(g->S + S_BPREV, g->Sh + S_BPREV, sizeof(double), hipMemcpyHostToDevice, g->stream) != hipSuccess)
         return;
     auto tune = [&](OpDesc &op, auto *d, auto *q, auto *d2, int *best, int &best_mode) {
         using T = typename std::remove_pointer<decltype(d)>::type;
@@ -416,10 +418,10 @@ static void autotune_tiles()
         launch_fill<T>(q2, 0.5, n, g->stream);
         auto run = [&](int mode) {
             if (fz && mode == 2) {
-                // with the deferred updates pending (alpha = 0): the launch
-                // streams q_old as in the loop; z is rewritten in place with
-                // the values it holds
-                launch_apply_cg1<T>(op, z2, q2, d, z2, d2, q, 1, &rt, g->stream);
+                // the loop's steady iteration, with the deferred updates
+                // pending (alpha = 0, S[BPREV] = 0): z recovered from the
+                // directions d and z2, q_old streamed, d2 and q written
+                launch_apply_cg1<T>(op, z2, q2, d, nullptr, d2, q, 1, &rt, g->stream, nullptr, (const T *)z2);
             } else if (fz && mode == 1) {
                 launch_apply_dir<T>(op, z2, d, d2, q, &rt, g->stream);
                 launch_kj_zupdate<T>(op, z2, q, &rt, g->stream);
