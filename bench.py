@@ -16,13 +16,18 @@ Methods timed in the same run, all at --tol and zero initial guess unless said:
             is trained before the timed region on --train-samples seeded
             64^3 images of the same recipe, solved by the library)
             (reported as `value`)
-Each is checked against C_ref, FP64 PCG at tolerance 1e-9 on the same image
-(untimed).  The operator kernel of the timed method is timed with HIP
+Each is checked against the CPU FP64 oracle's columns for this image
+(tests/golden/config_headline.json, matched by the image's SHA-256) and
+against C_ref, FP64 PCG at tolerance 1e-9 on the same image (untimed); a
+result over the metric's 1e-4 is marked invalid and exits non-zero.  The
+cold end-to-end time (hmgInit to C, autotuner included) is reported beside
+the resident-model step time.  The operator kernel of the timed method is timed with HIP
 events on the solver's stream (roofline); the CPU FP64 oracle is timed on a
 bounded sample of the same workload (cpu_baseline).
 """
 import argparse
 import ctypes as C
+import hashlib
 import json
 import os
 import sys
@@ -118,10 +123,22 @@ def train_guess(n_samples, size, epochs, seed):
     return net, info
 
 
-def cpu_baseline(img, n, gpu_iterations, iters_sample=3):
-    """CPU FP64 oracle (OpenMP, all host threads) on the same image: PCG
-    iterations of load case X timed, scaled by the FP64 GPU iteration count of
-    the full homogenization."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(img, n, gpu_iterations, iters_sample=50):
+    """CPU FP64 oracle (OpenMP, all host threads) on the same image: a
+    bounded sample of PCG iterations of load case X timed, scaled by the FP64
+    GPU iteration count of the full homogenization; and the 32^3 conductivity
+    workload solved fully on the CPU."""
     from oracle import oracle as O
     from chfem_amd import synthetic
     em = synthetic.elem_map(img, [0, 255])
@@ -135,10 +152,38 @@ def cpu_baseline(img, n, gpu_iterations, iters_sample=3):
     t2 = time.perf_counter()
     per_iter = ((t2 - t1) - (t1 - t0)) / iters_sample
     total_it = int(sum(gpu_iterations))
-    return {"value": per_iter * total_it, "unit": "s", "cores": int(O.lib().orc_num_threads()), "kind": "port",
+    threads = int(O.lib().orc_num_threads())
+    # the 32^3 thermal GRF workload (k 1 / 10, 3 load cases, tol 1e-6), whole
+    g = synthetic.grf((32, 32, 32), sigma=2.0, vf=0.4, seed=1)
+    t3 = time.perf_counter()
+    mc = O.Model(O.THERMAL, (32, 32, 32), synthetic.elem_map(g, [0, 255]), [1.0, 10.0])
+    Cc, info = mc.homogenize(tol=1e-6)
+    t4 = time.perf_counter()
+    return {"value": per_iter * total_it, "unit": "s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
             "sample": (f"oracle FP64 Jacobi-PCG (NBN) on the same {n}^3 image, load case X, {iters_sample} "
-                       f"iterations timed ({per_iter:.3f} s/iteration, model setup {setup:.1f} s), "
-                       f"x {total_it} iterations = FP64 GPU iteration count of the 6 load cases")}
+                       f"iterations timed ({per_iter:.4f} s/iteration on {threads} threads, model setup "
+                       f"{setup:.1f} s), x {total_it} iterations = FP64 GPU iteration count of the 6 load "
+                       f"cases at the same tolerance"),
+            "per_iteration_s": per_iter, "iterations_timed": iters_sample,
+            "c1_thermal_32": {"seconds": t4 - t3, "iterations": [info[d][1] for d in sorted(info)],
+                              "C": Cc.tolist(),
+                              "workload": "32^3 GRF (sigma 2, vf 0.4, seed 1), k 1/10, FP64 Jacobi-PCG NBN, "
+                                          "tol 1e-6, 3 load cases, solved fully on the CPU"}}
+
+
+def oracle_columns(n, seed, img_sha):
+    """Columns of C the CPU FP64 oracle solved for this image
+    (tests/golden/config_headline.json, written by
+    tests/golden/make_configs.py), if its image hash matches."""
+    p = os.path.join(ROOT, "tests", "golden", "config_headline.json")
+    if n != 256 or seed != 3 or not os.path.exists(p):
+        return {}, "no oracle fixture for this image"
+    with open(p) as f:
+        fx = json.load(f)
+    if fx.get("image_sha256") != img_sha:
+        return {}, "oracle fixture image hash differs: not used"
+    cols = {int(k): np.array(v["column"]) for k, v in fx.get("columns", {}).items()}
+    return cols, (f"oracle FP64 PCG at tol {fx.get('tol')}, load cases {sorted(cols)}, image hash matches")
 
 
 def read_traffic(kernel_key):
@@ -182,18 +227,38 @@ def main():
     net, net_info = train_guess(args.train_samples, args.train_size, args.train_epochs, args.seed)
     log(f"[bench] generating {n}^3 sphere image (seed {args.seed})")
     img = build_image(n, args.seed)
+    img_sha = hashlib.sha256(np.ascontiguousarray(img).tobytes()).hexdigest()
+    oracle_cols, oracle_note = oracle_columns(n, args.seed, img_sha)
     tmpd = tempfile.mkdtemp(prefix="chfem_bench_")
     nf = os.path.join(tmpd, f"spheres_{n}.raw")
     cio.export_for_chfem(nf, img, 1, {0: (1.0, 0.3), 255: (10.0, 0.2)}, voxel_size=1.0, export_raw=False,
                          solver_tolerance=args.tol, solver_maxiter=20000)
     nf = nf[:-4] + ".nf"
     flat = np.ascontiguousarray(img).reshape(-1)
-    if L.hmgInit(nf.encode(), None, None, ptr(flat)) != 1:
-        raise RuntimeError("hmgInit failed")
-    L.hmgSetSolverFlag(0)
-    L.hmgSetPreConditionerFlag(1)
-    L.hmgSetParallelStrategyFlag(0)
-    L.hmgSetHomogenizationFlag(6)
+
+    def init():
+        if L.hmgInit(nf.encode(), None, None, ptr(flat)) != 1:
+            raise RuntimeError("hmgInit failed")
+        L.hmgSetSolverFlag(0)
+        L.hmgSetPreConditionerFlag(1)
+        L.hmgSetParallelStrategyFlag(0)
+        L.hmgSetHomogenizationFlag(6)
+
+    # cold end to end: hmgInit (maps, local matrices) -> upload and solver
+    # allocation -> first homogenization (tile autotuner included) -> C,
+    # with the network already trained
+    cold = {}
+    for label, prec, nt in (("learned_mixed", 1, net), ("plain_fp64", 0, None)):
+        hip_sync()
+        t0 = time.perf_counter()
+        init()
+        try:
+            r = run_method(L, prec, args.tol, 1, 0, f"cold {label}", net=nt)
+        finally:
+            L.hmgEnd()
+        cold[label] = time.perf_counter() - t0
+        log(f"[bench] cold {label}: {cold[label]:.3f} s")
+    init()
     try:
         log("[bench] reference C: FP64 PCG, tol 1e-9")
         ref = run_method(L, 0, 1e-9, 1, 0, "reference")
@@ -208,6 +273,12 @@ def main():
 
     def err(Cm):
         return float(np.abs(Cm - Cref).max() / np.abs(Cref).max())
+
+    def err_oracle(Cm):
+        """max over the oracle's solved columns of max |C - C*| / max |C*|"""
+        if not oracle_cols:
+            return None
+        return float(max(np.abs(Cm[:, d] - ref).max() / np.abs(ref).max() for d, ref in oracle_cols.items()))
 
     N = n ** 3
     op_avg_ms = fast["op_ms_total"] / max(fast["op_launches"], 1)
@@ -259,15 +330,26 @@ def main():
         "config": {"workload": f"elastic_{n}^3_spheres_r6-12_f0.3_seed{args.seed}_E1-10_nu0.3-0.2_6loads",
                    "method": "learned initial guess + mixed (FP32 CG, FP64 reductions and reliable updates), "
                              "Jacobi, NBN",
-                   "solver_tolerance": args.tol, "reference": "FP64 PCG tol 1e-9"},
+                   "solver_tolerance": args.tol,
+                   "tolerance_note": ("PCG stops at sqrt(r.M^-1r / r0.M^-1r0) <= tol; the metric bounds the error "
+                                      "of C at 1e-4, which tol 1e-5 meets with a measured margin (err_vs_oracle, "
+                                      "err_vs_fp64); the plain and mixed methods run at the same tolerance"),
+                   "image_sha256": img_sha,
+                   "reference": "CPU FP64 oracle fixture (tests/golden/config_headline.json, tol 1e-8) on its "
+                                "columns; GPU FP64 PCG tol 1e-9 on all 36 entries"},
+        "err_vs_oracle": err_oracle(fast["C"]),
+        "oracle_fixture": oracle_note,
         "err_vs_fp64": err(fast["C"]),
+        "cold_seconds": cold,
         "plain_fp64": None if plain is None else {
-            "seconds": plain["seconds"], "err_vs_fp64": err(plain["C"]), "iterations": plain["iterations"],
+            "seconds": plain["seconds"], "err_vs_fp64": err(plain["C"]), "err_vs_oracle": err_oracle(plain["C"]),
+            "iterations": plain["iterations"],
             "op_avg_ms": plain["op_ms_total"] / max(plain["op_launches"], 1),
             "op_GBps": per_node64 * N / (plain["op_ms_total"] / max(plain["op_launches"], 1) * 1e-3) / 1e9,
             "loop_form": plain_form, "tile": plain["tile"]},
         "mixed": None if mixed is None else {
-            "seconds": mixed["seconds"], "err_vs_fp64": err(mixed["C"]), "iterations": mixed["iterations"],
+            "seconds": mixed["seconds"], "err_vs_fp64": err(mixed["C"]), "err_vs_oracle": err_oracle(mixed["C"]),
+            "iterations": mixed["iterations"],
             "sweeps": mixed["sweeps"]},
         "learned_mixed": {"seconds": fast["seconds"], "err_vs_fp64": err(fast["C"]), "iterations": fast["iterations"],
                           "sweeps": fast["sweeps"], "x0_residual": fast["x0_residual"], "x0_used": fast["x0_used"],
@@ -280,10 +362,16 @@ def main():
                      "launches": fast["op_launches"]},
         "cpu_baseline": cpu,
     }
+    # a result whose C misses the metric's bound is not a valid measurement
+    errs = [e for e in (out["err_vs_oracle"], out["err_vs_fp64"]) if e is not None]
+    out["valid"] = bool(errs) and max(errs) < 1e-4
     C.CDLL(None).fflush(None)
     sys.stdout.flush()
     os.dup2(json_fd, 1)
     print(json.dumps(out), flush=True)
+    if not out["valid"]:
+        print(f"bench.py: C misses the 1e-4 bound (errors {errs}); the measurement is invalid", file=sys.stderr)
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -388,6 +388,7 @@ static int make_ops()
 // solve uses and keeps the fastest.  Runs once per model, before its first
 // solve; CHFEM_WALSH_TILE=TRxTCxLZ fixes the tile instead.
 static bool is_minres();
+template <typename T> static Ring<T> dir_ring(T *d, T *s);
 static void autotune_tiles()
 {
     if (!g->autotune || g->tuned) return;
@@ -414,14 +415,22 @@ static void autotune_tiles()
         T *z2 = dbl ? (T *)g->z2_64 : (T *)g->z2_32;
         T *q2 = dbl ? (T *)g->q2_64 : (T *)g->q2_32;
         const bool fz = d2 && z2 && q2 && apply_dir_ok(op);
+        // the ring width decides the single-kernel form the loop runs: with
+        // 3 or more slots z is recovered from the two previous directions,
+        // with 2 it is stored and reloaded
+        const bool zrec = fz && dir_ring<T>(d, d2).w >= 3;
         launch_fill<T>(z2, 1.0, n, g->stream);
         launch_fill<T>(q2, 0.5, n, g->stream);
         auto run = [&](int mode) {
             if (fz && mode == 2) {
                 // the loop's steady iteration, with the deferred updates
                 // pending (alpha = 0, S[BPREV] = 0): z recovered from the
-                // directions d and z2, q_old streamed, d2 and q written
-                launch_apply_cg1<T>(op, z2, q2, d, nullptr, d2, q, 1, &rt, g->stream, nullptr, (const T *)z2);
+                // directions d and z2 (or, with a 2-slot ring, z2 read and
+                // rewritten), q_old streamed, d2 and q written
+                if (zrec) launch_apply_cg1<T>(op, z2, q2, d, nullptr, d2, q, 1, &rt, g->stream, nullptr, (const T *)z2);
+                // (z2 is read and written in place: with alpha = 0 the value
+                // written is the value read, bit for bit)
+                else launch_apply_cg1<T>(op, z2, q2, d, z2, d2, q, 1, &rt, g->stream, nullptr, (const T *)nullptr);
             } else if (fz && mode == 1) {
                 launch_apply_dir<T>(op, z2, d, d2, q, &rt, g->stream);
                 launch_kj_zupdate<T>(op, z2, q, &rt, g->stream);

@@ -58,6 +58,28 @@ static inline uint32_t pidx(long r, long c, long l, long ny, long nx, long nz)
     return (uint32_t)(r + c * ny + l * nx * ny);
 }
 
+
+/* The 27 periodic node indices around node/element i, nb[a + 3b + 9c] for
+ * row, column and layer offsets a-1, b-1, c-1 (PERIODICNUM_3D, utils.h:82,
+ * with the wrap done by comparison instead of a modulo per neighbour). */
+static inline void nbr27(size_t i, long ny, long nx, long nz, uint32_t nb[27])
+{
+    long r = (long)(i % (size_t)ny), c = (long)((i / (size_t)ny) % (size_t)nx), l = (long)(i / ((size_t)nx * ny));
+    long R[3] = {r == 0 ? ny - 1 : r - 1, r, r + 1 == ny ? 0 : r + 1};
+    long Cc[3] = {(c == 0 ? nx - 1 : c - 1) * ny, c * ny, (c + 1 == nx ? 0 : c + 1) * ny};
+    long Ll[3] = {(l == 0 ? nz - 1 : l - 1) * nx * ny, l * nx * ny, (l + 1 == nz ? 0 : l + 1) * nx * ny};
+    for (int cl = 0; cl < 3; cl++)
+        for (int b = 0; b < 3; b++)
+            for (int a = 0; a < 3; a++) nb[a + 3 * b + 9 * cl] = (uint32_t)(R[a] + Cc[b] + Ll[cl]);
+}
+/* offset slot of local node j of the element for which the centre node is
+ * local node e (element origin = node - (DR,DC,DL)[e]) */
+#define NB_SLOT(e, j) ((1 + DR[j] - DR[e]) + 3 * (1 + DC[j] - DC[e]) + 9 * (1 + DL[j] - DL[e]))
+/* slot of local node j of the element whose origin is the centre */
+#define NB_ELEM(j) ((1 + DR[j]) + 3 * (1 + DC[j]) + 9 * (1 + DL[j]))
+/* slot of the origin of the element for which the centre is local node e */
+#define NB_ORIG(e) ((1 - DR[e]) + 3 * (1 - DC[e]) + 9 * (1 - DL[e]))
+
 /* =========================================================================
  * A1: image -> element material map with colour keys, x<->y transpose and
  *     mesh refinement.  Restates readMaterialMapNumpy / readMaterialMapRAW
@@ -414,14 +436,14 @@ static void aprod_thermal_nbn(const orc_model *m, const double *v, double *q)
     const long ny = m->ny, nx = m->nx, nz = m->nz;
     #pragma omp parallel for schedule(static)
     for (long i = 0; i < (long)m->N; i++) {
-        long r = i % ny, c = (i / ny) % nx, l = i / (nx * ny);
+        uint32_t nb[27];
+        nbr27((size_t)i, ny, nx, nz, nb);
         unsigned key = m->keys[i];
         double res = 0.0;
         for (int e = 0; e < 8; e++) {
             const double *K = m->K + 64 * ((key >> (2 * e)) & 3u) + 8 * e;
-            long r0 = r - DR[e], c0 = c - DC[e], l0 = l - DL[e];
             for (int j = 0; j < 8; j++)
-                res += K[j] * v[pidx(r0 + DR[j], c0 + DC[j], l0 + DL[j], ny, nx, nz)];
+                res += K[j] * v[nb[NB_SLOT(e, j)]];
         }
         q[i] = res;
     }
@@ -432,14 +454,14 @@ static void aprod_elastic_nbn(const orc_model *m, const double *v, double *q)
     const long ny = m->ny, nx = m->nx, nz = m->nz;
     #pragma omp parallel for schedule(static)
     for (long i = 0; i < (long)m->N; i++) {
-        long r = i % ny, c = (i / ny) % nx, l = i / (nx * ny);
+        uint32_t nb[27];
+        nbr27((size_t)i, ny, nx, nz, nb);
         unsigned key = m->keys[i];
         double rx = 0, ry = 0, rz = 0;
         for (int e = 0; e < 8; e++) {
             const double *K = m->K + 576 * ((key >> (2 * e)) & 3u) + 72 * e;
-            long r0 = r - DR[e], c0 = c - DC[e], l0 = l - DL[e];
             for (int j = 0; j < 8; j++) {
-                size_t d = 3 * (size_t)pidx(r0 + DR[j], c0 + DC[j], l0 + DL[j], ny, nx, nz);
+                size_t d = 3 * (size_t)nb[NB_SLOT(e, j)];
                 double vx = v[d], vy = v[d + 1], vz = v[d + 2];
                 rx += K[3 * j] * vx + K[3 * j + 1] * vy + K[3 * j + 2] * vz;
                 ry += K[24 + 3 * j] * vx + K[24 + 3 * j + 1] * vy + K[24 + 3 * j + 2] * vz;
@@ -461,10 +483,11 @@ static void aprod_ebe(const orc_model *m, const double *v, double *q)
     double *fe = (double *)malloc(sizeof(double) * m->N * ke);
     #pragma omp parallel for schedule(static)
     for (long e = 0; e < (long)m->N; e++) {
-        long r = e % ny, c = (e / ny) % nx, l = e / (nx * ny);
+        uint32_t nb[27];
+        nbr27((size_t)e, ny, nx, nz, nb);
         double ue[24];
         for (int j = 0; j < 8; j++) {
-            size_t d = (size_t)nd * pidx(r + DR[j], c + DC[j], l + DL[j], ny, nx, nz);
+            size_t d = (size_t)nd * nb[NB_ELEM(j)];
             for (int a = 0; a < nd; a++) ue[nd * j + a] = v[d + a];
         }
         const double *K = m->K + (size_t)ksz * m->elem[e];
@@ -476,11 +499,12 @@ static void aprod_ebe(const orc_model *m, const double *v, double *q)
     }
     #pragma omp parallel for schedule(static)
     for (long i = 0; i < (long)m->N; i++) {
-        long r = i % ny, c = (i / ny) % nx, l = i / (nx * ny);
+        uint32_t nb[27];
+        nbr27((size_t)i, ny, nx, nz, nb);
         for (int a = 0; a < nd; a++) {
             double t = 0;
             for (int e = 0; e < 8; e++)
-                t += fe[(size_t)pidx(r - DR[e], c - DC[e], l - DL[e], ny, nx, nz) * ke + nd * e + a];
+                t += fe[(size_t)nb[NB_ORIG(e)] * ke + nd * e + a];
             q[(size_t)nd * i + a] = t;
         }
     }
@@ -495,15 +519,15 @@ static void aprod_fluid(const orc_model *m, const double *v, double *q)
     #pragma omp parallel for schedule(dynamic, 1024)
     for (long id = 0; id < (long)ntot; id++) {
         uint32_t i = m->dof2node[id];
-        long r = i % ny, c = (i / ny) % nx, l = i / (nx * ny);
+        uint32_t nbi[27];
+        nbr27((size_t)i, ny, nx, nz, nbi);
         int pore = id < (long)nvel;
         unsigned fkey = pore ? 255u : m->bkeys[id - nvel];
         double rx = 0, ry = 0, rz = 0, rp = 0;
         for (int e = 0; e < 8; e++) {
             if (!((fkey >> e) & 1u)) continue;
-            long r0 = r - DR[e], c0 = c - DC[e], l0 = l - DL[e];
             for (int j = 0; j < 8; j++) {
-                uint32_t nb = m->dof_id[pidx(r0 + DR[j], c0 + DC[j], l0 + DL[j], ny, nx, nz)];
+                uint32_t nb = m->dof_id[nbi[NB_SLOT(e, j)]];
                 if (nb < nvel) {
                     double vx = v[3 * (size_t)nb], vy = v[3 * (size_t)nb + 1], vz = v[3 * (size_t)nb + 2];
                     if (pore) {

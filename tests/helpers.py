@@ -104,3 +104,87 @@ class GpuSolver:
 
     def close(self):
         self.L.cudapcgEnd()
+
+
+class Resident:
+    """A model loaded through the reference's entry points the way
+    chfem.wrapper.run does (a .nf written by chfem_amd.io and a NumPy image
+    through hmgInit), with the solver flags set, then kept resident on the GPU
+    (hmgPrepare) so load cases can be solved one by one (hmgSolveLoadCase)."""
+
+    def __init__(self, img, analysis, mp, tol, precision=0, strategy=0, solver=None, maxit=20000, voxel=1.0):
+        import os
+        import tempfile
+        from chfem_amd import io as cio
+        from chfem_amd.lib import lib
+        self.L = lib()
+        self.analysis = analysis
+        self.dim = 6 if analysis == ELASTIC else 3
+        self._dir = tempfile.mkdtemp(prefix="chfem_res_")
+        nf = os.path.join(self._dir, "model.raw")
+        cio.export_for_chfem(nf, img, analysis, mp, voxel_size=voxel, export_raw=False, solver_tolerance=tol,
+                             solver_maxiter=maxit)
+        self.nf = nf[:-4] + ".nf"
+        self._flat = np.ascontiguousarray(img, dtype=np.uint8).reshape(-1)
+        L = self.L
+        assert L.hmgInit(self.nf.encode(), None, None, self._flat.ctypes.data_as(C.c_void_p)) == 1
+        self.open = True
+        L.hmgSetParallelStrategyFlag(strategy)
+        L.hmgSetSolverFlag(solver if solver is not None else (1 if analysis == FLUID else 0))
+        L.hmgSetPreConditionerFlag(1)
+        L.hmgSetPrecisionFlag(precision)
+        L.hmgSetSolverTolerance(tol)
+        L.hmgSetMaxIterationsOverride(maxit)
+        self.prepared = False
+
+    def sizes(self):
+        ne, nd, nv, nb = C.c_uint(), C.c_uint(), C.c_uint(), C.c_uint()
+        self.L.hmgGetSizes(C.byref(ne), C.byref(nd), C.byref(nv), C.byref(nb))
+        return ne.value, nd.value, nv.value, nb.value
+
+    def maps(self):
+        ne, nd, nv, nb = self.sizes()
+        keys = np.zeros(ne, np.uint16)
+        fk = np.zeros(ne, np.uint8)
+        ids = np.zeros(ne, np.uint32)
+        d2n = np.zeros(max(nv + nb, 1), np.uint32)
+        bk = np.zeros(max(nb, 1), np.uint8)
+        assert self.L.hmgGetMaps(*(a.ctypes.data_as(C.c_void_p) for a in (keys, fk, ids, d2n, bk))) == 1
+        return keys, fk, ids, d2n[:nv + nb], bk[:nb]
+
+    def solve(self, d, x0=None, want_x=False):
+        """(C column of load case d, solve statistics, converged[, solution])."""
+        from chfem_amd.lib import SolveStats
+        if not self.prepared:
+            assert self.L.hmgPrepare() == 1
+            self.prepared = True
+        col = np.zeros(self.dim)
+        ne, nd, _, _ = self.sizes()
+        x = np.zeros(nd) if want_x else None
+        x0p = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).ctypes.data_as(C.c_void_p)
+        ok = self.L.hmgSolveLoadCase(d, x0p, None if x is None else x.ctypes.data_as(C.c_void_p),
+                                     col.ctypes.data_as(C.c_void_p))
+        st = SolveStats()
+        self.L.cudapcgGetSolveStats(C.byref(st))
+        return (col, st, ok == 1, x) if want_x else (col, st, ok == 1)
+
+    def close(self):
+        import os
+        import shutil
+        if self.prepared:
+            self.L.hmgRelease()
+            self.prepared = False
+        if self.open:
+            self.L.hmgEnd()
+            self.open = False
+            self.L.hmgSetPrecisionFlag(0)
+            self.L.hmgSetParallelStrategyFlag(0)
+            self.L.hmgSetSolverFlag(0)
+        if os.path.isdir(self._dir):
+            shutil.rmtree(self._dir)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

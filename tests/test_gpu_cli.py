@@ -54,6 +54,73 @@ def test_cli_golden(tmp_path, case):
     assert rel_col(C, ref, d) < 1e-4
 
 
+def write_sdf_inputs(tmp_path, name):
+    """Unit-modulus .nf and a float32 field holding each voxel's modulus of
+    ggg40_100.raw (the reference's ggg40_100_<name>.bin carries exactly these
+    per-phase values)."""
+    from tests.test_gpu_homogenization import write_nf
+    img = np.fromfile(RAW, dtype=np.uint8)
+    lut = np.zeros(256, np.float32)
+    if name == "thermal":
+        lut[87], lut[182] = 129.0, 80.4
+        mats, an = [(87, 1.0), (182, 1.0)], 0
+    else:
+        lut[87], lut[182] = 39.7, 210.0
+        mats, an = [(87, 1.0, 0.2225), (182, 1.0, 0.3)], 1
+    field = str(tmp_path / "field.bin")
+    lut[img].tofile(field)
+    nf = str(tmp_path / "sdf.nf")
+    write_nf(nf, an, mats, (100, 100, 100))
+    return nf, field
+
+
+# The 3D cases of the reference's C harness (test/test_chfem.c:156-232):
+# thermal X and elastic Y, each NBN, EBE and density-field EBE, with one
+# level of recursive coarse initial guess (num_of_recursions_initguess = 1,
+# i.e. -r 1), compared at 1e-4 of max |C| (test_chfem.c:340-351).
+HARNESS = [
+    ("thermal", 0, ["-p", "0"]), ("thermal", 0, ["-p", "1"]), ("thermal", 0, ["sdf"]),
+    ("elastic", 1, ["-p", "0"]), ("elastic", 1, ["-p", "1"]), ("elastic", 1, ["sdf"]),
+]
+
+
+@pytest.mark.parametrize("case", HARNESS, ids=["thermal-nbn", "thermal-ebe", "thermal-sdf", "elastic-nbn",
+                                               "elastic-ebe", "elastic-sdf"])
+def test_cli_reference_harness_recursive_guess(tmp_path, case):
+    """-r 1: each load case is first solved on the 2x coarser mesh
+    (hmgFindInitialGuesses, femhmg.c:561-846), whose solution is projected
+    and interpolated onto the fine mesh as its initial guess (image.cu:19-121).
+    The density-field runs print the reference's warning and start from zero
+    (femhmg.c:572-577)."""
+    name, d, extra = case
+    dim = 3 if name == "thermal" else 6
+    out = str(tmp_path / "C.bin")
+    if extra == ["sdf"]:
+        nf, field = write_sdf_inputs(tmp_path, name)
+        log = run_cli(["-i", nf, RAW, "-f", field, "-d", str(d), "-r", "1", "-b", out], str(tmp_path))
+        assert "not available" in log and "coarse mesh" not in log
+    else:
+        nf, raw = copy_inputs(tmp_path, f"{name}_ggg40_100.nf", "ggg40_100.raw")
+        log = run_cli([nf, raw, "-d", str(d), "-r", "1", "-b", out] + extra, str(tmp_path))
+        assert "Computing initial guesses in coarse mesh 1" in log
+        its = [int(l.split("Solver ran")[1].split()[0]) for l in log.splitlines() if "Solver ran" in l]
+        # one coarse solve and one fine solve; the fine one starts from the coarse solution
+        assert len(its) >= 1
+    C = np.fromfile(out).reshape(dim, dim)
+    ref = np.fromfile(os.path.join(GOLDEN, f"{name}_ggg40_100_result.bin")).reshape(dim, dim)
+    assert rel_col(C, ref, d) < 1e-4
+
+
+def test_cli_elastic_ebe_golden(tmp_path):
+    """Elastic Y on the element-by-element operator (matrix cores), without
+    the recursive guess, against the golden column."""
+    nf, raw = copy_inputs(tmp_path, "elastic_ggg40_100.nf", "ggg40_100.raw")
+    out = str(tmp_path / "C.bin")
+    run_cli([nf, raw, "-d", "1", "-p", "1", "-b", out], str(tmp_path))
+    ref = np.fromfile(os.path.join(GOLDEN, "elastic_ggg40_100_result.bin")).reshape(6, 6)
+    assert rel_col(np.fromfile(out).reshape(6, 6), ref, 1) < 1e-4
+
+
 def test_cli_density_field(tmp_path):
     """-f: the thermal density-field run of test_chfem.c:63 (unit materials,
     float32 field holding each voxel's conductivity)."""

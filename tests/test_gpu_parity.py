@@ -122,10 +122,10 @@ def test_operator_fluid_tiles(shape, monkeypatch):
     monkeypatch.setenv("CHFEM_WALSH_TILE", "32x8x7")
     rng = np.random.default_rng(sum(shape))
     img = np.where(rng.random(shape) < 0.35, 255, 0).astype(np.uint8)
-    img[0, 0, 0] = 0
+    # the 8 elements around node (1, 1, 1) are pore, so it is fully fluid
+    img[:2, :2, :2] = 0
     m = H.model(H.FLUID, img, [0.0], colors=(0, 255))
-    if m.nvel == 0 or m.nvel + m.nborder < 3:
-        pytest.skip("pore space without a fully fluid node")
+    assert m.nvel > 0 and m.nborder > 0
     s = H.GpuSolver(m)
     try:
         v = rng.standard_normal(m.ndof)
@@ -325,15 +325,22 @@ def test_ebe_kernels(kernel, which, nphase, monkeypatch):
         m = H.model(H.ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25, 2.0, 0.35][:2 * nphase], colors=colors)
     v = rng.standard_normal(m.ndof)
     q_ref = m.apply(v, strategy=1)
-    for precision, tol in ((0, 1e-12), (1, 2e-5)):
-        s = H.GpuSolver(m, strategy=1, precision=precision)
+    v32 = v.astype(np.float32)
+    q_ref32 = m.apply(v32.astype(np.float64), strategy=1)
+    for precision in (0, 1):
+        s = H.GpuSolver(m, strategy=1, precision=precision, tol=1e-9)
         try:
             if precision == 0:
-                assert rel(s.apply(v), q_ref) < tol
+                assert rel(s.apply(v), q_ref) < 1e-12
             else:
-                b = m.rhs(0)
-                x, ok, st = s.solve(b)
-                assert ok
+                # FP32 operator on an FP32-representable input: FP32 rounding of
+                # sums over 8 elements
+                q32 = s.apply32(v32)
+                assert np.abs(q32 - q_ref32).max() < 1e-5 * np.abs(q_ref32).max()
+                x, ok, st = s.solve(m.rhs(0))
+                xr, okr, _, _ = m.solve(0, tol=1e-10, strategy=1)
+                assert ok and okr
+                assert rel(m.update_c(0, x), m.update_c(0, xr)) < 1e-7
         finally:
             s.close()
 
@@ -379,14 +386,17 @@ def test_fused_direction_update(fuse, precision, which, monkeypatch):
 
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("fuse", ["2", "1"])
-@pytest.mark.parametrize("ring", ["2", "5", "16"])
+@pytest.mark.parametrize("ring", ["2", "3", "4", "5", "16"])
 def test_direction_ring_widths(ring, fuse, precision, monkeypatch):
     """The fused CG loops keep their directions in a ring of CHFEM_RING
     buffers and add sum alpha_j d_j to the iterate once per ring window and
     at every loop or FP64-replacement end.  Windows that end inside a batch
     of queued iterations, at a replacement, or after the loop has converged
     (queued iterations that did not run) all give the oracle's solution
-    (widths 2 and 5 end windows inside the 16-iteration batches)."""
+    (widths 2 and 5 end windows inside the 16-iteration batches).  Width 3
+    is the smallest at which the single-kernel iteration recovers z from the
+    two previous directions: iteration j writes slot j mod 3 while it reads
+    slots j - 1 and j - 2, and every flush window is the three slots in use."""
     monkeypatch.setenv("CHFEM_FUSE_DIR", fuse)
     monkeypatch.setenv("CHFEM_RING", ring)
     monkeypatch.setenv("CHFEM_WALSH_TILE", "16x16x4")
@@ -433,5 +443,68 @@ def test_halo_tiles_fused_solve(which, tile, fuse, precision, monkeypatch):
         if precision == 0:
             assert rel(x, xr) < 1e-7
             assert abs(int(st.iterations) - itr) <= 2
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("which", ["thermal", "elastic"])
+@pytest.mark.parametrize("shape", [(8, 10, 12), (7, 9, 11)])
+def test_coarse_to_fine_interpolation(which, shape):
+    """cudapcgSetX0(x0, mustInterpolate=1) (k_project2 / k_interp) against
+    the oracle's restatement of image.cu:19-121 (orc_interpl2).  The coarse
+    vector is the oracle's solution on the 2x coarser mesh (the recursive
+    guess's input), so the interpolated guess beats the zero guess and a
+    solve at a loose tolerance returns it unchanged."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(sum(shape))
+    img = three_phase(shape, 71)
+    nz, ny, nx = shape
+    if which == "thermal":
+        m = H.model(H.THERMAL, img, [1.0, 10.0, 3.5], colors=(0, 100, 255))
+    else:
+        m = H.model(H.ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25], colors=(0, 100, 255))
+    nd = 1 if which == "thermal" else 3
+    ce = np.zeros((nx // 2) * (ny // 2) * (nz // 2), np.uint16)
+    O.lib().orc_coarse_elem_map(O._p(m.elem), nx, ny, nz, O._p(ce))
+    mc = O.Model(m.analysis, (nx // 2, ny // 2, nz // 2), ce, m.props[:2 * m.nmat] if nd == 3 else m.props[:m.nmat])
+    xc, okc, _, _ = mc.solve(0, tol=1e-10)
+    assert okc
+    fine = np.zeros(m.ndof)
+    O.lib().orc_interpl2(O._p(xc), ny, nx, nz, nd, O._p(fine))
+    s = H.GpuSolver(m, tol=1e3)
+    try:
+        assert s.L.cudapcgSetRHS(s.ptr(m.rhs(0))) == 1
+        assert s.L.cudapcgSetX0(s.ptr(xc), 1) == 1
+        x = np.zeros(m.ndof)
+        s.L.cudapcgSolve(s.ptr(x))
+        from chfem_amd.lib import SolveStats
+        import ctypes
+        st = SolveStats()
+        s.L.cudapcgGetSolveStats(ctypes.byref(st))
+        assert st.x0_used == 1 and st.iterations == 0
+        assert rel(x, fine) < 1e-14
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("solver", [4, 5], ids=["cg2", "minres2"])
+def test_reduced_velocity_solvers(porous, solver):
+    """CG2 / MINRES2 (two-vector solvers for permeability) return the
+    effective velocity of the load case instead of the solution vector: the
+    sum of each velocity component over the velocity DOFs times
+    voxel^2 / nelem (xreduce.cu:4-26, femhmg.c:922-923,1087-1093), which is
+    the column the effective-tensor integration gives from the full
+    solution."""
+    m = porous
+    xr, okr, _, _ = m.solve(2, solver=solver - 4, tol=1e-10)
+    assert okr
+    Cref = m.update_c(2, xr).reshape(3, 3)[:, 2]
+    s = H.GpuSolver(m, solver=solver, tol=1e-10)
+    try:
+        s.L.cudapcgSetXReduceScale(1.0 / m.N)
+        assert s.L.cudapcgSetRHS(s.ptr(m.rhs(2))) == 1
+        out = np.zeros(m.ndof)
+        assert s.L.cudapcgSolve(s.ptr(out)) == 1
+        assert rel(out[:3], Cref) < 1e-7
     finally:
         s.close()

@@ -458,6 +458,49 @@ def test_initial_guess_and_zero_rhs():
     assert okh
 
 
+@pytest.mark.parametrize("dims,nnodedof", [((5, 4, 3), 1), ((2, 7, 1), 3), ((1, 1, 1), 4), ((6, 6, 6), 3)])
+def test_node_dof_map(dims, nnodedof):
+    """Node (r, c, l) of the (ny+1)(nx+1)(nz+1) grid -> DOF.  Checked two
+    independent ways: the closed-form expression of assembleNodeDofMap_3D
+    (femhmg_3D.c:4-17, written with node counts m_nx = nx+1, m_ny = ny+1),
+    and the periodic identification node (r mod ny, c mod nx, l mod nz)
+    with index r + c*ny + l*nx*ny (PERIODICNUM_3D, utils.h:82)."""
+    nx, ny, nz = dims
+    NX, NY, NZ = nx + 1, ny + 1, nz + 1
+    out = np.zeros(NX * NY * NZ, dtype=np.uint32)
+    O.lib().orc_node_dof_map(nx, ny, nz, nnodedof, O._p(out))
+    n = np.arange(NX * NY * NZ, dtype=np.int64)
+    i = n % (NX * NY)
+    closed = ((i - i // NY - ny * ((i % NY) // ny)) % (nx * ny) + ((n // (NX * NY)) % nz) * nx * ny) * nnodedof
+    np.testing.assert_array_equal(out, closed)
+    r, c, l = n % NY, (n // NY) % NX, n // (NX * NY)
+    np.testing.assert_array_equal(out, ((r % ny) + (c % nx) * ny + (l % nz) * nx * ny) * nnodedof)
+    # interior nodes are numbered like elements; the last row/column/layer wraps to the first
+    grid = out.reshape(NZ, NX, NY)
+    np.testing.assert_array_equal(grid[-1], grid[0])
+    np.testing.assert_array_equal(grid[:, -1], grid[:, 0])
+    np.testing.assert_array_equal(grid[:, :, -1], grid[:, :, 0])
+    assert np.unique(grid[:-1, :-1, :-1]).size == nx * ny * nz
+
+
+@pytest.mark.parametrize("dims", [(8, 6, 4), (7, 5, 3), (2, 2, 2), (9, 4, 6)])
+def test_coarse_elem_map(dims):
+    """Coarse element map of the recursive initial guess: element (r, c, l)
+    of the coarse mesh takes element (2r, 2c, 2l) of the fine one, with
+    floor(n/2) coarse elements per axis (assemble_coarse_copy,
+    femhmg.c:1844-1916).  Checked against NumPy slicing of the element map
+    and against the element map of the image subsampled in file order."""
+    nx, ny, nz = dims
+    img = three_phase((nz, ny, nx), 11)
+    em = elem_of(img, (0, 100, 255))
+    out = np.zeros((nx // 2) * (ny // 2) * (nz // 2), dtype=np.uint16)
+    O.lib().orc_coarse_elem_map(O._p(em), nx, ny, nz, O._p(out))
+    grid = em.reshape(nz, nx, ny)
+    np.testing.assert_array_equal(out, grid[:2 * (nz // 2):2, :2 * (nx // 2):2, :2 * (ny // 2):2].reshape(-1))
+    sub = img[:2 * (nz // 2):2, :2 * (ny // 2):2, :2 * (nx // 2):2]
+    np.testing.assert_array_equal(out, elem_of(sub, (0, 100, 255)))
+
+
 def test_interpl2_linear_fields():
     """Coarse -> fine x0: nodal values doubled (the fine element is half the
     size, and the unknown is in element units), linear fields reproduced away
