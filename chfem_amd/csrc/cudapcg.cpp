@@ -975,7 +975,23 @@ static int run_mixed(bool x0set, unsigned &it_total, unsigned &sweeps, double &r
         if (!sync_scalars()) return -1;
         const double prev = delta;
         delta = g->Sh[S_DELTA];
-        if (it == 0 || !(fabs(delta) < fabs(prev))) break;  // no progress: stagnation in FP32
+        if (it == 0 || !(fabs(delta) < fabs(prev))) {
+            // FP32 sweeps no longer lower the FP64 residual (the saddle-point
+            // system at tight tolerances): finish in FP64 from the refined
+            // iterate.  r64 holds b - A x; b = r64 + A x restores the
+            // right-hand side the FP64 solver measures its residual against.
+            launch_apply<double>(o64, g->x64, g->q64, nullptr, st);
+            launch_add<double>(g->r64, g->q64, n, st);
+            unsigned it64 = 0;
+            double res64 = 0;
+            const int rc64 = run_minres<double>(o64, g->x64, g->r64, g->d64, g->q64, g->s64, g->dinv64, true, tol,
+                                                g->max_iterations - it_total, it64, res64);
+            if (rc64 < 0) return -1;
+            it_total += it64;
+            sweeps++;
+            residual = res64;
+            return rc64;
+        }
     }
     residual = sqrt(fabs(delta) / fabs(delta0));
     return !above(delta, delta0, tol);
