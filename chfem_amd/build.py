@@ -4,6 +4,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
@@ -21,8 +22,8 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
-SOURCES = ["kernels.hip", "walsh.hip", "stokes.hip", "ebe_mfma.hip", "batch.hip", "guess.hip", "cudapcg.cpp",
-           "femhmg.cpp", "guess.cpp"]
+SOURCES = ["kernels.hip", "walsh.hip", "stokes.hip", "ebe_mfma.hip", "batch.hip", "guess.hip", "noalloc.hip",
+           "cudapcg.cpp", "femhmg.cpp", "guess.cpp"]
 
 
 def _run(cmd, verbose):
@@ -36,14 +37,19 @@ def _run(cmd, verbose):
         print(out)
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for s in SOURCES:
         o = os.path.join(BUILD, s.replace(".", "_") + ".o")
         extra = ["-fopenmp"] if s.endswith(".cpp") else []
-        _run([HIPCC, "-x", "hip", "-c", os.path.join(SRC, s), "-o", o] + COMMON + extra, verbose)
+        cmds.append([HIPCC, "-x", "hip", "-c", os.path.join(SRC, s), "-o", o] + COMMON + extra)
         objs.append(o)
+    # one compiler process per source, at most one per CPU (16 at most)
+    jobs = jobs or max(1, min(len(cmds), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     _run([HIPCC, "-shared", "-o", LIB + ".tmp"] + objs + [f"--offload-arch={ARCH}", "-fopenmp"], verbose)
     os.replace(LIB + ".tmp", LIB)
     mo = os.path.join(BUILD, "main.o")

@@ -82,6 +82,8 @@ struct Solver {
     size_t fkeys_n = 0;
     // operator-launch timing
     bool timing = false;
+    // operator descriptors built (make_ops) for the current model
+    bool ops_ready = false;
     hipEvent_t opA = nullptr, opB = nullptr;
     double op_ms = 0;
     unsigned op_count = 0;
@@ -105,6 +107,7 @@ DeviceModel *active_model()
     if (g->model) make_ops();
     return &g->dm;
 }
+const DeviceModel *built_model() { return g && g->model && g->ops_ready ? &g->dm : nullptr; }
 hipStream_t active_stream() { return g ? g->stream : nullptr; }
 double *rhs_buffer() { return g ? g->r64 : nullptr; }
 double *solution_buffer() { return g ? g->x64 : nullptr; }
@@ -380,6 +383,7 @@ static int make_ops()
         g->dm.op32.tile_tr = g->tile32[0]; g->dm.op32.tile_tc = g->tile32[1]; g->dm.op32.tile_lz = g->tile32[2];
     }
     g->autotune = !fixed_tile && use_walsh && !o.ebe;
+    g->ops_ready = true;
     return 1;
 }
 
@@ -1254,6 +1258,7 @@ cudapcgFlag_t cudapcgSetModel(cudapcgModel_t *model)
     }
     g->model = model;
     g->tuned = false;
+    g->ops_ready = false;
     return alloc_model_device();
 }
 

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "batch.h"
+#include "noalloc.h"
 #include "guess.h"
 #include "solver.h"
 #include "voxel.h"
@@ -553,17 +554,25 @@ void free_aux(DeviceAux &a)
     a = DeviceAux();
 }
 
-bool set_seam(const Model &m, const DeviceAux &a, const LoadCase &lc)
+// seam term of the C column per material and row (6 * nmat entries): the sum
+// of CB over the load case's seam columns, times the jump
+void seam_table(const Model &m, const LoadCase &lc, double *seam)
 {
-    if (m.analysis == HMG_FLUID) return true;
     const int nd = m.analysis == HMG_ELASTIC ? 3 : 1, ke = 8 * nd, nr = nd == 3 ? 6 : 3;
-    std::vector<double> seam(6 * m.nmat, 0.0);
+    for (size_t i = 0; i < 6 * (size_t)m.nmat; i++) seam[i] = 0.0;
     for (unsigned t = 0; t < m.nmat; t++)
         for (int row = 0; row < nr; row++) {
             double s = 0;
             for (int k = 0; k < 4; k++) s += m.CB[(size_t)t * nr * ke + row * ke + nd * lc.S[k] + lc.comp];
             seam[t * 6 + row] = s * lc.L;
         }
+}
+
+bool set_seam(const Model &m, const DeviceAux &a, const LoadCase &lc)
+{
+    if (m.analysis == HMG_FLUID) return true;
+    std::vector<double> seam(6 * m.nmat, 0.0);
+    seam_table(m, lc, seam.data());
     return hipMemcpy(a.seam, seam.data(), seam.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
 }
 
@@ -1043,6 +1052,12 @@ logical hmgPrepare(void)
     }
     if (!solver_setup(*M, g_aux)) return HMG_FALSE;
     g_prepared = true;
+    // set-up for the allocation-free entry point: operator descriptors
+    // built, and the scratch array's device address looked up on this device
+    if (!active_model() || !noalloc_scratch()) {
+        hmgRelease();
+        return HMG_FALSE;
+    }
     return HMG_TRUE;
 }
 
@@ -1341,6 +1356,61 @@ logical hmgUpdateC(unsigned int dir, const cudapcgVar_t *x, var *C)
     free_aux(aux);
     cudapcgEnd();
     return ok ? HMG_TRUE : HMG_FALSE;
+}
+
+// Allocation-free variant of hmgUpdateC for the model hmgPrepare made
+// resident: the C column of load case dir from a device solution, with the
+// reduction partials, sums and seam table in the static scratch array of
+// noalloc.hip.  Same kernels and reduction order as hmgUpdateC.
+logical hmgUpdateCNoAlloc(unsigned int dir, const double *x_dev, double *Ccol_dev)
+{
+    if (!M || !g_prepared) {
+        fprintf(stderr, "chfem: hmgUpdateCNoAlloc needs a model made resident by hmgPrepare\n");
+        return HMG_FALSE;
+    }
+    Model &m = *M;
+    const unsigned nr = m.analysis == HMG_ELASTIC ? 6u : 3u;
+    if (!x_dev || !Ccol_dev || dir >= nr) {
+        fprintf(stderr, "chfem: hmgUpdateCNoAlloc: null vector or load case %u out of range\n", dir);
+        return HMG_FALSE;
+    }
+    if (m.analysis != HMG_FLUID && m.nmat > (unsigned)NOALLOC_MAX_MAT) {
+        fprintf(stderr, "chfem: hmgUpdateCNoAlloc: %u materials exceed the scratch limit of %d\n", m.nmat,
+                NOALLOC_MAX_MAT);
+        return HMG_FALSE;
+    }
+    const DeviceModel *dm = built_model();
+    double *sc = noalloc_scratch();
+    if (!dm || !sc) return HMG_FALSE;
+    const int nb = reduce_blocks(m.analysis == HMG_FLUID ? std::max(m.nvel, 1u) : m.nelem);
+    if ((size_t)nb * 6 > NOALLOC_PARTIALS) {
+        fprintf(stderr, "chfem: hmgUpdateCNoAlloc: %d reduction blocks exceed the scratch limit\n", nb);
+        return HMG_FALSE;
+    }
+    Reduce red;
+    red.partials = sc;
+    red.S = sc + NOALLOC_SUMS_OFF;
+    red.fin = FIN_STORE;
+    red.slot = 0;
+    red.guard = 0;
+    hipStream_t st = active_stream();
+    if (m.analysis == HMG_FLUID) {
+        launch_fluid_velocity_sum<double>(dm->op64, x_dev, &red, st);
+        launch_noalloc_column(red.S, Ccol_dev, 3, m.elem_size * m.elem_size / (double)m.nelem, 0, st);
+    } else {
+        const LoadCase lc = load_case(m, dir);
+        double seam[6 * NOALLOC_MAX_MAT];
+        seam_table(m, lc, seam);
+        launch_noalloc_seam(sc + NOALLOC_SEAM_OFF, seam, 6 * (int)m.nmat, st);
+        launch_updateC(dm->op64, g_aux.CB, sc + NOALLOC_SEAM_OFF, lc, x_dev, &red, st);
+        launch_noalloc_column(red.S, Ccol_dev, (int)nr, (double)m.nelem, 1, st);
+    }
+    const hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        fprintf(stderr, "chfem: hmgUpdateCNoAlloc: %s\n", hipGetErrorString(e));
+        return HMG_FALSE;
+    }
+    return HMG_TRUE;
 }
 
 // chfem/src/main.c:300-323

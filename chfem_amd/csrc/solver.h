@@ -28,6 +28,9 @@ struct DeviceModel {
 
 // Accessors used by the model layer.
 DeviceModel *active_model();
+// the operator descriptors as last built, without rebuilding them (nullptr
+// before the first build)
+const DeviceModel *built_model();
 hipStream_t active_stream();
 // device FP64 vector holding the right-hand side of the next solve
 double *rhs_buffer();

@@ -98,6 +98,7 @@ SIGNATURES = [
     ("cudapcgApplyPreconditioner", u8, [vp, vp]),
     ("hmgAssembleRHS", u8, [uint, vp]),
     ("hmgUpdateC", u8, [uint, vp, vp]),
+    ("hmgUpdateCNoAlloc", u8, [uint, vp, vp]),
     ("hmgGetMaps", u8, [vp, vp, vp, vp, vp]),
     ("hmgGetLocalMatrices", u8, [vp, vp]),
     ("hmgGetSizes", u8, [C.POINTER(uint), C.POINTER(uint), C.POINTER(uint), C.POINTER(uint)]),

@@ -69,6 +69,21 @@ cudapcgFlag_t cudapcgApplyPreconditioner(const cudapcgVar_t *v, cudapcgVar_t *z)
 logical hmgAssembleRHS(unsigned int dir, cudapcgVar_t *b);
 logical hmgUpdateC(unsigned int dir, const cudapcgVar_t *x, var *C);
 
+/* Allocation-free variant of hmgUpdateC.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  For the model hmgPrepare made resident, it
+ * writes column dir of C (3 or 6 doubles, the values hmgUpdateC gives) from
+ * the device solution x_dev (ndof doubles) to the device array Ccol_dev, and
+ * returns 1 when the column is there.  It calls no allocator: its reduction
+ * partials, sums and seam table live in one static device array per GPU
+ * (49,600 doubles, 387.5 KiB, held until the process ends; its address is
+ * looked up once per device by hmgPrepare).  Limit: at most 64 materials
+ * (thermal/elastic); larger models are rejected with a message.  It runs on
+ * the library's stream and synchronises it before returning; x_dev must be
+ * complete before the call.  Calls that overlap on one GPU race on the
+ * shared array. */
+logical hmgUpdateCNoAlloc(unsigned int dir, const double *x_dev, double *Ccol_dev);
+
 /* Model maps as built by hmgInit (host copies): NBN keys (uint16 per node),
  * or for Stokes the node fluid keys, node->DOF ids, DOF->node map and border
  * keys.  Any pointer may be NULL. */
