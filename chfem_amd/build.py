@@ -22,7 +22,7 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
-SOURCES = ["kernels.hip", "walsh.hip", "stokes.hip", "ebe_mfma.hip", "batch.hip", "guess.hip", "noalloc.hip",
+SOURCES = ["kernels.hip", "walsh.hip", "stokes.hip", "ebe_mfma.hip", "batch.hip", "guess.hip", "noalloc.hip", "lean.hip",
            "cudapcg.cpp", "femhmg.cpp", "guess.cpp"]
 
 

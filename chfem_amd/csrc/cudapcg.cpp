@@ -84,6 +84,19 @@ struct Solver {
     bool timing = false;
     // operator descriptors built (make_ops) for the current model
     bool ops_ready = false;
+    // Stokes Jacobi per DOF class for the memory-lean solvers: velocity and
+    // pore-pressure inverse diagonals, and the border-pressure inverse per
+    // fluid key (256 entries, device)
+    double fl_vinv = 1.0, fl_pinv = 1.0;
+    double *btab64 = nullptr;
+    // two-vector solvers: the reduced solution (xreduce FULL: nhmgvars
+    // entries, ONLYDIR: nhmgvars / nvarspernode) and its per-iteration
+    // history ((max_iterations + 1) x nvarspernode), and the values the last
+    // solve returns
+    double *xr64 = nullptr, *hist = nullptr;
+    size_t xr_n = 0, hist_n = 0;
+    double xred_out[4] = {0, 0, 0, 0};
+    int xred_count = 0;
     hipEvent_t opA = nullptr, opB = nullptr;
     double op_ms = 0;
     unsigned op_count = 0;
@@ -108,6 +121,13 @@ DeviceModel *active_model()
     return &g->dm;
 }
 const DeviceModel *built_model() { return g && g->model && g->ops_ready ? &g->dm : nullptr; }
+static bool is_two_vector();
+int reduced_solution(double *out)
+{
+    if (!g || !is_two_vector() || g->xred_count == 0) return 0;
+    for (int a = 0; a < g->xred_count; a++) out[a] = g->xred_out[a];
+    return g->xred_count;
+}
 hipStream_t active_stream() { return g ? g->stream : nullptr; }
 double *rhs_buffer() { return g ? g->r64 : nullptr; }
 double *solution_buffer() { return g ? g->x64 : nullptr; }
@@ -256,6 +276,8 @@ static cudapcgFlag_t free_solver_arrays()
     dfree(g->d64); dfree(g->q64); dfree(g->s64); dfree(g->dinv64);
     dfree(g->x32); dfree(g->r32); dfree(g->d32); dfree(g->q32); dfree(g->s32); dfree(g->dinv32);
     dfree(g->z2_64); dfree(g->q2_64); dfree(g->z2_32); dfree(g->q2_32);
+    dfree(g->xr64); dfree(g->hist);
+    g->xr_n = g->hist_n = 0;
     for (int k = 0; k < RING_MAX; k++) { dfree(g->ring64[k]); dfree(g->ring32[k]); }
     g->ring_w64 = g->ring_w32 = 0;
     dfree(g->partials); dfree(g->S);
@@ -267,11 +289,20 @@ static cudapcgFlag_t free_solver_arrays()
     return CUDAPCG_TRUE;
 }
 
+static bool is_lean();
+static bool is_two_vector();
 static cudapcgFlag_t alloc_solver_arrays()
 {
     if (!g || !g->model) return CUDAPCG_FALSE;
     const size_t n = nvars();
     const size_t b8 = n * sizeof(double), b4 = n * sizeof(float);
+    if (is_lean()) {
+        // memory-lean solvers: z (the RHS buffer) and d, plus x for CG3 and
+        // MINRES3; a stored diagonal or an operator work vector only where
+        // the model needs one (allocated by the solver on first use)
+        if (!is_two_vector() && !g->x64) CHECK_HIP(hipMalloc(&g->x64, b8));
+        if (!g->d64) CHECK_HIP(hipMalloc(&g->d64, b8));
+    } else {
     if (!g->x64) CHECK_HIP(hipMalloc(&g->x64, b8));
     if (!g->d64) CHECK_HIP(hipMalloc(&g->d64, b8));
     if (!g->q64) CHECK_HIP(hipMalloc(&g->q64, b8));
@@ -289,6 +320,7 @@ static cudapcgFlag_t alloc_solver_arrays()
         if (!g->z2_32) CHECK_HIP(hipMalloc(&g->z2_32, b4));
         if (!g->q2_32) CHECK_HIP(hipMalloc(&g->q2_32, b4));
         if (!g->dinv32) CHECK_HIP(hipMalloc(&g->dinv32, b4));
+    }
     }
     const size_t pn = (size_t)MAX_PARTIAL_BLOCKS * 8;
     if (!g->partials) CHECK_HIP(hipMalloc(&g->partials, pn * sizeof(double)));
@@ -661,6 +693,77 @@ static int run_pcg(const OpDesc &op, T *x, T *r, T *d, T *q, const T *dinv, bool
     return !above(delta, delta0, tol);
 }
 
+// Jacobi-PCG (FP64) with the INF or error stopping rule (pcg.cu:28-243, the
+// switch at :130-146 and :176-190): separate direction, operator and update
+// kernels with the stored diagonal, the rule's metric evaluated after every
+// iteration.  INF: max|r| / max|b| <= tol.  Error: |alpha| max|d| / max|x|
+// over the first nhmgvars entries <= tol, or the L2 rule met.
+static int absmax_of(const double *v, size_t n, const Jac<double> *scaled, int slot);
+static size_t hmg_vars();
+static int run_pcg_rule(bool x0set, unsigned &it, double &residual)
+{
+    const OpDesc &op = g->dm.op64;
+    const size_t n = op.ndof, nh = hmg_vars();
+    hipStream_t st = g->stream;
+    const double tol = g->num_tol;
+    const bool inf = g->resnorm == CUDAPCG_INF_NORM;
+    double *x = g->x64, *r = g->r64, *d = g->d64, *q = g->q64;
+    const double *dinv = g->dinv64;
+    double res0_inf = 0;
+    if (inf) {
+        if (!absmax_of(r, n, nullptr, S_TMP2)) return -1;
+        res0_inf = g->Sh[S_TMP2];
+    }
+    Reduce r0 = make_reduce(FIN_STORE, S_DELTA0);
+    launch_precond<double>(dinv, r, d, n, &r0, st);
+    if (!sync_scalars()) return -1;
+    const double delta0 = g->Sh[S_DELTA0];
+    const double res0 = inf ? res0_inf : delta0;
+    it = 0;
+    residual = 0;
+    if (fabs(res0) < 1e-12) {
+        launch_fill<double>(x, 0.0, n, st);
+        return 1;
+    }
+    double delta = delta0;
+    if (x0set) {
+        launch_apply<double>(op, x, q, nullptr, st);
+        Reduce rd = make_reduce(FIN_STORE, S_DELTA);
+        launch_sub_precond<double>(r, q, dinv, d, &rd, n, st);
+        if (!sync_scalars()) return -1;
+        delta = g->Sh[S_DELTA];
+        note_x0(delta, delta0);
+        if (!above(delta, delta0, tol)) {
+            residual = sqrt(fabs(delta) / fabs(delta0));
+            return 1;
+        }
+    } else {
+        launch_fill<double>(x, 0.0, n, st);
+    }
+    if (!set_scalar(S_DELTA, delta) || !arm_guard(-1.0)) return -1;
+    Reduce ra = make_reduce(FIN_ALPHA, 0), rdl = make_reduce(FIN_DELTA, 0);
+    double metric = 1.0;
+    bool done = false;
+    while (!done && it < g->max_iterations) {
+        if (it > 0) launch_pcg_dir<double>(d, q, g->S, 0, n, st);
+        timed_apply<double>(op, d, q, &ra);
+        launch_pcg_update<double>(x, r, d, q, dinv, &rdl, n, st);
+        if (!sync_loop()) return -1;
+        it++;
+        if (inf) {
+            if (!absmax_of(r, n, nullptr, S_TMP2)) return -1;
+            metric = g->Sh[S_TMP2] / res0;
+            done = !(metric > tol);
+        } else {
+            if (!absmax_of(d, nh, nullptr, S_TMP3) || !absmax_of(x, nh, nullptr, S_TMP4)) return -1;
+            metric = fabs(g->Sh[S_ALPHA]) * g->Sh[S_TMP3] / g->Sh[S_TMP4];
+            done = metric <= tol || !above(g->Sh[S_DELTA], delta0, tol);
+        }
+    }
+    residual = metric;
+    return metric <= tol;
+}
+
 // Jacobi-MINRES in precision T (pminres.cu:28-247).  r holds b on entry.
 // stagnation > 0 (the FP32 inner solves of mixed precision): the loop also
 // ends once the residual measure has not fallen below 0.9 of its best value
@@ -673,12 +776,43 @@ static int run_minres(const OpDesc &op, T *x, T *r, T *d, T *q, T *s, const T *d
 {
     const size_t n = op.ndof;
     hipStream_t st = g->stream;
+    // INF and error rules (pminres.cu:188-200): FP64 solves only (the FP32
+    // inner solves of mixed precision stop on the L2 measure)
+    constexpr bool dbl = std::is_same<T, double>::value;
+    const cudapcgFlag_t rule = dbl && !stagnation ? g->resnorm : (cudapcgFlag_t)CUDAPCG_L2_NORM;
+    double res0_inf = 0;
+    if constexpr (dbl) {
+        if (rule == CUDAPCG_INF_NORM) {
+            if (!absmax_of(r, n, nullptr, S_TMP2)) return -1;
+            res0_inf = g->Sh[S_TMP2];
+        }
+    }
     if (!x0set) launch_fill<T>(x, 0.0, n, st);
     Reduce rr0 = make_reduce(FIN_STORE, S_RES0), rd0 = make_reduce(FIN_STORE, S_DELTA0);
     launch_precond<T>(dinv, r, q, n, &rr0, st);        // q = M^-1 r ; res0 = r.q
     launch_apply<T>(op, q, s, &rd0, st);               // s = A q ; delta0 = q.s
     if (!sync_scalars()) return -1;
-    const double res0 = g->Sh[S_RES0], delta0 = g->Sh[S_DELTA0];
+    const double res0 = rule == CUDAPCG_INF_NORM ? res0_inf : g->Sh[S_RES0], delta0 = g->Sh[S_DELTA0];
+    // the rule's metric after an iteration (r holds M^-1 r by then)
+    auto rule_metric = [&](double &m) -> bool {
+        if constexpr (dbl) {
+            if (rule == CUDAPCG_INF_NORM) {
+                Jac<double> jd;
+                memset(&jd, 0, sizeof(jd));
+                jd.kind = JAC_STORED;
+                jd.dinv = (const double *)dinv;
+                if (!absmax_of((const double *)r, n, &jd, S_TMP2)) return false;
+                m = g->Sh[S_TMP2] / res0;
+            } else {
+                const size_t nh = hmg_vars();
+                if (!absmax_of((const double *)d, nh, nullptr, S_TMP3) ||
+                    !absmax_of((const double *)x, nh, nullptr, S_TMP4))
+                    return false;
+                m = fabs(g->Sh[S_ALPHA]) * g->Sh[S_TMP3] / g->Sh[S_TMP4];
+            }
+        }
+        return true;
+    };
     it = 0;
     residual = 0;
     if (fabs(res0) < 1e-12) {
@@ -708,8 +842,12 @@ static int run_minres(const OpDesc &op, T *x, T *r, T *d, T *q, T *s, const T *d
     launch_minres_update<T>(x, r, d, q, dinv, &rs, n, st);
     timed_apply<T>(op, r, s, &rdl);
     if (!sync_loop()) return -1;
-    double stop = g->Sh[S_STOP];
+    double stop = g->Sh[S_STOP], metric = 1.0;
     bool cont = above(stop, res0, tol);
+    if (rule != CUDAPCG_L2_NORM) {
+        if (!rule_metric(metric)) return -1;
+        cont = metric > tol;
+    }
     double best = fabs(stop);
     unsigned best_it = it;
     while (cont && it < maxit) {
@@ -720,6 +858,10 @@ static int run_minres(const OpDesc &op, T *x, T *r, T *d, T *q, T *s, const T *d
         if (!sync_loop()) return -1;
         stop = g->Sh[S_STOP];
         cont = above(stop, res0, tol);
+        if (rule != CUDAPCG_L2_NORM) {
+            if (!rule_metric(metric)) return -1;
+            cont = metric > tol;
+        }
         if (stagnation) {
             if (fabs(stop) < 0.9 * best) {
                 best = fabs(stop);
@@ -729,7 +871,7 @@ static int run_minres(const OpDesc &op, T *x, T *r, T *d, T *q, T *s, const T *d
             }
         }
     }
-    residual = sqrt(fabs(stop) / fabs(res0));
+    residual = rule != CUDAPCG_L2_NORM ? metric : sqrt(fabs(stop) / fabs(res0));
     return residual <= tol;
 }
 
@@ -739,10 +881,302 @@ static bool is_minres()
            g->solver_flag == CUDAPCG_MINRES2_SOLVER;
 }
 
+// ---------------------------------------------------------------------------
+// Memory-lean solvers: CG3 / MINRES3 keep x, z and d (pcg3.cu, pminres3.cu),
+// CG2 / MINRES2 keep z and d and reduce the solution on the fly (pcg2.cu,
+// pminres2.cu, xreduce.cu).  The same iterations as CG and MINRES, with A d
+// applied again instead of stored: CG3 and CG2 apply the operator twice per
+// iteration (d.Ad, then z -= alpha M^-1 A d), MINRES3 and MINRES2 three
+// times (Ad.M^-1 Ad, the z update, z.Az).  They iterate on z = M^-1 r, so
+// the Jacobi diagonal meets only operator results; the Walsh-basis NBN and
+// Stokes operators fold each form into their epilogue, other operators go
+// through one work vector.  FP64 in every precision mode.
+// ---------------------------------------------------------------------------
+static bool is_lean()
+{
+    return g->solver_flag >= CUDAPCG_CG3_SOLVER && g->solver_flag <= CUDAPCG_MINRES2_SOLVER;
+}
+static bool is_two_vector() { return g->solver_flag >= CUDAPCG_CG2_SOLVER && g->solver_flag <= CUDAPCG_MINRES2_SOLVER; }
+
+// entries the error rule and the on-the-fly reduction cover (nhmgvars: the
+// velocity DOFs for Stokes), and values per node of the reduction
+static size_t hmg_vars()
+{
+    const size_t n = nvars(), h = g->model->nhmgvars;
+    if (h && h <= n) return h;
+    // unset: the velocity DOFs for Stokes, every DOF otherwise (femhmg.c:82)
+    return g->analysis_flag == CUDAPCG_FLUID_3D ? 3 * (size_t)g->model->nporenodes : n;
+}
+static int vars_per_node()
+{
+    const unsigned p = g->model->nvarspernode;
+    return p < 1 ? 1 : (p > 4 ? 4 : (int)p);
+}
+
+static Jac<double> make_jac()
+{
+    Jac<double> j;
+    memset(&j, 0, sizeof(j));
+    const OpDesc &op = g->dm.op64;
+    j.nd = op.physics == PHY_ELASTIC ? 3 : 1;
+    j.vinv = j.pinv = 1.0;
+    if (!g->precond) {
+        j.kind = JAC_NONE;
+    } else if (op.physics == PHY_FLUID) {
+        j.kind = JAC_FLUID;
+        j.vinv = g->fl_vinv;
+        j.pinv = g->fl_pinv;
+        j.btab = g->btab64;
+        j.bkeys = op.bkeys;
+        j.nvel = op.nvel;
+    } else if (g->kinv64 && !op.ebe) {
+        j.kind = JAC_KEYS;
+        j.kinv = g->kinv64;
+        j.keys = op.keys;
+    } else {
+        j.kind = JAC_STORED;
+    }
+    return j;
+}
+
+static int ensure_dinv();
+
+// absolute maxima of the INF and error rules into S[slot]
+static int absmax_of(const double *v, size_t n, const Jac<double> *scaled, int slot)
+{
+    launch_absmax<double>(v, n, scaled, g->partials, g->S, slot, g->stream);
+    return sync_scalars();
+}
+
+// The lean solve on the RHS in r64 (returns 1 converged, 0 not, -1 error).
+static int run_lean(bool x0set, unsigned &it, double &residual)
+{
+    const OpDesc &op = g->dm.op64;
+    const size_t n = op.ndof, nh = hmg_vars();
+    const int p = vars_per_node();
+    hipStream_t st = g->stream;
+    const bool minres = is_minres(), two = is_two_vector();
+    const cudapcgFlag_t rule = g->resnorm;
+    const bool l2 = rule == CUDAPCG_L2_NORM;
+    const double tol = g->num_tol;
+    const unsigned maxit = g->max_iterations;
+    Jac<double> jac = make_jac();
+    if (jac.kind == JAC_STORED) {
+        if (!g->dinv64) {
+            if (hipMalloc(&g->dinv64, n * sizeof(double)) != hipSuccess) {
+                (void)hipGetLastError();
+                fprintf(stderr, "chfem: could not allocate the Jacobi diagonal\n");
+                return -1;
+            }
+            g->dinv_valid = false;
+        }
+        ensure_dinv();
+        jac.dinv = g->dinv64;
+    }
+    double *qtmp = nullptr;
+    if (lean_needs_qtmp(op)) {
+        if (!g->q64 && hipMalloc(&g->q64, n * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            fprintf(stderr, "chfem: could not allocate the operator work vector\n");
+            return -1;
+        }
+        qtmp = g->q64;
+    }
+    double *z = g->r64, *d = g->d64, *x = two ? nullptr : g->x64;
+    const double *x0 = x0set ? g->x64 : nullptr;
+    // two-vector solvers: reduced solution and its history, per solve
+    const int xmode = g->xreduce_flag;
+    const int shift = (int)std::min<unsigned>(g->xreduce_shift, (unsigned)p - 1);
+    const size_t xn = two ? (xmode == CUDAPCG_XREDUCE_FULL ? nh : (xmode == CUDAPCG_XREDUCE_ONLYDIR ? nh / p : 0)) : 0;
+    const size_t hlen = (size_t)maxit + 1;
+    if (two) {
+        if (g->xr_n != xn) {
+            dfree(g->xr64);
+            g->xr_n = 0;
+            if (xn && hipMalloc(&g->xr64, xn * sizeof(double)) != hipSuccess) return -1;
+            g->xr_n = xn;
+        }
+        if (g->hist_n != hlen * p) {
+            dfree(g->hist);
+            g->hist_n = 0;
+            if (hipMalloc(&g->hist, hlen * p * sizeof(double)) != hipSuccess) return -1;
+            g->hist_n = hlen * p;
+        }
+        if (xn) launch_fill<double>(g->xr64, 0.0, xn, st);
+        launch_fill<double>(g->hist, 0.0, hlen * p, st);
+    }
+    double *xr = g->xr64;
+    // INF rule: absolute maximum of the initial residual (before any guess)
+    double res0_inf = 0;
+    if (rule == CUDAPCG_INF_NORM) {
+        if (!absmax_of(z, n, nullptr, S_TMP2)) return -1;
+        res0_inf = g->Sh[S_TMP2];
+    }
+    // z = M^-1 b (r.M^-1 r of b to S_RES0); MINRES: delta0 = z.Az
+    Reduce r0 = make_reduce(FIN_STORE, S_RES0);
+    launch_lean_to_z<double>(jac, z, n, &r0, st);
+    if (minres) {
+        Reduce rd0 = make_reduce(FIN_STORE, S_DELTA0);
+        launch_lean_apply<double>(op, jac, LEAN_VAV, z, z, qtmp, &rd0, st);
+    }
+    if (!sync_scalars()) return -1;
+    const double rz0 = g->Sh[S_RES0];
+    const double delta0 = minres ? g->Sh[S_DELTA0] : rz0;
+    const double res0 = rule == CUDAPCG_INF_NORM ? res0_inf : rz0;
+    it = 0;
+    residual = 0;
+    // x (CG3, MINRES3) starts at the guess x0 (the same buffer) or zero
+    if (x && !x0) launch_fill<double>(x, 0.0, n, st);
+    if (fabs(res0) < 1e-12) {
+        // (MINRES zeroes a guess here, pminres3.cu; CG keeps it, pcg3.cu)
+        if (x && minres) launch_fill<double>(x, 0.0, n, st);
+        return 1;
+    }
+    double delta = delta0;
+    if (x0) {
+        // z = M^-1 (b - A x0); the reduced solution (CG2, MINRES2) starts
+        // from x0
+        if (!set_scalar(S_ALPHA, 1.0)) return -1;
+        Reduce rx = make_reduce(FIN_STORE, S_TMP0);
+        launch_lean_apply<double>(op, jac, LEAN_ZUPD, x0, z, qtmp, &rx, st);
+        if (minres) {
+            Reduce rdx = make_reduce(FIN_STORE, S_DELTA);
+            launch_lean_apply<double>(op, jac, LEAN_VAV, z, z, qtmp, &rdx, st);
+        }
+        if (two) {
+            Reduce rxr = make_reduce(FIN_STORE, 0);
+            launch_xreduce<double>(xr, nullptr, x0, g->hist, hlen, xmode, shift, p, nh, rxr, st);
+        }
+        if (!sync_scalars()) return -1;
+        delta = minres ? g->Sh[S_DELTA] : g->Sh[S_TMP0];
+        if (!above(delta, delta0, tol)) {
+            residual = sqrt(fabs(delta) / fabs(delta0));
+            return 1;
+        }
+    }
+    if (!set_scalar(S_DELTA, delta)) return -1;
+    // L2: the stop is decided on the device (CG on r.M^-1 r, MINRES on its
+    // residual measure) and iterations are queued in batches; the INF and
+    // error rules are evaluated on the host after every iteration
+    if (!arm_guard(l2 ? tol * tol * fabs(minres ? res0 : delta0) : -1.0)) return -1;
+    Reduce ra = make_guarded(FIN_ALPHA, 0);
+    Reduce rz = make_guarded(minres ? FIN_STOP : FIN_DELTA, 0);
+    Reduce rv = make_guarded(FIN_DELTA_NOCOUNT, 0);
+    Reduce rxr = make_guarded(FIN_STORE, 0);
+    bool done = false;
+    double metric = 1.0;
+    while (!done && it < maxit) {
+        const unsigned nb = l2 ? std::min(cg_batch(), maxit - it) : 1u;
+        for (unsigned b = 0; b < nb; b++) {
+            const unsigned j = it + b;
+            // L2: x += alpha d of the previous iteration rides on the
+            // direction update; the other rules need x current every step
+            launch_lean_dir<double>(l2 ? x : nullptr, d, z, g->S, j == 0 ? 1 : 0, 1, n, st);
+            launch_lean_apply<double>(op, jac, minres ? LEAN_QMQ : LEAN_VAV, d, z, qtmp, &ra, st);
+            if (!l2 && x) launch_lean_axpy<double>(x, d, g->S, n, st);
+            if (two) launch_xreduce<double>(xr, d, nullptr, g->hist, hlen, xmode, shift, p, nh, rxr, st);
+            launch_lean_apply<double>(op, jac, LEAN_ZUPD, d, z, qtmp, &rz, st);
+            if (minres) launch_lean_apply<double>(op, jac, LEAN_VAV, z, z, qtmp, &rv, st);
+        }
+        if (!sync_loop()) return -1;
+        it = (unsigned)g->Sh[S_ITER];
+        done = g->Sh[S_DONE] != 0.0;
+        if (!l2) {
+            const double a = g->Sh[S_ALPHA], dl = g->Sh[S_DELTA];
+            if (rule == CUDAPCG_INF_NORM) {
+                if (!absmax_of(z, n, jac.kind == JAC_NONE ? nullptr : &jac, S_TMP2)) return -1;
+                metric = g->Sh[S_TMP2] / res0;
+                done = !(metric > tol);
+            } else {
+                if (!absmax_of(d, nh, nullptr, S_TMP3)) return -1;
+                double ax = 1.0;
+                // denominators as the reference forms them: x for CG3 and
+                // MINRES3, the reduced solution for CG2, 1 for MINRES2
+                if (x) {
+                    if (!absmax_of(x, nh, nullptr, S_TMP4)) return -1;
+                    ax = g->Sh[S_TMP4];
+                } else if (!minres && xn) {
+                    if (!absmax_of(xr, xn, nullptr, S_TMP4)) return -1;
+                    ax = g->Sh[S_TMP4];
+                }
+                metric = fabs(a) * g->Sh[S_TMP3] / ax;
+                done = minres ? !(metric > tol) : (metric <= tol || !above(dl, delta0, tol));
+            }
+        }
+    }
+    if (l2 && x && it > 0) launch_lean_axpy<double>(x, d, g->S, n, st);
+    if (l2) {
+        if (minres) {
+            residual = sqrt(fabs(g->Sh[S_STOP]) / fabs(res0));
+            return residual <= tol;
+        }
+        residual = sqrt(fabs(g->Sh[S_DELTA]) / fabs(delta0));
+        return !above(g->Sh[S_DELTA], delta0, tol);
+    }
+    residual = metric;
+    return metric <= tol;
+}
+
+// The values a two-vector solve returns (xreduce.cu:4-26, pcg2.cu:246-262):
+// the history scaled by the xreduce scale and written to
+// <model name>_xreduce_metrics_<count>.bin, then per node value the total
+// (NONE: the per-iteration terms summed from the last back to the initial
+// one; ONLYDIR: the reduced direction only; FULL: the last entry).
+static int finish_two_vector(unsigned it)
+{
+    const int p = vars_per_node();
+    const size_t hl = (size_t)it + 1;
+    std::vector<double> h(hl * p, 0.0);
+    if (g->hist && hl * p <= g->hist_n &&
+        hipMemcpy(h.data(), g->hist, hl * p * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return 0;
+    const int mode = g->xreduce_flag;
+    const int shift = (int)std::min<unsigned>(g->xreduce_shift, (unsigned)p - 1);
+    const double scl = g->xreduce_scale;
+    // per node value ii: res_per_it[ii][0 .. it]
+    std::vector<std::vector<double>> per(p, std::vector<double>(hl));
+    for (int ii = 0; ii < p; ii++)
+        for (size_t jj = 0; jj < hl; jj++) per[ii][jj] = h[jj * p + ii];
+    for (int ii = 0; ii < p; ii++) {
+        if (mode == CUDAPCG_XREDUCE_ONLYDIR && ii != shift) continue;
+        for (size_t jj = 0; jj < hl; jj++) per[ii][jj] *= scl;
+    }
+    if (g->model->name) {
+        char fn[4096];
+        snprintf(fn, sizeof(fn), "%s_xreduce_metrics_%lu.bin", g->model->name, g->count);
+        FILE *f = fopen(fn, "wb");
+        if (f) {
+            for (int ii = 0; ii < p; ii++) {
+                if (mode == CUDAPCG_XREDUCE_ONLYDIR && ii != shift) continue;
+                fwrite(per[ii].data(), sizeof(double), hl, f);
+            }
+            fclose(f);
+        }
+    }
+    for (int ii = 0; ii < p; ii++) {
+        double v;
+        if (mode == CUDAPCG_XREDUCE_NONE) {
+            for (size_t jj = hl - 1; jj > 0; jj--) per[ii][jj - 1] += per[ii][jj];
+            v = per[ii][0];
+        } else if (mode == CUDAPCG_XREDUCE_ONLYDIR) {
+            v = ii == shift ? per[ii][hl - 1] : 0.0;
+        } else {
+            v = per[ii][hl - 1];
+        }
+        g->xred_out[ii] = v;
+    }
+    g->xred_count = p;
+    return 1;
+}
+
 static int ensure_dinv()
 {
     if (g->dinv_valid) return 1;
     const size_t n = nvars();
+    if (!g->dinv64) {
+        if (is_lean()) return 1;  // the lean solvers allocate it only when they need it
+        return 0;
+    }
     if (g->precond) launch_jacobi<double>(g->dm.op64, g->dinv64, g->stream);
     else launch_fill<double>(g->dinv64, 1.0, n, g->stream);
     if (g->precision == CHFEM_PREC_MIXED) launch_convert<double, float>(g->dinv64, g->dinv32, n, g->stream);
@@ -1006,11 +1440,15 @@ int solve_device(const double *x0_dev)
     if (!g || !g->model) return -1;
     if (!g->arrays && alloc_solver_arrays() != CUDAPCG_TRUE) return -1;
     if (!make_ops()) return -1;
-    autotune_tiles();
+    // (the lean solvers keep no buffers for the autotuner's trial launches:
+    // they run one fixed tile)
+    if (!is_lean()) autotune_tiles();
     ensure_dinv();
     const size_t n = nvars();
     bool x0set = g->x0_set;
     if (x0_dev) {
+        // (two-vector solvers hold a guess in x64 only while they need it)
+        if (!g->x64 && hipMalloc(&g->x64, n * sizeof(double)) != hipSuccess) return -1;
         hipMemcpyAsync(g->x64, x0_dev, n * sizeof(double), hipMemcpyDeviceToDevice, g->stream);
         x0set = true;
     }
@@ -1021,11 +1459,18 @@ int solve_device(const double *x0_dev)
     unsigned it = 0, sweeps = 1;
     double res = 0;
     int rc;
-    if (g->precision == CHFEM_PREC_MIXED) {
+    const bool rule = g->resnorm != CUDAPCG_L2_NORM;
+    g->xred_count = 0;
+    if (is_lean()) {
+        rc = run_lean(x0set, it, res);
+        if (rc >= 0 && is_two_vector() && !finish_two_vector(it)) rc = -1;
+    } else if (g->precision == CHFEM_PREC_MIXED && !rule) {
         rc = run_mixed(x0set, it, sweeps, res);
     } else if (is_minres()) {
         rc = run_minres<double>(g->dm.op64, g->x64, g->r64, g->d64, g->q64, g->s64, g->dinv64, x0set, g->num_tol,
                                 g->max_iterations, it, res);
+    } else if (rule) {
+        rc = run_pcg_rule(x0set, it, res);
     } else {
         rc = run_pcg<double>(g->dm.op64, g->x64, g->r64, g->d64, g->q64, g->dinv64, x0set, g->num_tol,
                              g->max_iterations, it, res, nullptr, g->s64);
@@ -1117,8 +1562,12 @@ cudapcgFlag_t cudapcgSetSolver(cudapcgFlag_t flag)
         printf("ERROR: Invalid \"solverFlag\" (%d).\n", flag);
         return CUDAPCG_FALSE;
     }
-    const bool was_minres = is_minres();
+    const bool was_minres = is_minres(), was_lean = is_lean(), was_two = is_two_vector();
     g->solver_flag = flag;
+    if (g->arrays && (was_lean != is_lean() || was_two != is_two_vector())) {
+        free_solver_arrays();
+        return alloc_solver_arrays();
+    }
     if (g->arrays && is_minres() && !was_minres) return alloc_solver_arrays();
     return CUDAPCG_TRUE;
 }
@@ -1135,8 +1584,6 @@ cudapcgFlag_t cudapcgSetResNorm(cudapcgFlag_t flag)
 {
     if (!g) return CUDAPCG_FALSE;
     g->resnorm = flag > CUDAPCG_ERROR_NORM ? CUDAPCG_L2_NORM : flag;
-    if (g->resnorm != CUDAPCG_L2_NORM)
-        fprintf(stderr, "chfem: only the L2 stopping rule is implemented; using L2\n");
     return CUDAPCG_TRUE;
 }
 
@@ -1166,6 +1613,7 @@ cudapcgFlag_t cudapcgEnd(void)
     g->fkeys_n = 0;
     dfree(g->kinv64);
     dfree(g->kinv32);
+    dfree(g->btab64);
     if (g->model && g->model->freeAllowed_flag) {
         free_model_device(g->model);
         free(g->model);
@@ -1443,6 +1891,30 @@ cudapcgFlag_t cudapcgSetLclMtxs(cudapcgVar_t *L)
             return CUDAPCG_FALSE;
         }
     }
+    // Stokes Jacobi per DOF class for the memory-lean solvers, summed in the
+    // order k_jacobi_fluid uses, so the values equal the stored diagonal's
+    if (ph == PHY_FLUID) {
+        double dv = 0, dp = 0, bt[256];
+        for (int e = 0; e < 8; e++) {
+            dv += K[(3 * e) * 32 + 3 * e];
+            dp -= K[(24 + e) * 32 + 24 + e];
+        }
+        g->fl_vinv = 1.0 / dv;
+        g->fl_pinv = 1.0 / dp;
+        for (int k = 0; k < 256; k++) {
+            double dd = 0;
+            for (int e = 0; e < 8; e++)
+                if ((k >> e) & 1) dd -= K[(24 + e) * 32 + 24 + e];
+            bt[k] = dd != 0.0 ? 1.0 / dd : 0.0;
+        }
+        if ((!g->btab64 && hipMalloc(&g->btab64, sizeof(bt)) != hipSuccess) ||
+            hipMemcpy(g->btab64, bt, sizeof(bt), hipMemcpyHostToDevice) != hipSuccess) {
+            fprintf(stderr, "chfem: failed to upload the Stokes Jacobi table\n");
+            free(K);
+            free(Kf);
+            return CUDAPCG_FALSE;
+        }
+    }
     dfree(g->dm.K64);
     dfree(g->dm.K32);
     hipError_t e1 = hipMalloc(&g->dm.K64, ksz * nk * sizeof(double));
@@ -1495,13 +1967,10 @@ cudapcgFlag_t cudapcgSolve(cudapcgVar_t *x)
     int rc = solve_device(nullptr);
     if (rc < 0) return CUDAPCG_FALSE;
     const size_t n = nvars();
-    if (g->solver_flag >= CUDAPCG_CG2_SOLVER && g->analysis_flag == CUDAPCG_FLUID_3D) {
-        // two-vector solvers return the effective velocity field only
-        // (chfem/src/femhmg/cudapcg/solvers/xreduce.cu:4-26)
-        Reduce r = make_reduce(FIN_STORE, S_TMP0);
-        launch_fluid_velocity_sum<double>(g->dm.op64, g->x64, &r, g->stream);
-        if (!sync_scalars()) return CUDAPCG_FALSE;
-        for (int a = 0; a < 3; a++) x[a] = g->Sh[S_TMP0 + a] * g->xreduce_scale;
+    if (is_two_vector()) {
+        // two-vector solvers return the reduced solution, one value per
+        // node value (chfem/src/femhmg/cudapcg/solvers/pcg2.cu:250-262)
+        for (int a = 0; a < g->xred_count; a++) x[a] = g->xred_out[a];
     } else {
         CHECK_HIP(hipMemcpyAsync(x, g->x64, n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
         CHECK_HIP(hipStreamSynchronize(g->stream));
@@ -1513,6 +1982,10 @@ cudapcgFlag_t cudapcgSolve(cudapcgVar_t *x)
 
 cudapcgFlag_t cudapcgApplyOperator(const cudapcgVar_t *v, cudapcgVar_t *q)
 {
+    if (g && is_lean()) {
+        fprintf(stderr, "chfem: this entry point needs the CG or MINRES solver's vectors\n");
+        return CUDAPCG_FALSE;
+    }
     if (!g || !g->model || !v || !q) return CUDAPCG_FALSE;
     if (!g->arrays && alloc_solver_arrays() != CUDAPCG_TRUE) return CUDAPCG_FALSE;
     make_ops();
@@ -1527,6 +2000,10 @@ cudapcgFlag_t cudapcgApplyOperator(const cudapcgVar_t *v, cudapcgVar_t *q)
 
 cudapcgFlag_t cudapcgApplyOperatorF32(const float *v, float *q)
 {
+    if (g && is_lean()) {
+        fprintf(stderr, "chfem: this entry point needs the CG or MINRES solver's vectors\n");
+        return CUDAPCG_FALSE;
+    }
     if (!g || !g->model || !v || !q) return CUDAPCG_FALSE;
     if (!g->arrays && alloc_solver_arrays() != CUDAPCG_TRUE) return CUDAPCG_FALSE;
     make_ops();
@@ -1550,6 +2027,10 @@ cudapcgFlag_t cudapcgApplyOperatorF32(const float *v, float *q)
 
 cudapcgFlag_t cudapcgApplyPreconditioner(const cudapcgVar_t *v, cudapcgVar_t *z)
 {
+    if (g && is_lean()) {
+        fprintf(stderr, "chfem: this entry point needs the CG or MINRES solver's vectors\n");
+        return CUDAPCG_FALSE;
+    }
     if (!g || !g->model || !v || !z) return CUDAPCG_FALSE;
     if (!g->arrays && alloc_solver_arrays() != CUDAPCG_TRUE) return CUDAPCG_FALSE;
     make_ops();
@@ -1564,6 +2045,10 @@ cudapcgFlag_t cudapcgApplyPreconditioner(const cudapcgVar_t *v, cudapcgVar_t *z)
 
 double cudapcgOperatorBench(cudapcgFlag_t precision, unsigned int reps, cudapcgFlag_t fused_dot)
 {
+    if (g && is_lean()) {
+        fprintf(stderr, "chfem: this entry point needs the CG or MINRES solver's vectors\n");
+        return -1.0;
+    }
     if (!g || !g->model) return -1.0;
     if (!g->arrays && alloc_solver_arrays() != CUDAPCG_TRUE) return -1.0;
     if (precision == CHFEM_PREC_MIXED && !g->d32) {

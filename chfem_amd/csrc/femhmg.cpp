@@ -591,6 +591,13 @@ bool device_updateC(Model &m, const DeviceAux &a, unsigned dir, const double *x)
     const LoadCase lc = load_case(m, dir);
     DeviceModel *dm = active_model();
     Reduce red = make_reduce(FIN_STORE, S_TMP0);
+    double xr[4];
+    if (m.analysis == HMG_FLUID && reduced_solution(xr) >= 3) {
+        // CG2 / MINRES2: the solver reduced the velocity on the fly and
+        // returns the column itself (femhmg.c:1087-1093)
+        for (int row = 0; row < 3; row++) m.C[row * 3 + dir] = xr[row];
+        return true;
+    }
     if (m.analysis == HMG_FLUID) {
         launch_fluid_velocity_sum<double>(dm->op64, x, &red, active_stream());
         if (!sync_scalars()) return false;
@@ -1135,7 +1142,9 @@ logical hmgSolveResident(var *Cout)
             ok = false;
             break;
         }
-        if (m.export_x) {
+        if (m.export_x && g_solver >= CUDAPCG_CG2_SOLVER) {
+            printf("    WARNING: CG2 and MINRES2 keep no solution vector to export.\n");
+        } else if (m.export_x) {
             std::vector<double> xh(m.ndof);
             hipMemcpy(xh.data(), solution_buffer(), m.ndof * sizeof(double), hipMemcpyDeviceToHost);
             char fb[2048];
@@ -1197,6 +1206,10 @@ logical hmgSolveLoadCase(unsigned int dir, const var *x0, var *x, var *Ccol)
         rc = solve_device(nullptr);
     }
     if (rc < 0) return HMG_FALSE;
+    if (x && g_solver >= CUDAPCG_CG2_SOLVER) {
+        fprintf(stderr, "chfem: CG2 and MINRES2 keep no solution vector\n");
+        return HMG_FALSE;
+    }
     if (x && hipMemcpy(x, solution_buffer(), m.ndof * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
         return HMG_FALSE;
     if (Ccol) {

@@ -40,6 +40,11 @@ enum Finalize {
     // delta, S[DELTA] = delta_new; guarded: counts the iteration and sets
     // S[DONE] when |delta_new| <= S[THRESH]
     FIN_CG1 = 3,
+    // memory-lean MINRES: S[S_STOP] = sum (the residual measure r.M^-1 r);
+    // guarded: counts the iteration and sets S[DONE] when |sum| <= S[THRESH]
+    FIN_STOP = 4,
+    // S[S_DELTA_OLD] = S[S_DELTA]; S[S_DELTA] = sum, never counted
+    FIN_DELTA_NOCOUNT = 5,
 };
 
 struct Reduce {
@@ -97,6 +102,79 @@ struct OpDesc {
 };
 
 template <typename T> void launch_apply(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s);
+
+// Jacobi preconditioner of the memory-lean solvers, evaluated per DOF
+// without a stored diagonal where the model allows it:
+//   JAC_NONE   no preconditioner (M = I)
+//   JAC_KEYS   thermal/elastic NBN: kinv[key * nd + a], diagonal at
+//              kinv[65536 * nd + key * nd + a], key = keys[dof / nd]
+//   JAC_FLUID  Stokes: velocity DOFs vinv, pore pressures pinv, border
+//              pressures btab[border key] (256 entries: 1 / -(sum of the
+//              pressure diagonal over the key's pore elements))
+//   JAC_STORED a stored inverse diagonal dinv[dof] (EBE, generic operators)
+enum JacKind { JAC_NONE = 0, JAC_KEYS = 1, JAC_FLUID = 2, JAC_STORED = 3 };
+template <typename T>
+struct Jac {
+    int kind;
+    int nd;
+    const T *kinv;
+    const uint16_t *keys;
+    const T *dinv;
+    T vinv, pinv;
+    const T *btab;
+    const uint8_t *bkeys;
+    uint32_t nvel;
+};
+
+// Operator forms of the memory-lean solvers (3 or 2 solver vectors, no
+// stored q = A v):
+//   LEAN_VAV  reduce v.(A v), nothing stored
+//   LEAN_QMQ  reduce (A v).M^-1 (A v), nothing stored
+//   LEAN_ZUPD z <- z - S[S_ALPHA] M^-1 A v in place, reduce z.diag(M) z
+//             (= r.M^-1 r for the residual r = diag(M) z)
+enum LeanForm { LEAN_VAV = 1, LEAN_QMQ = 2, LEAN_ZUPD = 3 };
+// One lean operator launch (red required, finished as red->fin says).  The
+// Walsh-basis NBN operators and the Walsh-basis Stokes operator evaluate the
+// form in their epilogue; every other operator writes A v to qtmp (a work
+// vector the solver then holds) and a streaming kernel evaluates the form.
+template <typename T>
+void launch_lean_apply(const OpDesc &op, const Jac<T> &jac, int form, const T *v, T *z, T *qtmp, const Reduce *red,
+                       hipStream_t s);
+// whether launch_lean_apply needs qtmp for this operator
+bool lean_needs_qtmp(const OpDesc &op);
+// the lean forms in the epilogues of the Walsh-basis NBN operator (kinv:
+// key-Jacobi table, nullptr: no preconditioner) and of the Walsh-basis
+// Stokes operator (Jacobi per DOF class, jac.kind JAC_FLUID or JAC_NONE)
+template <typename T>
+void launch_walsh_lean(const OpDesc &op, int form, const T *v, T *z, const void *kinv, const Reduce *red,
+                       hipStream_t s);
+template <typename T>
+void launch_stokes_lean(const OpDesc &op, const Jac<T> &jac, int form, const T *v, T *z, const Reduce *red,
+                        hipStream_t s);
+// z = M^-1 z in place, red <- z.diag(M) z (= r.M^-1 r of the r z held)
+template <typename T> void launch_lean_to_z(const Jac<T> &jac, T *z, size_t n, const Reduce *red, hipStream_t s);
+// first: d = z; else (x non-null: x += S[S_ALPHA] d) then d = z + beta d
+// with beta = S[S_DELTA] / S[S_DELTA_OLD] (guarded by red.guard via S)
+template <typename T> void launch_lean_dir(T *x, T *d, const T *z, const double *S, int first, int guard, size_t n,
+                                           hipStream_t s);
+// x += S[S_ALPHA] d
+template <typename T> void launch_lean_axpy(T *x, const T *d, const double *S, size_t n, hipStream_t s);
+// On-the-fly reduction of the solution (chfem/src/femhmg/cudapcg/solvers/
+// xreduce.cu:4-26) for the 2-vector solvers, after iteration j's alpha, with
+// p = nvpn values per node (1..4) over the first nh entries:
+//   mode 0 (NONE):    hist[p (j+1) + a] = alpha * sum_i d[p i + a]
+//   mode 1 (ONLYDIR): xr[i] += alpha d[p i + shift]; hist[p (j+1) + shift] = sum_i xr[i]
+//   mode 2 (FULL):    xr[i] += alpha d[i]; hist[p (j+1) + a] = sum_i xr[p i + a]
+// with j = S[S_ITER] (the guarded count of finished iterations) and alpha =
+// S[S_ALPHA]; entries past hist_len iterations are not recorded.  x0
+// non-null: the initial entry instead, hist[a] = sum_i x0[p i + a], and xr
+// seeded from x0 (ONLYDIR: xr[i] = x0[p i + shift]; FULL: xr = x0).
+template <typename T> void launch_xreduce(T *xr, const T *d, const T *x0, double *hist, size_t hist_len, int mode,
+                                          int shift, int nvpn, size_t nh, const Reduce &red, hipStream_t s);
+// S[slot] = max |v_i| over n entries (scaled: max |v_i / dinv_i|, the
+// residual of a preconditioned z), deterministic
+template <typename T> void launch_absmax(const T *v, size_t n, const Jac<T> *scaled, double *partials, double *S,
+                                         int slot, hipStream_t s);
 
 // EBE operator on the matrix cores (ebe_mfma.hip), thermal/elastic, nmat <= 4
 template <typename T> void launch_ebe_mfma(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s);

@@ -1183,6 +1183,15 @@ __global__ void __launch_bounds__(1024) k_reduce_finish(Reduce red, unsigned nb)
                 S[S_ITER] += 1.0;
                 if (fabs(dnew) <= S[S_THRESH]) S[S_DONE] = 1.0;
             }
+        } else if (red.fin == FIN_STOP) {
+            S[S_STOP] = s[0];
+            if (red.guard) {
+                S[S_ITER] += 1.0;
+                if (fabs(s[0]) <= S[S_THRESH]) S[S_DONE] = 1.0;
+            }
+        } else if (red.fin == FIN_DELTA_NOCOUNT) {
+            S[S_DELTA_OLD] = S[S_DELTA];
+            S[S_DELTA] = s[0];
         } else {
 #pragma unroll
             for (int k = 0; k < NV; k++) S[red.slot + k] = s[k];

@@ -39,6 +39,9 @@ double *solution_buffer();
 // run the configured solver on the RHS already in rhs_buffer(); x0 (device,
 // FP64) may be null.  Returns 1 if converged.
 int solve_device(const double *x0_dev);
+// the values the last two-vector solve (CG2, MINRES2) returned, already
+// scaled (count, 0 if the last solve was not one)
+int reduced_solution(double *out);
 // scalar reduction scratch of the active solver
 Reduce make_reduce(int fin, int slot);
 double *scalars_host();

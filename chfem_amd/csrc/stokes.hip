@@ -106,14 +106,42 @@ __global__ void k_fluid_node_keys(const uint32_t *__restrict__ node2dof, const u
     }
 }
 
+// Lean operator form of the memory-lean solvers (LeanForm; form 0: plain
+// apply) with the Jacobi inverse diagonal per DOF class: velocity vinv, pore
+// pressure pinv, border pressure btab[key] (btab == nullptr and vinv = pinv
+// = 1: no preconditioner).  Form 3 reads and writes z through q.
+template <typename T>
+struct StokesLean {
+    int form;
+    T alpha, vinv, pinv;
+    const T *btab;
+};
+
+// one DOF of a lean form: the operator result o at a DOF whose input value
+// was u, M^-1 entry inv, z (form 3) at zq
+template <typename T>
+__device__ __forceinline__ void lean_dof(const StokesLean<T> &L, T u, T o, T inv, T *zq, double &acc)
+{
+    if (L.form == LEAN_VAV) {
+        acc += (double)u * o;
+    } else if (L.form == LEAN_QMQ) {
+        acc += (double)o * (double)(inv * o);
+    } else {
+        const T zn = *zq - L.alpha * (inv * o);
+        *zq = zn;
+        acc += (double)zn * (double)zn / (double)inv;
+    }
+}
+
 // One tile of the Stokes operator (see walsh_tile for the march, the tile
 // order and the load pipelining; here a plane node holds (u, v, w, p)
 // gathered through the node-to-DOF map, whose entries and keys are loaded
 // one layer ahead of the value gathers).
-template <typename T, int TR, int TC, bool DOT>
+template <typename T, int TR, int TC, bool DOT, bool LEAN = false>
 __device__ __forceinline__ double stokes_tile(const T *__restrict__ v, T *__restrict__ q,
                                               const uint32_t *__restrict__ node2dof, const uint8_t *__restrict__ fk,
-                                              uint32_t nvel, int ny, int nx, int nz, int LZ, int bx, int by, int bz)
+                                              uint32_t nvel, int ny, int nx, int nz, int LZ, int bx, int by, int bz,
+                                              StokesLean<T> L = StokesLean<T>{})
 {
     constexpr int NT = TR * TC, NR = TR + 1, NC = TC + 1, NN = NR * NC;
     constexpr int NS = (NN + NT - 1) / NT;  // plane nodes per work-item
@@ -256,14 +284,26 @@ __device__ __forceinline__ double stokes_tile(const T *__restrict__ v, T *__rest
                 o[a] = acc[a] + A[a] + bleft[a];
                 acc[a] = Af[a] + bleft[4 + a];
             }
-            if (key == 255u) {
-                T *qo = q + 3 * (size_t)id;
-                qo[0] = o[0]; qo[1] = o[1]; qo[2] = o[2];
-                if (DOT) dsum += (double)own_u[0] * o[0] + (double)own_u[1] * o[1] + (double)own_u[2] * o[2];
-            }
-            if (key != 0u) {
-                q[3 * (size_t)nvel + id] = o[3];
-                if (DOT) dsum += (double)own_u[3] * o[3];
+            if constexpr (LEAN) {
+                if (key == 255u) {
+                    T *qo = q + 3 * (size_t)id;
+#pragma unroll
+                    for (int a = 0; a < 3; a++) lean_dof(L, own_u[a], o[a], L.vinv, qo + a, dsum);
+                }
+                if (key != 0u) {
+                    const T inv = key == 255u ? L.pinv : (L.btab ? L.btab[key] : T(1));
+                    lean_dof(L, own_u[3], o[3], inv, q + 3 * (size_t)nvel + id, dsum);
+                }
+            } else {
+                if (key == 255u) {
+                    T *qo = q + 3 * (size_t)id;
+                    qo[0] = o[0]; qo[1] = o[1]; qo[2] = o[2];
+                    if (DOT) dsum += (double)own_u[0] * o[0] + (double)own_u[1] * o[1] + (double)own_u[2] * o[2];
+                }
+                if (key != 0u) {
+                    q[3 * (size_t)nvel + id] = o[3];
+                    if (DOT) dsum += (double)own_u[3] * o[3];
+                }
             }
         }
         {
@@ -274,19 +314,21 @@ __device__ __forceinline__ double stokes_tile(const T *__restrict__ v, T *__rest
     return dsum;
 }
 
-template <typename T, int TR, int TC, bool DOT>
+template <typename T, int TR, int TC, bool DOT, bool LEAN = false>
 __global__ void __launch_bounds__(TR *TC) k_stokes_walsh(const T *__restrict__ v, T *__restrict__ q,
                                                           const uint32_t *__restrict__ node2dof,
                                                           const uint8_t *__restrict__ fk, uint32_t nvel, int ny, int nx,
-                                                          int nz, int LZ, Reduce red)
+                                                          int nz, int LZ, Reduce red,
+                                                          StokesLean<T> L = StokesLean<T>{})
 {
     if (DOT && stopped(red)) return;
+    if (LEAN) L.alpha = L.form == LEAN_ZUPD ? (T)red.S[S_ALPHA] : T(0);
     // XCD-contiguous tile order (see k_walsh)
     const unsigned gx = (unsigned)(ny + TR - 2) / (TR - 1), gy = (unsigned)(nx + TC - 2) / (TC - 1);
     const unsigned b = blockIdx.x, nb = gridDim.x, per = nb / 8u, rem = nb % 8u, xcd = b % 8u, k = b / 8u;
     const unsigned t = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + k;
-    const double dsum = stokes_tile<T, TR, TC, DOT>(v, q, node2dof, fk, nvel, ny, nx, nz, LZ, (int)(t % gx),
-                                                    (int)((t / gx) % gy), (int)(t / (gx * gy)));
+    const double dsum = stokes_tile<T, TR, TC, DOT, LEAN>(v, q, node2dof, fk, nvel, ny, nx, nz, LZ, (int)(t % gx),
+                                                          (int)((t / gx) % gy), (int)(t / (gx * gy)), L);
     if (DOT) {
         double vv[1] = {dsum};
         store_partials<1>(red, vv);
@@ -320,6 +362,26 @@ void launch_stokes_walsh(const OpDesc &op, const T *v, T *q, const Reduce *red, 
     }
 }
 
+template <typename T>
+void launch_stokes_lean(const OpDesc &op, const Jac<T> &jac, int form, const T *v, T *z, const Reduce *red,
+                        hipStream_t s)
+{
+    constexpr int TR = 32, TC = 8;
+    int LZ = op.tile_lz > 0 ? op.tile_lz : 16;
+    const long cols = (long)((op.ny + TR - 2) / (TR - 1)) * ((op.nx + TC - 2) / (TC - 1));
+    while (cols * ((op.nz + LZ - 1) / LZ) > MAX_PARTIAL_BLOCKS) LZ *= 2;
+    const unsigned grid = (unsigned)cols * (unsigned)((op.nz + LZ - 1) / LZ);
+    const bool pc = jac.kind == JAC_FLUID;
+    const StokesLean<T> L = {form, T(0), pc ? jac.vinv : T(1), pc ? jac.pinv : T(1), pc ? jac.btab : nullptr};
+    k_stokes_walsh<T, TR, TC, true, true><<<grid, TR * TC, 0, s>>>(v, z, op.node2dof, op.fkeys, op.nvel, op.ny, op.nx,
+                                                                   op.nz, LZ, *red, L);
+    launch_reduce_finish(*red, grid, 1, s);
+}
+
+template void launch_stokes_lean<float>(const OpDesc &, const Jac<float> &, int, const float *, float *,
+                                        const Reduce *, hipStream_t);
+template void launch_stokes_lean<double>(const OpDesc &, const Jac<double> &, int, const double *, double *,
+                                         const Reduce *, hipStream_t);
 template void launch_stokes_walsh<float>(const OpDesc &, const float *, float *, const Reduce *, hipStream_t);
 template void launch_stokes_walsh<double>(const OpDesc &, const double *, double *, const Reduce *, hipStream_t);
 

@@ -238,6 +238,10 @@ struct DirFuse {
     // then points at d_old2
     int zring;
     T bprev;
+    // lean operator form (FM 3, LeanForm): 1 v.Av; 2 Av.M^-1 Av; 3 z -= alpha
+    // M^-1 A v in place in q (z) with z.diag z; M^-1 from kinv by the owned
+    // node's packed key (kinv == nullptr: no preconditioner)
+    int lean;
 };
 
 // Sums a tile contributes to the launch's reductions: d.q, and for the
@@ -278,8 +282,10 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
 {
     static_assert(ND == 1 || ND == 3, "1 (conduction) or 3 (elastic) values per node");
     // FM 0: plain apply; 1: direction update inside (FUSE); 2: single-kernel
-    // CG, the z update inside as well (CG1)
-    constexpr bool FUSE = FM >= 1, CG1 = FM == 2;
+    // CG, the z update inside as well (CG1); 3: a lean form of the
+    // memory-lean solvers (fz.lean), the operator's result reduced or folded
+    // into z instead of stored
+    constexpr bool FUSE = FM == 1 || FM == 2, CG1 = FM == 2, LEAN = FM == 3;
     static_assert(!HALO || TR == 64, "the halo wavefront variant takes 64-row columns");
     // NCOL column work-items (+ the halo wavefront); OR node rows owned per
     // tile; the node planes hold rows r0 - 1 .. r0 + OR
@@ -515,12 +521,22 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
         mk = mat_key(lw1);
         // CG1: dinv and diag of the owned node (its key is the element's,
         // loaded a step ahead), read now and used after this layer's elements
-        T mo[CG1 ? ND : 1], dgo[CG1 ? ND : 1];
+        T mo[CG1 || LEAN ? ND : 1], dgo[CG1 || LEAN ? ND : 1];
         if constexpr (CG1) {
 #pragma unroll
             for (int a = 0; a < ND; a++) {
                 mo[a] = fz.kinv[ND * kcur + a];
                 dgo[a] = fz.kinv[(size_t)65536 * ND + ND * kcur + a];
+            }
+        }
+        // LEAN: M^-1 and diag(M) of the owned node, and (form 3) its z
+        T zo[LEAN ? ND : 1];
+        if constexpr (LEAN) {
+#pragma unroll
+            for (int a = 0; a < ND; a++) {
+                mo[a] = fz.kinv ? fz.kinv[ND * kcur + a] : T(1);
+                dgo[a] = fz.kinv ? fz.kinv[(size_t)65536 * ND + ND * kcur + a] : T(1);
+                zo[a] = fz.lean == 3 ? q[on + a] : T(0);
             }
         }
         // owned node's d (and, CG1, its updated z), read from the planes of
@@ -562,7 +578,22 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
                 acc[a] = Af[a] + bleft[ND + a];
             }
             T *qo = q + on;
-            if constexpr (CG1) {
+            if constexpr (LEAN) {
+                if (fz.lean == 1) {
+#pragma unroll
+                    for (int a = 0; a < ND; a++) dsum += (double)own_u[a] * o[a];
+                } else if (fz.lean == 2) {
+#pragma unroll
+                    for (int a = 0; a < ND; a++) dsum += (double)o[a] * (double)(mo[a] * o[a]);
+                } else {
+#pragma unroll
+                    for (int a = 0; a < ND; a++) {
+                        const T zn = zo[a] - fz.alpha * (mo[a] * o[a]);
+                        qo[a] = zn;
+                        dsum += (double)zn * (double)(dgo[a] * zn);
+                    }
+                }
+            } else if constexpr (CG1) {
 #pragma unroll
                 for (int a = 0; a < ND; a++) qo[a] = mo[a] * o[a];
             } else {
@@ -587,7 +618,7 @@ __device__ __forceinline__ TileSums walsh_tile(const T *__restrict__ v, T *__res
                     zdz += (double)own_z[a] * (double)(dgo[a] * own_z[a]);
                 }
             }
-            if (DOT) {
+            if (DOT && !LEAN) {
 #pragma unroll
                 for (int a = 0; a < ND; a++) dsum += (double)own_u[a] * o[a];
             }

@@ -11,7 +11,7 @@ __global__ void __launch_bounds__(TR *TC + (HALO ? 64 : 0)) k_walsh(const T *__r
                                                    const uint16_t *__restrict__ keys, const T *__restrict__ lamu,
                                                    int ebe, int ny, int nx, int nz, int LZ, Reduce red, DirFuse<T> fz)
 {
-    constexpr bool FUSE = FM >= 1;
+    constexpr bool FUSE = FM == 1 || FM == 2;
     if (DOT && stopped(red)) return;
     // beta of the fused direction update, as k_kj_pcg_dir forms it
     if (FUSE && !fz.first) fz.beta = (T)(red.S[S_DELTA] / red.S[S_DELTA_OLD]);
@@ -21,6 +21,8 @@ __global__ void __launch_bounds__(TR *TC + (HALO ? 64 : 0)) k_walsh(const T *__r
     // single-kernel CG with z recovered from the ring: the beta that formed
     // d_old
     if (FM == 2 && fz.zring) fz.bprev = (T)red.S[S_BPREV];
+    // lean z update: alpha of this iteration
+    if (FM == 3) fz.alpha = fz.lean == LEAN_ZUPD ? (T)red.S[S_ALPHA] : T(0);
     // 1-D grid; blocks are dispatched to the 8 XCDs round-robin (block b to
     // XCD b % 8), so each XCD is given one contiguous range of tiles (rows of
     // tiles fastest): tiles that share halo nodes then run on the same XCD at
@@ -46,8 +48,10 @@ static void launch_walsh_nd(const OpDesc &op, const T *lamu, const T *v, T *q, c
 {
     Reduce rr = {nullptr, nullptr, 0, 0, 0};
     if (red) rr = *red;
-    DirFuse<T> f = {nullptr, nullptr, T(0), T(0), 1, 0, nullptr, nullptr, nullptr, 0, T(0)};
+    DirFuse<T> f = {nullptr, nullptr, T(0), T(0), 1, 0, nullptr, nullptr, nullptr, 0, T(0), 0};
     if (fz) f = *fz;
+    // lean forms of the memory-lean solvers: one tile shape (32 x 8)
+    const bool lean = fz && fz->lean;
     // single-kernel CG form: the dinv table given (only launch_apply_cg1
     // sets it; its z output may be absent)
     const bool cg1 = fz && fz->kinv;
@@ -90,6 +94,19 @@ static void launch_walsh_nd(const OpDesc &op, const T *lamu, const T *v, T *q, c
     // the halo-wavefront variant of 64-row tiles.  15 columns (1024
     // work-items) only in FP32: the FP64 planes of the single-kernel CG
     // iteration would not fit in LDS.
+    if (lean) {
+#define WALSH_LAUNCH_LEAN(TRv, TCv)                                                                     \
+    {                                                                                                   \
+        constexpr int nt = TRv * TCv;                                                                   \
+        const unsigned grid = (unsigned)((op.ny + TRv - 2) / (TRv - 1)) * ((op.nx + TCv - 2) / (TCv - 1)) * \
+                              ((op.nz + LZ - 1) / LZ);                                                  \
+        WALSH_GO(TRv, TCv, false, true, 3)                                                              \
+        launch_reduce_finish(rr, grid, 1, s);                                                           \
+        return;                                                                                         \
+    }
+        WALSH_LAUNCH_LEAN(32, 8)
+#undef WALSH_LAUNCH_LEAN
+    }
     if constexpr (sizeof(T) == 4) {
         if (t.TR == 65 && t.TC == 15) WALSH_LAUNCH(64, 15, true)
     }
@@ -111,6 +128,19 @@ void launch_walsh(const OpDesc &op, const T *lamu, const T *v, T *q, const Reduc
     else launch_walsh_nd<T, 1>(op, lamu, v, q, red, t, s, fz, ev);
 }
 
+template <typename T>
+void launch_walsh_lean(const OpDesc &op, int form, const T *v, T *z, const void *kinv, const Reduce *red,
+                       hipStream_t s)
+{
+    DirFuse<T> f = {nullptr, nullptr, T(0), T(0), 1, 0, nullptr, nullptr, (const T *)kinv, 0, T(0), form};
+    const WalshTiles t = {32, 8, op.tile_lz};
+    launch_walsh<T>(op, (const T *)op.lamu, v, z, red, t, s, &f, nullptr);
+}
+template void launch_walsh_lean<float>(const OpDesc &, int, const float *, float *, const void *, const Reduce *,
+                                       hipStream_t);
+template void launch_walsh_lean<double>(const OpDesc &, int, const double *, double *, const void *,
+                                        const Reduce *, hipStream_t);
+
 bool apply_dir_ok(const OpDesc &op)
 {
     // the operators launch_apply sends to the Walsh kernel, with key-Jacobi on
@@ -121,7 +151,7 @@ template <typename T>
 void launch_apply_dir(const OpDesc &op, const T *z, const T *dold, T *dnew, T *q, const Reduce *red, hipStream_t s,
                       const hipEvent_t *ev)
 {
-    const DirFuse<T> f = {z, dnew, T(0), T(0), dold ? 0 : 1, 0, nullptr, nullptr, nullptr, 0, T(0)};
+    const DirFuse<T> f = {z, dnew, T(0), T(0), dold ? 0 : 1, 0, nullptr, nullptr, nullptr, 0, T(0), 0};
     const WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
     launch_walsh<T>(op, (const T *)op.lamu, dold, q, red, t, s, &f, ev);
 }
@@ -135,7 +165,7 @@ void launch_apply_cg1(const OpDesc &op, const T *zold, const T *qold, const T *d
     // pending update)
     const int zr = dold2 && p ? 1 : 0;
     const DirFuse<T> f = {zr ? dold2 : zold, dnew, T(0), T(0), first, p, p ? qold : zold, znew, (const T *)op.kinv,
-                          zr, T(0)};
+                          zr, T(0), 0};
     const WalshTiles t = {op.tile_tr, op.tile_tc, op.tile_lz};
     launch_walsh<T>(op, (const T *)op.lamu, dold, qnew, red, t, s, &f, ev);
 }

@@ -948,9 +948,22 @@ static void precond(const double *dinv, int on, const double *v, double *out, si
 
 static int above_tol(double delta, double delta0, double tol) { return fabs(delta) > tol * tol * fabs(delta0); }
 
-/* returns 1 if converged; x holds the solution; iters/res reported */
-int orc_solve(void *h, int dir, int solver, int precon, int strategy, double tol, unsigned maxit,
-              const double *x0, double *x, unsigned *iters, double *res_out)
+static double absmax_n(const double *v, size_t n)
+{
+    double m = 0.0;
+    for (size_t i = 0; i < n; i++) m = fmax(m, fabs(v[i]));
+    return m;
+}
+
+/* The stopping rules (solvers/pcg.cu:130-146 and the switch at :176-190,
+ * pminres.cu:188-200, stopping_criteria.cu:7-14):
+ *   0 L2:    r.M^-1 r <= tol^2 r0.M^-1 r0
+ *   1 INF:   max|r| / max|b| <= tol (b: the residual of the zero guess)
+ *   2 error: |alpha| max|d| / max|x| <= tol over the first nhmg entries;
+ *            CG also stops when the L2 rule is met
+ * resnorm L2 is orc_solve.  Returns 1 if the rule was met. */
+int orc_solve_rule(void *h, int dir, int solver, int precon, int strategy, double tol, unsigned maxit, int resnorm,
+                   size_t nhmg, const double *x0, double *x, unsigned *iters, double *res_out)
 {
     orc_model *m = (orc_model *)h;
     const size_t n = m->ndof;
@@ -964,10 +977,14 @@ int orc_solve(void *h, int dir, int solver, int precon, int strategy, double tol
     unsigned it = 0;
     double residual = 0;
     int found = 0;
+    if (nhmg == 0 || nhmg > n) nhmg = n;
+    const double rinf0 = absmax_n(r, n);
+    double metric = 1.0;
     if (solver == 0) {
         precond(dinv, precon, r, d, n);
         double delta0 = dot(r, d, n), delta, delta_old, a;
-        if (fabs(delta0) < 1e-12) { memset(x, 0, n * sizeof(double)); found = 1; goto done; }
+        const double res0 = resnorm == 1 ? rinf0 : delta0;
+        if (fabs(res0) < 1e-12) { memset(x, 0, n * sizeof(double)); found = 1; goto done; }
         if (x0) {
             orc_apply(h, strategy, x, q);
             for (size_t i = 0; i < n; i++) r[i] -= q[i];
@@ -983,7 +1000,13 @@ int orc_solve(void *h, int dir, int solver, int precon, int strategy, double tol
         for (long i = 0; i < (long)n; i++) { x[i] += a * d[i]; r[i] -= a * q[i]; }
         precond(dinv, precon, r, q, n);
         delta = dot(r, q, n);
-        int cont = above_tol(delta, delta0, tol);
+        /* the rule after an iteration (a, d, x, r of that iteration) */
+#define PCG_RULE()                                                                                 \
+        (resnorm == 1 ? ((metric = absmax_n(r, n) / res0) > tol)                                   \
+         : resnorm == 2 ? (metric = fabs(a) * absmax_n(d, nhmg) / absmax_n(x, nhmg),               \
+                           !(metric <= tol || !above_tol(delta, delta0, tol)))                     \
+                        : above_tol(delta, delta0, tol))
+        int cont = PCG_RULE();
         while (cont && it < maxit) {
             it++;
             double beta = delta / delta_old;
@@ -996,15 +1019,21 @@ int orc_solve(void *h, int dir, int solver, int precon, int strategy, double tol
             precond(dinv, precon, r, q, n);
             delta_old = delta;
             delta = dot(r, q, n);
-            cont = above_tol(delta, delta0, tol);
+            cont = PCG_RULE();
         }
-        residual = sqrt(fabs(delta) / fabs(delta0));
-        found = !above_tol(delta, delta0, tol);
+#undef PCG_RULE
+        if (resnorm == 0) {
+            residual = sqrt(fabs(delta) / fabs(delta0));
+            found = !above_tol(delta, delta0, tol);
+        } else {
+            residual = metric;
+            found = metric <= tol;
+        }
     } else {
         /* MINRES */
         double res0, delta0, delta, delta_old, a, b, stop;
         precond(dinv, precon, r, q, n);
-        res0 = dot(r, q, n);
+        res0 = resnorm == 1 ? rinf0 : dot(r, q, n);
         orc_apply(h, strategy, q, s);
         delta0 = dot(q, s, n);
         if (fabs(res0) < 1e-12) { memset(x, 0, n * sizeof(double)); found = 1; goto done; }
@@ -1034,7 +1063,11 @@ int orc_solve(void *h, int dir, int solver, int precon, int strategy, double tol
             double *t = malloc(n * sizeof(double));
             precond(dinv, precon, r, t, n);
             stop = dot(r, t, n);
-            int cont = above_tol(stop, res0, tol);
+#define MINRES_RULE()                                                                              \
+            (resnorm == 1 ? ((metric = absmax_n(r, n) / res0) > tol)                               \
+             : resnorm == 2 ? ((metric = fabs(a) * absmax_n(d, nhmg) / absmax_n(x, nhmg)) > tol)   \
+                            : above_tol(stop, res0, tol))
+            int cont = MINRES_RULE();
             while (cont && it < maxit) {
                 it++;
                 b = delta / delta_old;
@@ -1055,11 +1088,12 @@ int orc_solve(void *h, int dir, int solver, int precon, int strategy, double tol
                 for (long i = 0; i < (long)n; i++) r[i] = precon ? r[i] / dinv[i] : r[i];
                 precond(dinv, precon, r, t, n);
                 stop = dot(r, t, n);
-                cont = above_tol(stop, res0, tol);
+                cont = MINRES_RULE();
             }
+#undef MINRES_RULE
             free(t);
         }
-        residual = sqrt(fabs(stop) / fabs(res0));
+        residual = resnorm == 0 ? sqrt(fabs(stop) / fabs(res0)) : metric;
         found = residual <= tol;
     }
 done:
@@ -1067,6 +1101,13 @@ done:
     if (res_out) *res_out = residual;
     free(r); free(d); free(q); free(s); free(dinv);
     return found;
+}
+
+/* returns 1 if converged; x holds the solution; iters/res reported */
+int orc_solve(void *h, int dir, int solver, int precon, int strategy, double tol, unsigned maxit,
+              const double *x0, double *x, unsigned *iters, double *res_out)
+{
+    return orc_solve_rule(h, dir, solver, precon, strategy, tol, maxit, 0, 0, x0, x, iters, res_out);
 }
 
 /* =========================================================================

@@ -48,6 +48,9 @@ def lib():
         L.orc_jacobi_inv.argtypes = [vp, vp]
         L.orc_rhs.argtypes = [vp, i32, vp]
         L.orc_updateC.argtypes = [vp, i32, vp, vp]
+        L.orc_solve_rule.restype = i32
+        L.orc_solve_rule.argtypes = [vp, i32, i32, i32, i32, dbl, C.c_uint, i32, sz, vp, vp,
+                                     C.POINTER(C.c_uint), C.POINTER(C.c_double)]
         L.orc_solve.restype = i32
         L.orc_solve.argtypes = [vp, i32, i32, i32, i32, dbl, C.c_uint, vp, vp,
                                 C.POINTER(C.c_uint), C.POINTER(dbl)]
@@ -187,6 +190,21 @@ class Model:
             x0p = _p(x0)
         ok = lib().orc_solve(self.h, direction, solver, precond, strategy, tol, maxit, x0p, _p(x),
                              C.byref(it), C.byref(res))
+        return x, bool(ok), it.value, res.value
+
+    def solve_rule(self, direction, resnorm, solver=0, precond=1, strategy=0, tol=1e-6, maxit=10000, x0=None,
+                   nhmg=0):
+        """solve() with the stopping rule resnorm (0 L2, 1 INF, 2 error over
+        the first nhmg entries, 0: all) as pcg.cu / pminres.cu apply it."""
+        x = np.zeros(self.ndof)
+        it = C.c_uint(0)
+        res = C.c_double(0)
+        x0p = None
+        if x0 is not None:
+            x0 = np.ascontiguousarray(x0, dtype=np.float64)
+            x0p = _p(x0)
+        ok = lib().orc_solve_rule(self.h, direction, solver, precond, strategy, tol, maxit, resnorm, nhmg, x0p,
+                                  _p(x), C.byref(it), C.byref(res))
         return x, bool(ok), it.value, res.value
 
     def set_alpha(self, alpha):

@@ -49,6 +49,8 @@ class GpuSolver:
         cm.nelem = m.N
         cm.nvars = m.ndof
         cm.nvarspernode = 3
+        # (femhmg.c:82: the velocity DOFs for Stokes, every DOF otherwise)
+        cm.nhmgvars = 3 * m.nvel if m.analysis == FLUID else m.ndof
         cm.nkeys = 1 if m.analysis == FLUID else m.nmat
         cm.localmtxdim = {THERMAL: 40, ELASTIC: 576, FLUID: 1024}[m.analysis]
         if m.analysis == FLUID:

@@ -427,6 +427,73 @@ def test_solvers_vs_dense(analysis):
                                            atol=1e-9 * np.abs(col).max())
 
 
+@pytest.mark.parametrize("solver", [0, 1], ids=["cg", "minres"])
+def test_inf_rule_pinned_by_residual(solver):
+    """The INF rule (pcg.cu:130-146, pminres.cu:188-200) restated in the
+    oracle stops at the first iteration whose true residual b - A x,
+    evaluated here through the operator, has max|r| / max|b| <= tol; the
+    iteration before it is above tol.  A rule on the preconditioned residual
+    or on the L2 measure would stop elsewhere."""
+    img = three_phase((5, 6, 4), 23)
+    m = make(ELASTIC, img, [1.0, 0.3, 10.0, 0.2, 4.0, 0.25], (0, 100, 255))
+    tol = 1e-5
+    for d in (0, 4):
+        b = m.rhs(d)
+        x, ok, it, res = m.solve_rule(d, 1, solver=solver, tol=tol, maxit=5000)
+        assert ok and it > 2
+        true = np.abs(b - m.apply(x)).max() / np.abs(b).max()
+        assert true <= tol * (1 + 1e-6)
+        assert abs(res - true) <= 1e-6 * true
+        xp, okp, itp, resp = m.solve_rule(d, 1, solver=solver, tol=tol, maxit=it - 1)
+        assert not okp and itp == it - 1
+        assert np.abs(b - m.apply(xp)).max() / np.abs(b).max() > tol
+
+
+@pytest.mark.parametrize("solver", [0, 1], ids=["cg", "minres"])
+def test_error_rule_pinned_by_iterate_change(solver):
+    """The error rule: |alpha| max|d| / max|x| is the relative change of the
+    iterate in the last step, max|x_k - x_(k-1)| / max|x_k|, read off two
+    runs that stop at k and k - 1 iterations; CG also stops on the L2 rule."""
+    img = three_phase((5, 4, 6), 29)
+    m = make(THERMAL, img, [1.0, 10.0, 3.5], (0, 100, 255))
+    tol = 1e-3
+    stopped_by_change = 0
+    for d in range(3):
+        x, ok, it, res = m.solve_rule(d, 2, solver=solver, tol=tol, maxit=5000)
+        assert it > 2
+        xp, _, itp, _ = m.solve_rule(d, 2, solver=solver, tol=tol, maxit=it - 1)
+        xpp, _, _, _ = m.solve_rule(d, 2, solver=solver, tol=tol, maxit=it - 2)
+        change = np.abs(x - xp).max() / np.abs(x).max()
+        prev = np.abs(xp - xpp).max() / np.abs(xp).max()
+        assert abs(res - change) <= 1e-9 * max(change, 1e-300)
+        # met means the rule's own metric is within tol (pcg.cu:233-236)
+        assert ok == (change <= tol)
+        assert prev > tol
+        if solver == 1:
+            assert change <= tol
+        else:
+            # CG stops at the first iteration where the change is within tol
+            # or the L2 rule holds (its own count k2)
+            _, _, k2, _ = m.solve_rule(d, 0, solver=solver, tol=tol, maxit=5000)
+            assert it <= k2 and (change <= tol or it == k2)
+        stopped_by_change += change <= tol
+    # (Jacobi-CG reaches the L2 rule first on these images; MINRES has no
+    # L2 alternative)
+    if solver == 1:
+        assert stopped_by_change == 3
+
+
+def test_solve_rule_l2_equals_solve():
+    img = three_phase((4, 5, 3), 31)
+    m = make(THERMAL, img, [1.0, 10.0, 3.5], (0, 100, 255))
+    for solver in (0, 1):
+        a = m.solve(1, solver=solver, tol=1e-8)
+        b = m.solve_rule(1, 0, solver=solver, tol=1e-8)
+        # (OpenMP dot products: equal to the order of their partial sums)
+        np.testing.assert_allclose(a[0], b[0], rtol=0, atol=1e-12 * np.abs(a[0]).max())
+        assert a[1] == b[1] and abs(a[2] - b[2]) <= 1
+
+
 def test_fluid_solvers_vs_dense():
     rng = np.random.default_rng(18)
     img = np.where(rng.random((4, 4, 5)) < 0.75, 0, 255).astype(np.uint8)
