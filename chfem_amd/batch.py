@@ -14,10 +14,12 @@ from .lib import SolveStats, lib, ptr
 PROPERTIES = {"conductivity": 0, "elasticity": 1}
 
 
-def batch_homogenize(samples, property, tol=1e-8, maxit=20000, voxel_size=1.0):
+def batch_homogenize(samples, property, tol=1e-8, maxit=20000, voxel_size=1.0, guess=None):
     """samples: list of (uint8 image (z, y, x), mat_props) -> list of effective
     matrices, and a dict with per-sample, per-load-case statistics and the
-    batch's device time."""
+    batch's device time.  guess: a trained chfem_amd.guess.LearnedGuess of
+    the same physics, whose prediction starts every load case (statistics
+    then carry x0_used and x0_residual)."""
     if property not in PROPERTIES:
         raise ValueError(f"Invalid property: {property}")
     analysis = PROPERTIES[property]
@@ -37,6 +39,8 @@ def batch_homogenize(samples, property, tol=1e-8, maxit=20000, voxel_size=1.0):
             flat = np.ascontiguousarray(img).reshape(-1)
             if L.chfemBatchAddSample(h, nf.encode(), ptr(flat)) != 1:
                 raise RuntimeError(f"chfemBatchAddSample failed for sample {k}")
+        if guess is not None and L.chfemBatchSetInitialGuessNetwork(h, guess.h) != 1:
+            raise RuntimeError("chfemBatchSetInitialGuessNetwork failed")
         rc = L.chfemBatchSolve(h, tol, maxit)
         if rc == 0:
             raise RuntimeError("chfemBatchSolve failed")
@@ -49,7 +53,8 @@ def batch_homogenize(samples, property, tol=1e-8, maxit=20000, voxel_size=1.0):
             st = (SolveStats * 6)()
             cnt = C.c_uint(0)
             L.chfemBatchGetStats(h, k, st, C.byref(cnt))
-            stats.append([(st[i].iterations, st[i].sweeps, st[i].residual) for i in range(cnt.value)])
+            stats.append([(st[i].iterations, st[i].sweeps, st[i].residual, st[i].x0_used, st[i].x0_residual)
+                          for i in range(cnt.value)])
         info = {"converged": rc == 1, "seconds": L.chfemBatchSeconds(h), "stats": stats}
         return out, info
     finally:

@@ -7,9 +7,16 @@
 namespace chfem {
 namespace batch {
 
-// Walsh tile of the batched operator (rows x columns per block)
-#define TILE_R 32
-#define TILE_C 8
+// Walsh tile of the batched operators: 64 x 7 work-items own 64 node rows
+// (the halo-wavefront variant of walsh_tile: one more wavefront evaluates
+// the row above the tile), so 256 rows take 4 tiles; 512 work-items per block
+#define TILE_R 64
+#define TILE_C 7
+#define TILE_HALO true
+#define TILE_ROWS (TILE_HALO ? TILE_R : TILE_R - 1)
+#define TILE_THREADS (TILE_R * TILE_C + (TILE_HALO ? 64 : 0))
+// node layers per block
+#define TILE_LZ 32
 
 // one linear system (a load case of a sample), device view
 struct Sys {
@@ -40,6 +47,8 @@ enum {
     SB_FRESH,   // next iteration is the first of a segment: direction = dinv r
     SB_REPL,    // residual replacement selected by the host
     SB_START,   // fresh FP32 start selected by the host
+    SB_BPREV,   // single-kernel step: beta that formed the newest direction
+    SB_BASE,    // single-kernel step: iteration count at the segment's start
     // alpha of each step of the current ring window (fused loop), zeroed
     // after every window flush
     SB_AHIST = 16, SB_RING_MAX = 16,
@@ -47,7 +56,7 @@ enum {
 };
 
 enum { OP_ITER = 0, OP_REPL = 1 };
-enum { FIN_B_ALPHA = 0, FIN_B_DELTA, FIN_B_TRUE, FIN_B_DELTA0, FIN_B_START };
+enum { FIN_B_ALPHA = 0, FIN_B_DELTA, FIN_B_TRUE, FIN_B_DELTA0, FIN_B_START, FIN_B_CG1 };
 
 struct Batch {
     int nsys = 0, nd = 1;
@@ -57,7 +66,7 @@ struct Batch {
     const Chunk *chunks = nullptr;
     int nchunks = 0;
     double *Sb = nullptr;      // nsys * SB_STRIDE
-    double *part = nullptr;    // max(ntiles, nchunks)
+    double *part = nullptr;    // max(4 * ntiles, nchunks)
 };
 
 template <typename T> void apply(const Batch &b, const T *v, T *q, int mode, hipStream_t s);
@@ -82,9 +91,22 @@ void finish_alpha(const Batch &b, int w, hipStream_t s);
 void zupdate(const Batch &b, float *z, const float *q, hipStream_t s);
 void ring_flush(const Batch &b, float *x32, const float *ring, size_t total, int w, hipStream_t s);
 // z-form start and replacement: as start / resid with z32 = M^-1 r in place
-// of r32 (and no d32)
-void start_z(const Batch &b, const double *r64, float *z32, float *x32, hipStream_t s);
-void resid_z(const Batch &b, double *r64, const double *q64, float *z32, hipStream_t s);
+// of r32 (and no d32).  z2 non-null (single-kernel steps): a system's z goes
+// to the buffer its next step reads, z32 when its iteration count j is even
+// and z2 when it is odd.
+void start_z(const Batch &b, const double *r64, float *z32, float *x32, hipStream_t s, float *z2 = nullptr);
+void resid_z(const Batch &b, double *r64, const double *q64, float *z32, hipStream_t s, float *z2 = nullptr);
+// Single-kernel step (the single-system loop's form 2 per system): the
+// deferred z update z = z_old - alpha M^-1 q_old, d = z + beta d_old into
+// ring slot j % w, q = A d stored as M^-1 q, and the partials of d.q, z.q,
+// q.M^-1 q and z.D z, then one finishing block per system (FIN_B_CG1:
+// alpha, the next r.z by the recurrence restarted from z.D z, the iteration
+// counted).  z and q alternate between two buffers by the parity of j (step
+// j reads buffer (j + 1) % 2 and writes buffer j % 2 of {z2, z} and {q,
+// q2}); from a segment's third step on (SB_BASE + 2, with w >= 3) z is
+// recovered from the two previous directions and neither read nor written.
+void apply_cg1(const Batch &b, float *z, float *z2, float *q, float *q2, float *ring, size_t total, int w,
+               hipStream_t s);
 
 }  // namespace batch
 }  // namespace chfem

@@ -45,7 +45,7 @@ __device__ __forceinline__ bool selected(const double *Sb, int sys, int mode)
 // (a system's tiles are contiguous, rows of tiles fastest) and neighbouring
 // tiles share its L2.
 template <typename T, int ND, int TR, int TC>
-__global__ void __launch_bounds__(TR *TC) k_apply_walsh(const Sys *__restrict__ sys, const Tile *__restrict__ tiles,
+__global__ void __launch_bounds__(TR *TC + (TILE_HALO ? 64 : 0)) k_apply_walsh(const Sys *__restrict__ sys, const Tile *__restrict__ tiles,
                                                          const T *v, T *q, const double *__restrict__ Sb,
                                                          double *__restrict__ part, int mode)
 {
@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(TR *TC) k_apply_walsh(const Sys *__restrict__ 
     if (!selected<T>(Sb, t.sys, mode)) return;
     const Sys &S = sys[t.sys];
     const T *lamu = (const T *)(sizeof(T) == 4 ? S.mat32 : S.mat64);
-    const double d = walsh_tile<T, ND, TR, TC, true>(v + S.off, q + S.off, S.keys, lamu, 0, S.ny, S.nx, S.nz, S.LZ,
+    const double d = walsh_tile<T, ND, TR, TC, true, 0, TILE_HALO>(v + S.off, q + S.off, S.keys, lamu, 0, S.ny, S.nx, S.nz, S.LZ,
                                                      t.bx, t.by, t.bz).dq;
     const double s = bsum(d);
     if (threadIdx.x == 0) part[ti] = s;
@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(TR *TC) k_apply_walsh(const Sys *__restrict__ 
 // slot j % w and reads slot (j - 1) % w: the direction carries over the
 // steps a system sits out and its residual replacements.
 template <int ND, int TR, int TC>
-__global__ void __launch_bounds__(TR *TC) k_apply_dir(const Sys *__restrict__ sys, const Tile *__restrict__ tiles,
+__global__ void __launch_bounds__(TR *TC + (TILE_HALO ? 64 : 0)) k_apply_dir(const Sys *__restrict__ sys, const Tile *__restrict__ tiles,
                                                        const float *z, float *ring, size_t total, int w, float *q,
                                                        const double *__restrict__ Sb, double *__restrict__ part)
 {
@@ -84,15 +84,87 @@ __global__ void __launch_bounds__(TR *TC) k_apply_dir(const Sys *__restrict__ sy
     fz.dnew = ring + (size_t)(j % uw) * total + S.off;
     fz.first = R[SB_FRESH] != 0.0 ? 1 : 0;
     fz.beta = fz.first ? 0.f : (float)(R[SB_DELTA] / R[SB_DELTA_OLD]);
-    const double d = walsh_tile<float, ND, TR, TC, true, 1>(dold + S.off, q + S.off, S.keys, (const float *)S.mat32, 0,
+    const double d = walsh_tile<float, ND, TR, TC, true, 1, TILE_HALO>(dold + S.off, q + S.off, S.keys, (const float *)S.mat32, 0,
                                                             S.ny, S.nx, S.nz, S.LZ, t.bx, t.by, t.bz, fz).dq;
     const double s = bsum(d);
     if (threadIdx.x == 0) part[ti] = s;
 }
 
+// Single-kernel step of the z-form loop (see apply_cg1 in batch.h); 4
+// partials per tile at part[k * ntiles + tile].
+template <int ND, int TR, int TC>
+__global__ void __launch_bounds__(TR *TC + (TILE_HALO ? 64 : 0)) k_apply_cg1(const Sys *__restrict__ sys, const Tile *__restrict__ tiles,
+                                                       int ntiles, float *z1, float *z2, float *q1, float *q2,
+                                                       float *ring, size_t total, int w,
+                                                       const double *__restrict__ Sb, double *__restrict__ part)
+{
+    const unsigned b = blockIdx.x, nb = gridDim.x, per = nb / 8u, rem = nb % 8u, xcd = b % 8u, k = b / 8u;
+    const unsigned ti = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + k;
+    const Tile t = tiles[ti];
+    if (!selected<float>(Sb, t.sys, OP_ITER)) return;
+    const Sys &S = sys[t.sys];
+    const double *R = Sb + t.sys * SB_STRIDE;
+    const unsigned j = (unsigned)R[SB_ITER], base = (unsigned)R[SB_BASE], uw = (unsigned)w;
+    float *zb[2] = {z2 + S.off, z1 + S.off}, *qb[2] = {q1 + S.off, q2 + S.off};
+    const bool first = R[SB_FRESH] != 0.0;
+    const bool pend = !first && j > base;
+    const bool zr = uw >= 3 && pend && j >= base + 2;
+    const float *dold = first ? nullptr : ring + (size_t)((j + uw - 1) % uw) * total + S.off;
+    DirFuse<float> fz{};
+    fz.z = zr ? ring + (size_t)((j + uw - 2) % uw) * total + S.off : zb[(j + 1) & 1];
+    fz.dnew = ring + (size_t)(j % uw) * total + S.off;
+    fz.first = first ? 1 : 0;
+    fz.beta = first ? 0.f : (float)(R[SB_DELTA] / R[SB_DELTA_OLD]);
+    fz.zpend = pend ? 1 : 0;
+    fz.qold = pend ? qb[(j + 1) & 1] : fz.z;
+    fz.alpha = pend ? (float)R[SB_ALPHA] : 0.f;
+    fz.znew = (uw >= 3 && j >= base + 1) ? nullptr : zb[j & 1];
+    fz.kinv = (const float *)S.kinv32;
+    fz.zring = zr ? 1 : 0;
+    fz.bprev = zr ? (float)R[SB_BPREV] : 0.f;
+    fz.lean = 0;
+    const TileSums ts = walsh_tile<float, ND, TR, TC, true, 2, TILE_HALO>(dold, qb[j & 1], S.keys, (const float *)S.mat32, 0,
+                                                             S.ny, S.nx, S.nz, S.LZ, t.bx, t.by, t.bz, fz);
+    double v[4] = {ts.dq, ts.zq, ts.qmq, ts.zdz};
+    block_sum<4>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) part[(size_t)c * ntiles + ti] = v[c];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // finishing kernels: one block per system sums its partial range
 // ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BB) k_finish_cg1(const Sys *__restrict__ sys, double *__restrict__ Sb,
+                                                   const double *__restrict__ part, int ntiles, int w)
+{
+    const int s = blockIdx.x;
+    double *R = Sb + s * SB_STRIDE;
+    if (R[SB_ACTIVE] == 0.0 || R[SB_DONE] != 0.0) return;
+    const Sys &S = sys[s];
+    double v[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        v[c] = 0.0;
+        for (int b = S.tb0 + (int)threadIdx.x; b < S.tb1; b += blockDim.x) v[c] += part[(size_t)c * ntiles + b];
+    }
+    block_sum<4>(v);
+    if (threadIdx.x != 0) return;
+    // as FIN_CG1 of the single-system loop: this step's delta from its own
+    // z, alpha, the next r.z by the recurrence
+    const double delta = v[3], alpha = delta / v[0];
+    const double dnew = delta - 2.0 * alpha * v[1] + alpha * alpha * v[2];
+    R[SB_BPREV] = R[SB_DELTA] / R[SB_DELTA_OLD];
+    R[SB_DQ] = v[0];
+    R[SB_ALPHA] = alpha;
+    R[SB_DELTA_OLD] = delta;
+    R[SB_DELTA] = dnew;
+    if (w > 0) R[SB_AHIST + (unsigned)R[SB_ITER] % (unsigned)w] = alpha;
+    R[SB_ITER] += 1.0;
+    R[SB_FRESH] = 0.0;
+    if (fabs(dnew) <= R[SB_THRESH]) R[SB_DONE] = 1.0;
+}
 __global__ void __launch_bounds__(BB) k_finish(const Sys *__restrict__ sys, double *__restrict__ Sb,
                                                const double *__restrict__ part, int what, int w)
 {
@@ -252,11 +324,12 @@ __global__ void k_clear_alpha(double *Sb, int w)
 // r.z (systems with `start` set)
 __global__ void __launch_bounds__(BB) k_start_z(const Sys *__restrict__ sys, const Chunk *__restrict__ chunks,
                                                 const double *__restrict__ Sb, const double *__restrict__ r64,
-                                                float *z32, float *x32, double *__restrict__ part, int nd)
+                                                float *z32, float *x32, double *__restrict__ part, int nd, float *z2)
 {
     const Chunk ch = chunks[blockIdx.x];
     const double *R = Sb + ch.sys * SB_STRIDE;
     if (R[SB_START] == 0.0) return;
+    if (z2 && ((unsigned)R[SB_ITER] & 1u)) z32 = z2;
     const Sys &S = sys[ch.sys];
     const uint32_t i = ch.i0 + threadIdx.x;
     double acc = 0;
@@ -376,10 +449,11 @@ __global__ void __launch_bounds__(BB) k_resid(const Sys *__restrict__ sys, const
 __global__ void __launch_bounds__(BB) k_resid_z(const Sys *__restrict__ sys, const Chunk *__restrict__ chunks,
                                                 const double *__restrict__ Sb, double *r64,
                                                 const double *__restrict__ q64, float *z32, double *__restrict__ part,
-                                                int nd)
+                                                int nd, float *z2)
 {
     const Chunk ch = chunks[blockIdx.x];
     if (Sb[ch.sys * SB_STRIDE + SB_REPL] == 0.0) return;
+    if (z2 && ((unsigned)Sb[ch.sys * SB_STRIDE + SB_ITER] & 1u)) z32 = z2;
     const Sys &S = sys[ch.sys];
     const uint32_t i = ch.i0 + threadIdx.x;
     double acc = 0;
@@ -407,9 +481,9 @@ void apply(const Batch &b, const T *v, T *q, int mode, hipStream_t s)
 {
     if (b.ntiles == 0) return;
     if (b.nd == 3)
-        k_apply_walsh<T, 3, TILE_R, TILE_C><<<b.ntiles, TILE_R * TILE_C, 0, s>>>(b.sys, b.tiles, v, q, b.Sb, b.part, mode);
+        k_apply_walsh<T, 3, TILE_R, TILE_C><<<b.ntiles, TILE_THREADS, 0, s>>>(b.sys, b.tiles, v, q, b.Sb, b.part, mode);
     else
-        k_apply_walsh<T, 1, TILE_R, TILE_C><<<b.ntiles, TILE_R * TILE_C, 0, s>>>(b.sys, b.tiles, v, q, b.Sb, b.part, mode);
+        k_apply_walsh<T, 1, TILE_R, TILE_C><<<b.ntiles, TILE_THREADS, 0, s>>>(b.sys, b.tiles, v, q, b.Sb, b.part, mode);
 }
 template void apply<float>(const Batch &, const float *, float *, int, hipStream_t);
 template void apply<double>(const Batch &, const double *, double *, int, hipStream_t);
@@ -428,10 +502,10 @@ void apply_dir(const Batch &b, const float *z, float *ring, size_t total, int w,
 {
     if (b.ntiles == 0) return;
     if (b.nd == 3)
-        k_apply_dir<3, TILE_R, TILE_C><<<b.ntiles, TILE_R * TILE_C, 0, s>>>(b.sys, b.tiles, z, ring, total, w, q, b.Sb,
+        k_apply_dir<3, TILE_R, TILE_C><<<b.ntiles, TILE_THREADS, 0, s>>>(b.sys, b.tiles, z, ring, total, w, q, b.Sb,
                                                                           b.part);
     else
-        k_apply_dir<1, TILE_R, TILE_C><<<b.ntiles, TILE_R * TILE_C, 0, s>>>(b.sys, b.tiles, z, ring, total, w, q, b.Sb,
+        k_apply_dir<1, TILE_R, TILE_C><<<b.ntiles, TILE_THREADS, 0, s>>>(b.sys, b.tiles, z, ring, total, w, q, b.Sb,
                                                                           b.part);
 }
 
@@ -447,14 +521,27 @@ void ring_flush(const Batch &b, float *x32, const float *ring, size_t total, int
     k_clear_alpha<<<b.nsys, SB_RING_MAX, 0, s>>>(b.Sb, w);
 }
 
-void start_z(const Batch &b, const double *r64, float *z32, float *x32, hipStream_t s)
+void start_z(const Batch &b, const double *r64, float *z32, float *x32, hipStream_t s, float *z2)
 {
-    k_start_z<<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, r64, z32, x32, b.part, b.nd);
+    k_start_z<<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, r64, z32, x32, b.part, b.nd, z2);
 }
 
-void resid_z(const Batch &b, double *r64, const double *q64, float *z32, hipStream_t s)
+void resid_z(const Batch &b, double *r64, const double *q64, float *z32, hipStream_t s, float *z2)
 {
-    k_resid_z<<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, r64, q64, z32, b.part, b.nd);
+    k_resid_z<<<b.nchunks, BB, 0, s>>>(b.sys, b.chunks, b.Sb, r64, q64, z32, b.part, b.nd, z2);
+}
+
+void apply_cg1(const Batch &b, float *z, float *z2, float *q, float *q2, float *ring, size_t total, int w,
+               hipStream_t s)
+{
+    if (b.ntiles == 0) return;
+    if (b.nd == 3)
+        k_apply_cg1<3, TILE_R, TILE_C><<<b.ntiles, TILE_THREADS, 0, s>>>(b.sys, b.tiles, b.ntiles, z, z2, q, q2, ring,
+                                                                           total, w, b.Sb, b.part);
+    else
+        k_apply_cg1<1, TILE_R, TILE_C><<<b.ntiles, TILE_THREADS, 0, s>>>(b.sys, b.tiles, b.ntiles, z, z2, q, q2, ring,
+                                                                           total, w, b.Sb, b.part);
+    k_finish_cg1<<<b.nsys, BB, 0, s>>>(b.sys, b.Sb, b.part, b.ntiles, w);
 }
 
 void dir(const Batch &b, float *d, const float *r, hipStream_t s)

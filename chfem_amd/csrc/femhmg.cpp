@@ -1524,6 +1524,8 @@ struct BatchRun {
     std::vector<BatchSample> samples;
     int analysis = -1;
     double seconds = 0;
+    // learned initial guess for every system (nullptr: zero guess)
+    void *net = nullptr;
 };
 
 template <typename T> void bfree(T *&p)
@@ -1669,6 +1671,77 @@ int chfemBatchAddSample(void *h, const char *nf_file, const uint8_t *img)
     return 1;
 }
 
+}  // extern "C"
+
+// Learned initial guesses of a batch (see chfemBatchSetInitialGuessNetwork):
+// x64 of every system of a sample = ramp + the network's prediction for its
+// load case, then one FP64 replacement step (r64 -= A x64, r.M^-1 r per
+// system).  A system whose guess leaves a larger residual than b goes back
+// to x = 0, r = b.  Leaves R[SB_DELTA] = r.M^-1 r of the guess and x0used
+// set where a guess was kept.
+static bool batch_guess(BatchRun *B, const batch::Batch &bt, const std::vector<batch::Sys> &sys,
+                        const std::vector<std::pair<int, int>> &owner, int nd, double *x64, double *r64, double *q64,
+                        float *r32, std::vector<double> &Sh, std::vector<double> &x0res, std::vector<int> &x0used,
+                        hipStream_t st)
+{
+    using namespace batch;
+    const int nsys = (int)sys.size();
+    auto R = [&](int s, int slot) -> double & { return Sh[(size_t)s * SB_STRIDE + slot]; };
+    std::vector<int> has(nsys, 0);
+    for (size_t si = 0; si < B->samples.size(); si++) {
+        const Model &m = *B->samples[si].m;
+        int gC = 0;
+        const float *gy = guess_predict_device(B->net, m.elem_map.data(), m.props, m.nmat, m.ex, m.ey, m.ez, &gC);
+        if (!gy || hipStreamSynchronize(guess_stream(B->net)) != hipSuccess) continue;
+        for (int s = 0; s < nsys; s++) {
+            if (owner[s].first != (int)si || fabs(R(s, SB_DELTA0)) < 1e-12) continue;
+            nn::guess_to_x0(gy, gC, owner[s].second, nd, nn::Grid3{(int)m.ey, (int)m.ex, (int)m.ez}, 1.0f,
+                            x64 + sys[s].off, st);
+            has[s] = 1;
+        }
+        // the next prediction reuses the network's output buffer
+        if (hipStreamSynchronize(st) != hipSuccess) return false;
+    }
+    for (int s = 0; s < nsys; s++) R(s, SB_REPL) = has[s];
+    if (hipMemcpyAsync((double *)bt.Sb, Sh.data(), Sh.size() * 8, hipMemcpyHostToDevice, st) != hipSuccess) return false;
+    apply<double>(bt, x64, q64, OP_REPL, st);
+    resid(bt, r64, q64, r32, st);
+    finish(bt, FIN_B_TRUE, st);
+    if (hipMemcpyAsync(Sh.data(), bt.Sb, Sh.size() * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return false;
+    for (int s = 0; s < nsys; s++) {
+        if (!has[s]) continue;
+        R(s, SB_REPL) = 0.0;
+        const double dl = R(s, SB_DELTA), d0 = R(s, SB_DELTA0);
+        x0res[s] = sqrt(fabs(dl) / fabs(d0));
+        if (fabs(dl) > fabs(d0)) {
+            const size_t len = (size_t)nd * sys[s].N;
+            launch_add<double>(r64 + sys[s].off, q64 + sys[s].off, len, st);
+            launch_fill<double>(x64 + sys[s].off, 0.0, len, st);
+            R(s, SB_DELTA) = d0;
+        } else {
+            x0used[s] = 1;
+        }
+    }
+    return hipMemcpyAsync((double *)bt.Sb, Sh.data(), Sh.size() * 8, hipMemcpyHostToDevice, st) == hipSuccess &&
+           hipStreamSynchronize(st) == hipSuccess;
+}
+
+extern "C" {
+
+int chfemBatchSetInitialGuessNetwork(void *h, void *net)
+{
+    BatchRun *B = (BatchRun *)h;
+    if (!B) return 0;
+    if (net && guess_physics(net) != (B->analysis == HMG_ELASTIC ? 1 : 0) && B->analysis >= 0) {
+        fprintf(stderr, "chfem: the guess network's physics differs from the batch's analysis\n");
+        return 0;
+    }
+    B->net = net;
+    return 1;
+}
+
 int chfemBatchSolve(void *h, double tol, unsigned int maxit)
 {
     using namespace batch;
@@ -1688,7 +1761,7 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
             Sys S;
             memset(&S, 0, sizeof(S));
             S.ny = (int)m.ey; S.nx = (int)m.ex; S.nz = (int)m.ez;
-            S.LZ = 16;
+            S.LZ = TILE_LZ;
             S.N = (uint32_t)m.nelem;
             S.off = off;
             S.keys = B->samples[si].keys;
@@ -1697,7 +1770,7 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
             const int sid = (int)sys.size();
             S.tb0 = (int)tiles.size();
             {
-                const int gx = (S.ny + TILE_R - 2) / (TILE_R - 1), gy = (S.nx + TILE_C - 2) / (TILE_C - 1);
+                const int gx = (S.ny + TILE_ROWS - 1) / TILE_ROWS, gy = (S.nx + TILE_C - 2) / (TILE_C - 1);
                 const int gz = (S.nz + S.LZ - 1) / S.LZ;
                 for (int z = 0; z < gz; z++)
                     for (int y = 0; y < gy; y++)
@@ -1725,7 +1798,7 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
     bool ok = bupload(dsys, sys.data(), sys.size()) && bupload(dtiles, tiles.data(), tiles.size()) &&
               bupload(dchunks, chunks.data(), chunks.size()) &&
               hipMalloc(&Sb, sizeof(double) * nsys * SB_STRIDE) == hipSuccess &&
-              hipMalloc(&part, sizeof(double) * std::max(tiles.size(), chunks.size())) == hipSuccess &&
+              hipMalloc(&part, sizeof(double) * std::max(4 * tiles.size(), chunks.size())) == hipSuccess &&
               hipMalloc(&x64, total * 8) == hipSuccess && hipMalloc(&r64, total * 8) == hipSuccess &&
               hipMalloc(&e64, total * 8) == hipSuccess && hipMalloc(&q64, total * 8) == hipSuccess &&
               hipMalloc(&x32, total * 4) == hipSuccess && hipMalloc(&r32, total * 4) == hipSuccess &&
@@ -1737,12 +1810,32 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
     // synchronisations).  Without room for two slots: the separate
     // direction, operator and update kernels on r.
     int w = 0;
-    for (int cand = SB_RING_MAX; ok && cand >= 2 && !w; cand /= 2)
+    // CHFEM_BATCH_RING caps the width (16, 8, 4 or 2: a width divides the 16
+    // steps between host synchronisations, where the window is flushed)
+    int wmax = SB_RING_MAX;
+    if (const char *re = getenv("CHFEM_BATCH_RING"))
+        while (wmax > 2 && wmax > atoi(re)) wmax /= 2;
+    for (int cand = wmax; ok && cand >= 2 && !w; cand /= 2)
         if (hipMalloc(&ring, total * 4 * (size_t)cand) == hipSuccess) w = cand;
         else (void)hipGetLastError();
     const bool zform = w >= 2;
     if (zform) ok = ok && hipMemsetAsync(ring, 0, total * 4 * (size_t)w, st) == hipSuccess;
     else ok = ok && hipMalloc(&d32, total * 4) == hipSuccess;
+    // single-kernel steps (the default; CHFEM_BATCH_FORM=1 keeps the
+    // operator with the direction update and a separate z update): a second
+    // z and q buffer, which z and q alternate with by each system's
+    // iteration parity
+    float *z2 = nullptr, *q2 = nullptr;
+    const char *bf = getenv("CHFEM_BATCH_FORM");
+    bool cg1 = zform && !(bf && atoi(bf) == 1);
+    if (ok && cg1) {
+        if (hipMalloc(&z2, total * 4) != hipSuccess || hipMalloc(&q2, total * 4) != hipSuccess) {
+            (void)hipGetLastError();
+            bfree(z2);
+            bfree(q2);
+            cg1 = false;
+        }
+    }
     Batch bt;
     bt.nsys = nsys; bt.nd = nd; bt.sys = dsys; bt.tiles = dtiles; bt.ntiles = (int)tiles.size();
     bt.chunks = dchunks; bt.nchunks = (int)chunks.size(); bt.Sb = Sb; bt.part = part;
@@ -1758,7 +1851,8 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
     auto R = [&](int s, int slot) -> double & { return Sh[(size_t)s * SB_STRIDE + slot]; };
     hipEvent_t e0 = nullptr, e1 = nullptr;
     ok = ok && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
-    std::vector<double> d0v(nsys, 0.0), delta_rel(nsys, 0.0);
+    std::vector<double> d0v(nsys, 0.0), delta_rel(nsys, 0.0), x0res(nsys, 0.0);
+    std::vector<int> x0used(nsys, 0);
     std::vector<unsigned> repl(nsys, 0);
     std::vector<int> conv(nsys, 0);
     if (ok) {
@@ -1775,10 +1869,14 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
         ok = up();
         finish(bt, FIN_B_DELTA0, st);
         ok = ok && down();
+        // learned initial guess: x64 = ramp + prediction for every system of
+        // a sample the network can predict; the FP64 residual b - A x0 then
+        // replaces b, unless it is larger than b's (the zero guess)
+        if (ok && B->net) ok = batch_guess(B, bt, sys, owner, nd, x64, r64, q64, r32, Sh, x0res, x0used, st);
         // fresh FP32 start of every system with a non-zero right-hand side
         for (int s = 0; s < nsys; s++) {
             d0v[s] = R(s, SB_DELTA0);
-            delta_rel[s] = d0v[s];
+            delta_rel[s] = x0used[s] ? R(s, SB_DELTA) : d0v[s];
             const bool live = fabs(d0v[s]) >= 1e-12;
             conv[s] = live ? 0 : 1;
             R(s, SB_ACTIVE) = live;
@@ -1787,10 +1885,17 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
             R(s, SB_DONE) = 0.0;
             R(s, SB_REPL) = 0.0;
             R(s, SB_ITER) = 0.0;
-            R(s, SB_THRESH) = std::max(tol * tol * fabs(d0v[s]), drop * fabs(d0v[s]));
+            R(s, SB_BASE) = 0.0;
+            R(s, SB_THRESH) = std::max(tol * tol * fabs(d0v[s]), drop * fabs(delta_rel[s]));
+            if (live && x0used[s] && fabs(delta_rel[s]) <= tol * tol * fabs(d0v[s])) {
+                // the guess already meets the tolerance
+                conv[s] = 1;
+                R(s, SB_ACTIVE) = 0.0;
+                R(s, SB_START) = 0.0;
+            }
         }
         ok = ok && up();
-        if (zform) start_z(bt, r64, r32, x32, st);
+        if (zform) start_z(bt, r64, r32, x32, st, cg1 ? z2 : nullptr);
         else start(bt, r64, r32, d32, x32, st);
         finish(bt, FIN_B_START, st);
         ok = ok && down();
@@ -1803,6 +1908,11 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
         for (int s = 0; s < nsys; s++) live += R(s, SB_ACTIVE) != 0.0;
         if (!live) break;
         for (unsigned k = 0; k < nb; k++) {
+            if (cg1) {
+                apply_cg1(bt, r32, z2, q32, q2, ring, total, w, st);
+                if ((k + 1) % (unsigned)w == 0) ring_flush(bt, x32, ring, total, w, st);
+                continue;
+            }
             if (zform) {
                 apply_dir(bt, r32, ring, total, w, q32, st);
                 finish_alpha(bt, w, st);
@@ -1832,7 +1942,7 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
         // residual replacement of the selected systems
         flush(bt, x64, e64, x32, st);
         apply<double>(bt, e64, q64, OP_REPL, st);
-        if (zform) resid_z(bt, r64, q64, r32, st);
+        if (zform) resid_z(bt, r64, q64, r32, st, cg1 ? z2 : nullptr);
         else resid(bt, r64, q64, r32, st);
         finish(bt, FIN_B_TRUE, st);
         if (!(ok = down())) break;
@@ -1849,6 +1959,7 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
             } else {
                 delta_rel[s] = dl;
                 R(s, SB_DONE) = 0.0;
+                R(s, SB_BASE) = R(s, SB_ITER);
                 R(s, SB_THRESH) = std::max(tol * tol * fabs(d0v[s]), drop * fabs(dl));
             }
         }
@@ -1885,8 +1996,8 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
         stt.sweeps = repl[s];
         stt.solve_seconds = B->seconds;
         stt.residual = sqrt(fabs(R(s, SB_DELTA)) / std::max(fabs(d0v[s]), 1e-300));
-        stt.x0_residual = 0;
-        stt.x0_used = 0;
+        stt.x0_residual = x0res[s];
+        stt.x0_used = (unsigned)x0used[s];
     }
     int all = 1;
     for (int s = 0; s < nsys; s++) all &= conv[s];
@@ -1894,6 +2005,7 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
     if (e1) (void)hipEventDestroy(e1);
     bfree(dsys); bfree(dtiles); bfree(dchunks); bfree(Sb); bfree(part); bfree(x64); bfree(r64); bfree(e64);
     bfree(q64); bfree(x32); bfree(r32); bfree(d32); bfree(q32); bfree(ring); bfree(rpart); bfree(rS);
+    bfree(z2); bfree(q2);
     (void)hipStreamDestroy(st);
     if (!ok) return 0;
     return all ? 1 : 2;

@@ -290,6 +290,7 @@ const float *guess_predict_device(void *h, const uint16_t *elem_host, const doub
 }
 
 hipStream_t guess_stream(void *h) { return h ? as_net(h)->s : nullptr; }
+int guess_physics(void *h) { return h ? as_net(h)->physics : -1; }
 }  // namespace chfem
 
 extern "C" {

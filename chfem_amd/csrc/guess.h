@@ -82,5 +82,7 @@ extern void *guess_active;
 const float *guess_predict_device(void *net, const uint16_t *elem_host, const double *props, unsigned nmat,
                                   unsigned nx, unsigned ny, unsigned nz, int *Cout);
 hipStream_t guess_stream(void *net);
+// physics of a network (0 thermal, 1 elastic), -1 for nullptr
+int guess_physics(void *net);
 
 }  // namespace chfem

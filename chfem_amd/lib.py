@@ -136,6 +136,7 @@ SIGNATURES = [
     ("chfemBatchGetC", C.c_int, [vp, uint, vp]),
     ("chfemBatchGetStats", C.c_int, [vp, uint, vp, C.POINTER(C.c_uint)]),
     ("chfemBatchSeconds", dbl, [vp]),
+    ("chfemBatchSetInitialGuessNetwork", C.c_int, [vp, vp]),
 ]
 
 

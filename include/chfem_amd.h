@@ -170,6 +170,13 @@ int chfemBatchSolve(void *batch, double tol, unsigned int maxit);
 int chfemBatchGetC(void *batch, unsigned int sample, double *C);
 int chfemBatchGetStats(void *batch, unsigned int sample, chfemSolveStats_t *stats, unsigned int *count);
 double chfemBatchSeconds(void *batch);
+/* Learned initial guess for every system of a batch (no reference
+ * counterpart): each sample's load cases start from ramp + the network's
+ * prediction (chfemGuessCreate, same physics as the batch; grids multiples of
+ * 8), with the zero guess kept for a system whose guess leaves a larger
+ * residual (chfemSolveStats_t x0_used / x0_residual).  NULL: zero guesses.
+ * Returns 1 on success. */
+int chfemBatchSetInitialGuessNetwork(void *batch, void *net);
 
 /* BF16 matrix-core lane-layout check: D = A * B for 16x16 row-major BF16
  * A and B, FP32 D. */

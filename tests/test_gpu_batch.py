@@ -55,3 +55,51 @@ def test_batch_homogeneous_and_tiny():
     m = H.model(H.ELASTIC, b, [1.0, 0.3, 3.0, 0.3])
     Cr, _ = m.homogenize(tol=1e-11)
     assert relmax(Cs[1], Cr) < 1e-6
+
+
+@pytest.mark.parametrize("form", ["2", "1"], ids=["single-kernel", "dir-fused"])
+@pytest.mark.parametrize("ring", ["16", "4", "2"])
+def test_batch_step_forms_and_rings(form, ring, monkeypatch):
+    """The batch loop's two step forms (the single-kernel step with z
+    recovered from the ring, the default; the direction-fused operator with a
+    separate z update) and ring widths down to 2 (z stored instead of
+    recovered) give the oracle's tensors."""
+    monkeypatch.setenv("CHFEM_BATCH_FORM", form)
+    monkeypatch.setenv("CHFEM_BATCH_RING", ring)
+    samples = []
+    for k, shape in enumerate(SHAPES[:3]):
+        img = synthetic.spheres(shape, rmin=1, rmax=3, fraction=0.3, seed=80 + k)
+        samples.append((img, {0: (1.0, 0.3), 255: (12.0, 0.2)}))
+    Cs, info = batch.batch_homogenize(samples, "elasticity", tol=1e-8)
+    assert info["converged"], info
+    for (img, mp), C in zip(samples, Cs):
+        Cr, _ = H.model(H.ELASTIC, img, [*mp[0], *mp[255]]).homogenize(tol=1e-10)
+        assert relmax(C, Cr) < 1e-6
+
+
+def test_batch_learned_guess():
+    """Learned initial guesses in a batch: a network trained on a few
+    solutions of the same recipe starts every load case below the zero
+    guess's residual, the tensors still meet the oracle's, and the guessed
+    batch takes fewer iterations."""
+    from chfem_amd import guess
+    el = {0: (1.0, 0.3), 255: (10.0, 0.2)}
+    train = [guess.solve_fluctuations(synthetic.spheres((16, 16, 16), rmin=2, rmax=4, fraction=0.3, seed=200 + k),
+                                      el, "elastic", tol=1e-7) for k in range(8)]
+    net = guess.LearnedGuess("elastic", seed=1)
+    net.train(train, epochs=25, lr=3e-3, batch=4)
+    samples = [(synthetic.spheres(s, rmin=2, rmax=4, fraction=0.3, seed=300 + k), el)
+               for k, s in enumerate([(16, 16, 16), (24, 16, 16)])]
+    C0, i0 = batch.batch_homogenize(samples, "elasticity", tol=1e-8)
+    C1, i1 = batch.batch_homogenize(samples, "elasticity", tol=1e-8, guess=net)
+    assert i0["converged"] and i1["converged"]
+    for (img, mp), Ca, Cb in zip(samples, C0, C1):
+        Cr, _ = H.model(H.ELASTIC, img, [*mp[0], *mp[255]]).homogenize(tol=1e-10)
+        assert relmax(Ca, Cr) < 1e-6 and relmax(Cb, Cr) < 1e-6
+    used = [s[3] for st in i1["stats"] for s in st]
+    res = [s[4] for st in i1["stats"] for s in st]
+    it0 = sum(s[0] for st in i0["stats"] for s in st)
+    it1 = sum(s[0] for st in i1["stats"] for s in st)
+    print(f"batch zero guess {it0} iterations, learned guess {it1} (x0 residuals {np.round(res, 3).tolist()})")
+    assert all(u == 1 for u in used) and max(res) < 1.0
+    assert it1 < it0
