@@ -5,11 +5,10 @@
 // thread and scatters them with atomics).
 //
 // Per element layer a block of TR x TC work-items (4 wavefronts for 32x8)
-// owns TR x TC elements.  Each wavefront multiplies the stacked local
-// matrices [K_0 | K_1 | ...] (KE x KE*nmat) by the material-masked element
-// vectors of its 64 elements (KE*nmat x 64) with v_mfma_{f32,f64}_16x16x4:
-// row tiles of 16 local DOFs, column tiles of 16 elements, 4 stacked DOFs per
-// step; material blocks absent from the wavefront's elements are skipped.
+// owns TR x TC elements.  Each wavefront groups its 64 elements by material
+// into column tiles of 16 and multiplies each tile by its material's local
+// matrix (KE x KE) with v_mfma_{f32,f64}_16x16x4: row tiles of 16 local
+// DOFs, 4 DOFs of the reduction per step.
 // Element forces land in LDS and every interior node sums its 8 element
 // contributions there, as in the Walsh-basis kernel (walsh.hip): tiles
 // overlap by one node row/column, nothing is scattered with atomics, and the
@@ -71,6 +70,9 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
     __shared__ T sf[NT * SF];
     __shared__ T sK[4 * KE * KE];              // K[m][row][col]
     __shared__ int smat[NT];
+    // per wavefront: block element at each position of the material-grouped
+    // column tiles (at most 64 elements in at most 7 tiles)
+    __shared__ int sperm[NT / 64][112];
     __shared__ T sesc[NT];  // scalar density field: element scale
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6, li = lane & 15, kg = lane >> 4;
@@ -114,61 +116,65 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
     };
 
     // element forces of the wavefront's 64 elements (block elements 64*wave ..)
+    // The elements are grouped by material into column tiles of 16: material
+    // m's c_m elements fill ceil(c_m / 16) tiles (lane order), so every
+    // element is multiplied by its own material's matrix only.  sperm holds,
+    // per wavefront, the block element at each tile position.
     auto elements = [&](const T *cur, const T *nxt) {
-        // materials present in this wavefront
         const int me = smat[tid];
-        unsigned present = 0;
+        const unsigned long long below = (1ull << lane) - 1ull;
+        int cnt[4], tb[4];
+        int ntiles = 0;
 #pragma unroll
-        for (int m = 0; m < 4; m++)
-            if (__ballot(me == m)) present |= 1u << m;
-        typename MfmaAcc<T>::type acc[MT][4];
-#pragma unroll
-        for (int mt = 0; mt < MT; mt++)
-#pragma unroll
-            for (int nt = 0; nt < 4; nt++) acc[mt][nt] = typename MfmaAcc<T>::type{0, 0, 0, 0};
-        // the 4 elements this lane feeds as B columns: block element 64*wave + 16*nt + li
-        int ebase[4], emat[4];
-        T esc[4];
-#pragma unroll
-        for (int nt = 0; nt < 4; nt++) {
-            const int e = 64 * wave + 16 * nt + li;
-            ebase[nt] = ((e / TR) * NR + (e % TR)) * ND;
-            emat[nt] = smat[e];
-            esc[nt] = sesc[e];
-        }
         for (int m = 0; m < 4; m++) {
-            if (!((present >> m) & 1u)) continue;
+            const unsigned long long bal = __ballot(me == m);
+            cnt[m] = __popcll(bal);
+            tb[m] = ntiles;
+            if (me == m) sperm[wave][16 * ntiles + __popcll(bal & below)] = tid;
+            ntiles += (cnt[m] + 15) >> 4;
+        }
+        __syncthreads();  // sperm written before any lane reads it
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            if (!cnt[m]) continue;
             const T *Km = sK + m * KE * KE;
+            for (int t = 0; t < ((cnt[m] + 15) >> 4); t++) {
+                // column li of this tile: the material's element 16 t + li, if any
+                const bool valid = 16 * t + li < cnt[m];
+                const int e = valid ? sperm[wave][16 * (tb[m] + t) + li] : 0;
+                const int eb = ((e / TR) * NR + (e % TR)) * ND;
+                const T esc = valid ? sesc[e] : T(0);
+                typename MfmaAcc<T>::type acc[MT];
 #pragma unroll
-            for (int s = 0; s < KE / 4; s++) {
-                const int d = 4 * s + kg;           // stacked DOF of this lane in the step
-                const int kn = d / ND, comp = d % ND;
-                const T *pl = edl(kn) ? nxt : cur;
-                const int doff = (edc(kn) * NR + edr(kn)) * ND + comp;
-                T b[4];
+                for (int mt = 0; mt < MT; mt++) acc[mt] = typename MfmaAcc<T>::type{0, 0, 0, 0};
 #pragma unroll
-                for (int nt = 0; nt < 4; nt++) b[nt] = emat[nt] == m ? esc[nt] * pl[ebase[nt] + doff] : T(0);
+                for (int s = 0; s < KE / 4; s++) {
+                    const int d = 4 * s + kg;           // local DOF of this lane in the step
+                    const int kn = d / ND, comp = d % ND;
+                    const T *pl = edl(kn) ? nxt : cur;
+                    const int doff = (edc(kn) * NR + edr(kn)) * ND + comp;
+                    const T b = esc * pl[eb + doff];
 #pragma unroll
-                for (int mt = 0; mt < MT; mt++) {
-                    const int row = 16 * mt + li;
-                    const T a = row < KE ? Km[row * KE + d] : T(0);
+                    for (int mt = 0; mt < MT; mt++) {
+                        const int row = 16 * mt + li;
+                        const T a = row < KE ? Km[row * KE + d] : T(0);
+                        acc[mt] = MfmaAcc<T>::mac(a, b, acc[mt]);
+                    }
+                }
+                // D[row][col]: this lane holds column li (element e) and 4 rows
+                // of each row tile
+                if (valid) {
+                    T *dst = sf + e * SF;
 #pragma unroll
-                    for (int nt = 0; nt < 4; nt++) acc[mt][nt] = MfmaAcc<T>::mac(a, b[nt], acc[mt][nt]);
+                    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const int row = 16 * mt + MfmaAcc<T>::row(kg, i);
+                            if (row < KE) dst[row] = acc[mt][i];
+                        }
                 }
             }
         }
-        // D[row][col]: this lane holds column li (element) and 4 rows of each row tile
-#pragma unroll
-        for (int mt = 0; mt < MT; mt++)
-#pragma unroll
-            for (int nt = 0; nt < 4; nt++) {
-                T *dst = sf + (64 * wave + 16 * nt + li) * SF;
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int row = 16 * mt + MfmaAcc<T>::row(kg, i);
-                    if (row < KE) dst[row] = acc[mt][nt][i];
-                }
-            }
     };
     const bool own = tr >= 1 && tc >= 1 && tr - 1 < vr && tc - 1 < vc;
     auto fslot = [&](int k) { return ((tc - EDC(k)) * TR + (tr - EDR(k))) * SF + ND * k; };

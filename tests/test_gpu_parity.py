@@ -114,11 +114,13 @@ def test_fluid_mixed_precision(porous, kernel, monkeypatch):
         s.close()
 
 
-@pytest.mark.parametrize("shape", [(40, 70, 9), (17, 33, 65), (3, 5, 4)])
-def test_operator_fluid_tiles(shape, monkeypatch):
-    """Walsh-basis Stokes operator (FP64) on ragged, multi-block and tiny
-    grids with random pore spaces (several tiles per direction, layer
-    chunks), against the oracle."""
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("shape", [(40, 70, 9), (17, 33, 65), (3, 5, 4), (9, 130, 14)])
+def test_operator_fluid_tiles(shape, precision, monkeypatch):
+    """Walsh-basis Stokes operator on ragged, multi-block and tiny grids with
+    random pore spaces (several tiles per direction, layer chunks), against
+    the oracle: FP64 to 1e-12, FP32 on an FP32-representable input to 1e-5
+    of the largest entry."""
     monkeypatch.setenv("CHFEM_WALSH_TILE", "32x8x7")
     rng = np.random.default_rng(sum(shape))
     img = np.where(rng.random(shape) < 0.35, 255, 0).astype(np.uint8)
@@ -129,7 +131,12 @@ def test_operator_fluid_tiles(shape, monkeypatch):
     s = H.GpuSolver(m)
     try:
         v = rng.standard_normal(m.ndof)
-        assert rel(s.apply(v), m.apply(v)) < 1e-12
+        if precision == "fp64":
+            assert rel(s.apply(v), m.apply(v)) < 1e-12
+        else:
+            v32 = v.astype(np.float32)
+            ref = m.apply(v32.astype(np.float64))
+            assert np.abs(s.apply32(v32) - ref).max() < 1e-5 * np.abs(ref).max()
     finally:
         s.close()
 
