@@ -66,9 +66,13 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
     constexpr int NT = TR * TC, NR = TR + 1, NC = TC + 1, NV = NR * NC * ND;
     constexpr int SLOTS = (NV + NT - 1) / NT;
     constexpr int SF = KE + 1;                 // padded element stride of the force buffer
+    // padded row stride of the local matrices: the A operand's 16 rows per
+    // step then fall on distinct LDS banks (a stride of KE = 24 or 8 put
+    // every fourth row on the same bank)
+    constexpr int KP = KE + 1;
     __shared__ T su[2][NV];
     __shared__ T sf[NT * SF];
-    __shared__ T sK[4 * KE * KE];              // K[m][row][col]
+    __shared__ T sK[4 * KE * KP];              // K[m][row][col], row stride KP
     __shared__ int smat[NT];
     // per wavefront: block element at each position of the material-grouped
     // column tiles (at most 64 elements in at most 7 tiles)
@@ -80,7 +84,7 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
     const int r0 = blockIdx.x * (TR - 1), c0 = blockIdx.y * (TC - 1), l0 = blockIdx.z * LZ;
     const int vr = min(TR - 1, ny - r0), vc = min(TC - 1, nx - c0), vl = min(LZ, nz - l0);
     const size_t nxy = (size_t)nx * ny;
-    for (int i = tid; i < nmat * KE * KE; i += NT) sK[i] = K[i];
+    for (int i = tid; i < nmat * KE * KE; i += NT) sK[(i / KE) * KP + i % KE] = K[i];
 
     uint32_t goff[SLOTS];
     bool gval[SLOTS];
@@ -137,7 +141,7 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             if (!cnt[m]) continue;
-            const T *Km = sK + m * KE * KE;
+            const T *Km = sK + m * KE * KP;
             for (int t = 0; t < ((cnt[m] + 15) >> 4); t++) {
                 // column li of this tile: the material's element 16 t + li, if any
                 const bool valid = 16 * t + li < cnt[m];
@@ -157,7 +161,7 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
 #pragma unroll
                     for (int mt = 0; mt < MT; mt++) {
                         const int row = 16 * mt + li;
-                        const T a = row < KE ? Km[row * KE + d] : T(0);
+                        const T a = row < KE ? Km[row * KP + d] : T(0);
                         acc[mt] = MfmaAcc<T>::mac(a, b, acc[mt]);
                     }
                 }

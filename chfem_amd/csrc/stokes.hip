@@ -23,12 +23,15 @@
 
 namespace chfem {
 
+// Element forces f = K_e u (scale: 1, or 0 for an element that contributes
+// nothing, which then yields zero forces without a branch around the
+// evaluation)
 template <typename T>
-__device__ __forceinline__ void stokes_forces(const T (&u)[8][4], T (&f)[8][4])
+__device__ __forceinline__ void stokes_forces(const T (&u)[8][4], T (&f)[8][4], T scale = T(1))
 {
     T c[8];
     Walsh<T> Wu, Wv, Ww, Wp;
-    T P1 = T(0);
+    T P1;
 #pragma unroll
     for (int k = 0; k < 8; k++) c[k] = u[k][0];
     Wu = wht(c);
@@ -39,13 +42,11 @@ __device__ __forceinline__ void stokes_forces(const T (&u)[8][4], T (&f)[8][4])
     for (int k = 0; k < 8; k++) c[k] = u[k][2];
     Ww = wht(c);
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        c[k] = u[k][3];
-        P1 += c[k];
-    }
-    Wp = wht(c);
-    const T h8 = T(1.0 / 8.0), h16 = T(1.0 / 16.0), h24 = T(1.0 / 24.0), h32 = T(1.0 / 32.0), h36 = T(1.0 / 36.0),
-            h48 = T(1.0 / 48.0), h96 = T(1.0 / 96.0), h144 = T(1.0 / 144.0), h288 = T(1.0 / 288.0);
+    for (int k = 0; k < 8; k++) c[k] = u[k][3];
+    Wp = wht(c, &P1);
+    const T h8 = scale * T(1.0 / 8.0), h16 = scale * T(1.0 / 16.0), h24 = scale * T(1.0 / 24.0),
+            h32 = scale * T(1.0 / 32.0), h36 = scale * T(1.0 / 36.0), h48 = scale * T(1.0 / 48.0),
+            h96 = scale * T(1.0 / 96.0), h144 = scale * T(1.0 / 144.0), h288 = scale * T(1.0 / 288.0);
     Walsh<T> Fu, Fv, Fw, Fp;
     Fu.x = h8 * Wu.x + h32 * P1;
     Fu.y = h16 * (Wu.y + Wv.x);
@@ -85,9 +86,35 @@ __device__ __forceinline__ void stokes_forces(const T (&u)[8][4], T (&f)[8][4])
     iwht(Fw, c);
 #pragma unroll
     for (int k = 0; k < 8; k++) f[k][2] = c[k];
-    iwht(Fp, c);
+    iwht(Fp, c, &Fp1);
 #pragma unroll
-    for (int k = 0; k < 8; k++) f[k][3] = c[k] + Fp1;
+    for (int k = 0; k < 8; k++) f[k][3] = c[k];
+}
+
+// LDS plane entries of 4 values (one node's u, v, w, p), 16-byte aligned,
+// moved as 16-byte vectors: a lane stride of 16 bytes read or written one
+// value at a time falls on a quarter of the LDS banks (measured: 42 M
+// bank-conflict cycles per 256^3 FP32 launch, none with the vectors)
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void lds_get4(const float *p, float (&o)[4])
+{
+    const f4v v = *reinterpret_cast<const f4v *>(p);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+__device__ __forceinline__ void lds_get4(const double *p, double (&o)[4])
+{
+    const d2v a = *reinterpret_cast<const d2v *>(p), b = *reinterpret_cast<const d2v *>(p + 2);
+    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+__device__ __forceinline__ void lds_put4(float *p, const float (&o)[4])
+{
+    *reinterpret_cast<f4v *>(p) = f4v{o[0], o[1], o[2], o[3]};
+}
+__device__ __forceinline__ void lds_put4(double *p, const double (&o)[4])
+{
+    *reinterpret_cast<d2v *>(p) = d2v{o[0], o[1]};
+    *reinterpret_cast<d2v *>(p + 2) = d2v{o[2], o[3]};
 }
 
 // Per-node fluid keys from the DOF maps: a node is pore if its DOF is below
@@ -147,7 +174,7 @@ __device__ __forceinline__ double stokes_tile(const T *__restrict__ v, T *__rest
     constexpr int NS = (NN + NT - 1) / NT;  // plane nodes per work-item
     constexpr int SB = 9;                   // 2 x 4 row-combined forces + pad
     static_assert(TR == 16 || TR == 32 || TR == 64, "rows per tile must tile a 64-wide wavefront");
-    __shared__ T su[2][NS * NT * 4];
+    __shared__ __align__(16) T su[2][NS * NT * 4];
     __shared__ T sb[NT * SB];
     const int tid = threadIdx.x;
     const int tr = tid % TR, tc = tid / TR;
@@ -193,10 +220,7 @@ __device__ __forceinline__ double stokes_tile(const T *__restrict__ v, T *__rest
     };
     auto store = [&](T *dst) {
 #pragma unroll
-        for (int k = 0; k < NS; k++) {
-            T *o = dst + (tid + k * NT) * 4;
-            o[0] = pre[k][0]; o[1] = pre[k][1]; o[2] = pre[k][2]; o[3] = pre[k][3];
-        }
+        for (int k = 0; k < NS; k++) lds_put4(dst + (tid + k * NT) * 4, pre[k]);
     };
     const bool elem_ok = tr <= vr && tc <= vc;
     const uint32_t eoff = elem_ok ? (uint32_t)wrapi(c0 - 1 + tc, nx) * (uint32_t)ny + (uint32_t)wrapi(r0 - 1 + tr, ny)
@@ -205,29 +229,52 @@ __device__ __forceinline__ double stokes_tile(const T *__restrict__ v, T *__rest
     auto elem_key = [&](int le) -> uint32_t { return fk[(size_t)wrapl(le) * nxy + eoff]; };
     auto is_pore = [&](uint32_t kv) -> bool { return elem_ok && ((kv >> 3) & 1u); };
     auto element = [&](const T *cur, const T *nxt, bool pore_e, T (&A)[4], T (&Af)[4]) {
-        T f[8][4];
-        if (pore_e) {
-            T u[8][4];
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const T *src = (WDL(k) ? nxt : cur) + ((tc + WDC(k)) * NR + tr + WDR(k)) * 4;
-#pragma unroll
-                for (int a = 0; a < 4; a++) u[k][a] = src[a];
-            }
-            stokes_forces(u, f);
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; k++)
-#pragma unroll
-                for (int a = 0; a < 4; a++) f[k][a] = T(0);
-        }
         T *dst = sb + tid * SB;
+        auto row_sums = [&](const T (&f)[8][4]) {
 #pragma unroll
-        for (int a = 0; a < 4; a++) {
-            A[a] = f[3][a] + lane_up1(f[0][a]);
-            Af[a] = f[7][a] + lane_up1(f[4][a]);
-            dst[a] = f[2][a] + lane_up1(f[1][a]);
-            dst[4 + a] = f[6][a] + lane_up1(f[5][a]);
+            for (int a = 0; a < 4; a++) {
+                A[a] = f[3][a] + lane_up1(f[0][a]);
+                Af[a] = f[7][a] + lane_up1(f[4][a]);
+                dst[a] = f[2][a] + lane_up1(f[1][a]);
+                dst[4 + a] = f[6][a] + lane_up1(f[5][a]);
+            }
+        };
+        if constexpr (sizeof(T) == 4) {
+            // FP32: a wavefront-uniform branch; a wavefront without a pore
+            // element skips the evaluation, in the others the non-pore
+            // elements evaluate with scale 0 (a per-lane branch's join
+            // zeroed all 32 forces in every wavefront: 7 % slower at 256^3)
+            if (__ballot(pore_e)) {
+                T u[8][4], f[8][4];
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    lds_get4((WDL(k) ? nxt : cur) + ((tc + WDC(k)) * NR + tr + WDR(k)) * 4, u[k]);
+                stokes_forces(u, f, pore_e ? T(1) : T(0));
+                row_sums(f);
+            } else {
+#pragma unroll
+                for (int a = 0; a < 4; a++) {
+                    A[a] = Af[a] = T(0);
+                    dst[a] = dst[4 + a] = T(0);
+                }
+            }
+        } else {
+            // FP64: the per-lane branch (the scaled evaluation of non-pore
+            // elements was 3 % slower)
+            T f[8][4];
+            if (pore_e) {
+                T u[8][4];
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    lds_get4((WDL(k) ? nxt : cur) + ((tc + WDC(k)) * NR + tr + WDR(k)) * 4, u[k]);
+                stokes_forces(u, f);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+#pragma unroll
+                    for (int a = 0; a < 4; a++) f[k][a] = T(0);
+            }
+            row_sums(f);
         }
     };
     const bool own = tr >= 1 && tc >= 1 && tr - 1 < vr && tc - 1 < vc;
@@ -243,6 +290,8 @@ __device__ __forceinline__ double stokes_tile(const T *__restrict__ v, T *__rest
     store(nxt);
     bool pe = is_pore(ek);
     ek = elem_key(l0);
+    // owned node's DOF id and key, one layer ahead
+    uint32_t oid = node2dof[(size_t)l0 * nxy + own_off], okey = fk[(size_t)l0 * nxy + own_off];
     __syncthreads();
     fetch(l0 + 1);
     T A[4], Af[4];
@@ -257,8 +306,6 @@ __device__ __forceinline__ double stokes_tile(const T *__restrict__ v, T *__rest
         T *t = cur; cur = nxt; nxt = t;
     }
     __syncthreads();
-    // owned node's DOF id and key, one layer ahead
-    uint32_t oid = node2dof[(size_t)l0 * nxy + own_off], okey = fk[(size_t)l0 * nxy + own_off];
     double dsum = 0.0;
     for (int s = 0; s < vl; s++) {
         const int l = l0 + s;
@@ -268,11 +315,7 @@ __device__ __forceinline__ double stokes_tile(const T *__restrict__ v, T *__rest
         ek = elem_key(l + 1);
         fetch(l + 2);
         T own_u[4] = {0, 0, 0, 0};
-        if (DOT && own) {
-            const T *uu = cur + (tc * NR + tr) * 4;
-#pragma unroll
-            for (int a = 0; a < 4; a++) own_u[a] = uu[a];
-        }
+        if (DOT && own) lds_get4(cur + (tc * NR + tr) * 4, own_u);
         element(cur, nxt, pe, A, Af);
         __syncthreads();
         store(cur);  // layer l + 2 replaces layer l

@@ -34,9 +34,10 @@ namespace chfem {
 template <typename T>
 struct Walsh { T x, y, z, xy, yz, xz, xyz; };
 
-// forward: Walsh sums of 8 nodal values (local node order)
+// forward: Walsh sums of 8 nodal values (local node order); total (if
+// given) receives the constant mode, the plain sum of the 8 values
 template <typename T>
-__device__ __forceinline__ Walsh<T> wht(const T (&u)[8])
+__device__ __forceinline__ Walsh<T> wht(const T (&u)[8], T *total = nullptr)
 {
     // x stage: pairs (xb=1) - (xb=0) and (xb=1) + (xb=0) for each (yb, zb)
     T a[2][2], b[2][2];
@@ -64,21 +65,23 @@ __device__ __forceinline__ Walsh<T> wht(const T (&u)[8])
     w.xz = bs[1] - bs[0];
     w.xy = bd[1] + bd[0];
     w.xyz = bd[1] - bd[0];
+    if (total) *total = as[1] + as[0];
     return w;
 }
 
-// inverse: f_k = sum_s F_s s_k (no constant mode)
+// inverse: f_k = sum_s F_s s_k, plus *c0 (a constant mode, if given)
 template <typename T>
-__device__ __forceinline__ void iwht(const Walsh<T> &F, T (&f)[8])
+__device__ __forceinline__ void iwht(const Walsh<T> &F, T (&f)[8], const T *c0 = nullptr)
 {
 #pragma unroll
     for (int yb = 0; yb < 2; yb++) {
         const T sy = yb ? T(1) : T(-1);
         const T A = F.x + sy * F.xy, B = F.z + sy * F.yz, C = F.xz + sy * F.xyz;
+        const T Y = c0 ? sy * F.y + *c0 : sy * F.y;
 #pragma unroll
         for (int xb = 0; xb < 2; xb++) {
             const T sx = xb ? T(1) : T(-1);
-            const T P = sx * A + sy * F.y, Q = B + sx * C;
+            const T P = sx * A + Y, Q = B + sx * C;
             f[CUBE(xb, yb, 1)] = P + Q;
             f[CUBE(xb, yb, 0)] = P - Q;
         }

@@ -4,9 +4,9 @@ plain copy kernel of known traffic for scaling hardware byte counters.
     python -m chfem_amd.opbench --physics elastic --size 256 --precision mixed --reps 20
 
 Thermal and elastic use bench.py's sphere image (seed 3, inclusion fraction
-0.3); fluid uses solid spheres until the porosity is 0.35 (seed 4, bumped
-until the pore space percolates), the Stokes model of the permeability
-workload.
+0.3); fluid uses solid spheres until the porosity is 0.35 (--porosity; seed
+4, bumped until the pore space percolates), the Stokes model of the
+permeability workload.
 """
 import argparse
 import json
@@ -40,6 +40,7 @@ def main():
     ap.add_argument("--dir", action="store_true",
                     help="the CG loop's form: direction update d = z + beta d inside the operator (Walsh NBN)")
     ap.add_argument("--both", action="store_true", help="time the --dot form, then the --dir form")
+    ap.add_argument("--porosity", type=float, default=0.35, help="fluid: porosity of the sphere packing")
     ap.add_argument("--cg1", action="store_true",
                     help="also time the single-kernel CG form (z = d_old - beta' d_old2 - alpha M^-1 q inside the "
                          "operator too)")
@@ -47,7 +48,7 @@ def main():
     n = a.size
     if a.physics == "fluid":
         # solid spheres (radius 6-12) until porosity <= 0.35, pore space percolating
-        img, _ = synthetic.porous_spheres((n, n, n), rmin=6, rmax=12, porosity=0.35, seed=4)
+        img, _ = synthetic.porous_spheres((n, n, n), rmin=6, rmax=12, porosity=a.porosity, seed=4)
         props = None
     else:
         img = synthetic.spheres((n, n, n), rmin=6, rmax=12, fraction=0.3, seed=3)
