@@ -166,14 +166,25 @@ def train_guess(n_samples, size, epochs, seed):
 
 
 def cpu_model():
+    """The host CPU as /proc/cpuinfo names it, with its family and model
+    numbers (a virtualised host may report only the brand) and the logical
+    CPU count."""
+    fields = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k in ("model name", "cpu family", "model") and k not in fields:
+                    fields[k] = v.strip()
+                if not line.strip() and fields:
+                    break
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = fields.get("model name", "unknown")
+    if "cpu family" in fields:
+        name += f" (family {fields['cpu family']}, model {fields.get('model', '?')}, {os.cpu_count()} logical CPUs)"
+    return name
 
 
 def cpu_baseline(img, n, gpu_iterations, iters_sample=50):

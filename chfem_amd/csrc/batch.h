@@ -32,6 +32,7 @@ struct Sys {
 
 struct Tile {
     int sys, bx, by, bz;      // Walsh tile of system sys
+    int slot;                 // its index in the system-contiguous order [tb0, tb1): its partials' position
 };
 
 struct Chunk {

@@ -41,9 +41,11 @@ __device__ __forceinline__ bool selected(const double *Sb, int sys, int mode)
 
 // Walsh-basis operator over the tile table (ND = 3 elastic, 1 conduction).
 // Blocks are dispatched to the 8 XCDs round-robin; block b takes table entry
-// start(b % 8) + b / 8, so each XCD walks one contiguous range of the table
-// (a system's tiles are contiguous, rows of tiles fastest) and neighbouring
-// tiles share its L2.
+// start(b % 8) + b / 8, so each XCD walks one contiguous range of the table.
+// The table holds, for each XCD in turn, one contiguous eighth of every
+// system's tiles (rows of tiles fastest): neighbouring tiles share an XCD's
+// L2, and every XCD keeps work while any system is still iterating.  Block
+// partials go to the tile's slot in its system's contiguous range.
 template <typename T, int ND, int TR, int TC>
 __global__ void __launch_bounds__(TR *TC + (TILE_HALO ? 64 : 0)) k_apply_walsh(const Sys *__restrict__ sys, const Tile *__restrict__ tiles,
                                                          const T *v, T *q, const double *__restrict__ Sb,
@@ -58,7 +60,7 @@ __global__ void __launch_bounds__(TR *TC + (TILE_HALO ? 64 : 0)) k_apply_walsh(c
     const double d = walsh_tile<T, ND, TR, TC, true, 0, TILE_HALO>(v + S.off, q + S.off, S.keys, lamu, 0, S.ny, S.nx, S.nz, S.LZ,
                                                      t.bx, t.by, t.bz).dq;
     const double s = bsum(d);
-    if (threadIdx.x == 0) part[ti] = s;
+    if (threadIdx.x == 0) part[t.slot] = s;
 }
 
 // Fused step of the z-form loop: d = z + beta d_old (or d = z on a fresh
@@ -87,7 +89,7 @@ __global__ void __launch_bounds__(TR *TC + (TILE_HALO ? 64 : 0)) k_apply_dir(con
     const double d = walsh_tile<float, ND, TR, TC, true, 1, TILE_HALO>(dold + S.off, q + S.off, S.keys, (const float *)S.mat32, 0,
                                                             S.ny, S.nx, S.nz, S.LZ, t.bx, t.by, t.bz, fz).dq;
     const double s = bsum(d);
-    if (threadIdx.x == 0) part[ti] = s;
+    if (threadIdx.x == 0) part[t.slot] = s;
 }
 
 // Single-kernel step of the z-form loop (see apply_cg1 in batch.h); 4
@@ -129,7 +131,7 @@ __global__ void __launch_bounds__(TR *TC + (TILE_HALO ? 64 : 0)) k_apply_cg1(con
     block_sum<4>(v);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int c = 0; c < 4; c++) part[(size_t)c * ntiles + ti] = v[c];
+        for (int c = 0; c < 4; c++) part[(size_t)c * ntiles + t.slot] = v[c];
     }
 }
 
@@ -304,10 +306,13 @@ __global__ void __launch_bounds__(BB) k_ring_flush(const Sys *__restrict__ sys, 
     if (threadIdx.x < (unsigned)w) coef[threadIdx.x] = (float)R[SB_AHIST + threadIdx.x];
     __syncthreads();
     const Sys &S = sys[ch.sys];
-    const uint32_t i = ch.i0 + threadIdx.x;
-    if (i >= ch.i1) return;
-    for (int a = 0; a < nd; a++) {
-        const size_t o = S.off + (size_t)nd * i + a;
+    // the chunk's nodes as one contiguous run of nd values each: consecutive
+    // work-items take consecutive values (coalesced; a node per work-item
+    // strided its loads by nd values)
+    const size_t o0 = S.off + (size_t)nd * ch.i0;
+    const uint32_t len = (uint32_t)nd * (ch.i1 - ch.i0);
+    for (uint32_t f = threadIdx.x; f < len; f += BB) {
+        const size_t o = o0 + f;
         float acc = x32[o];
         for (int k = 0; k < w; k++) acc += coef[k] * ring[(size_t)k * total + o];
         x32[o] = acc;

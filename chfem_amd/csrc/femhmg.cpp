@@ -1742,6 +1742,38 @@ int chfemBatchSetInitialGuessNetwork(void *h, void *net)
     return 1;
 }
 
+// The operator's tile table in dispatch order.  The kernels give block b
+// table entry start(b % 8) + b / 8 (the hardware sends block b to XCD
+// b % 8), so XCD x runs the contiguous range [start(x), start(x) + n_x),
+// n_x = T / 8 (+1 for x < T % 8).  That range is made of one contiguous
+// eighth of every system's tiles (the remainders of a system's split go to
+// the XCDs in rotation, continuing across systems, which gives exactly those
+// n_x): tiles that share halo nodes meet in one XCD's L2, and every XCD has
+// work for as long as any system iterates.  tiles comes in system-contiguous
+// order and keeps that position in Tile::slot.
+static std::vector<batch::Tile> xcd_table(const std::vector<batch::Sys> &sys, const std::vector<batch::Tile> &tiles)
+{
+    std::vector<batch::Tile> out;
+    out.reserve(tiles.size());
+    std::vector<int> first(sys.size() * 8), count(sys.size() * 8);
+    int rot = 0;
+    for (size_t s = 0; s < sys.size(); s++) {
+        const int n = sys[s].tb1 - sys[s].tb0, base = n / 8, extra = n % 8;
+        int at = sys[s].tb0;
+        for (int x = 0; x < 8; x++) {
+            const int c = base + ((x - rot + 8) % 8 < extra ? 1 : 0);
+            first[s * 8 + x] = at;
+            count[s * 8 + x] = c;
+            at += c;
+        }
+        rot = (rot + extra) % 8;
+    }
+    for (int x = 0; x < 8; x++)
+        for (size_t s = 0; s < sys.size(); s++)
+            for (int i = 0; i < count[s * 8 + x]; i++) out.push_back(tiles[first[s * 8 + x] + i]);
+    return out;
+}
+
 int chfemBatchSolve(void *h, double tol, unsigned int maxit)
 {
     using namespace batch;
@@ -1774,7 +1806,7 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
                 const int gz = (S.nz + S.LZ - 1) / S.LZ;
                 for (int z = 0; z < gz; z++)
                     for (int y = 0; y < gy; y++)
-                        for (int x = 0; x < gx; x++) tiles.push_back(Tile{sid, x, y, z});
+                        for (int x = 0; x < gx; x++) tiles.push_back(Tile{sid, x, y, z, (int)tiles.size()});
             }
             S.tb1 = (int)tiles.size();
             S.vb0 = (int)chunks.size();
@@ -1787,6 +1819,7 @@ int chfemBatchSolve(void *h, double tol, unsigned int maxit)
     }
     const size_t total = off;
     const int nsys = (int)sys.size();
+    tiles = xcd_table(sys, tiles);
     hipStream_t st = nullptr;
     if (hipStreamCreate(&st) != hipSuccess) return 0;
     Sys *dsys = nullptr;
