@@ -1094,11 +1094,33 @@ logical hmgSolveResident(var *Cout)
     const float *gy = nullptr;
     int gC = 0;
     double *x0dev = nullptr;
+    // the guesses of every load case, formed in one pass over the network's
+    // output (a vector per load case; if they do not fit, one vector filled
+    // per load case)
+    double *x0all[6] = {};
+    const int gnd = m.analysis == HMG_ELASTIC ? 3 : 1;
     if (guess_active && !m.using_x0 && (m.analysis == HMG_THERMAL || m.analysis == HMG_ELASTIC)) {
         gy = guess_predict_device(guess_active, m.elem_map.data(), m.props, m.nmat, m.ex, m.ey, m.ez, &gC);
-        if (gy && (hipStreamSynchronize(guess_stream(guess_active)) != hipSuccess ||
-                   hipMalloc(&x0dev, m.ndof * sizeof(double)) != hipSuccess))
-            gy = nullptr;
+        if (gy && hipStreamSynchronize(guess_stream(guess_active)) != hipSuccess) gy = nullptr;
+        if (gy) {
+            bool all = true;
+            for (unsigned d : dirs)
+                if (d < 6 && hipMalloc(&x0all[d], m.ndof * sizeof(double)) != hipSuccess) {
+                    (void)hipGetLastError();
+                    all = false;
+                    break;
+                }
+            if (all) {
+                nn::guess_to_x0_all(gy, gC, gnd, nn::Grid3{(int)m.ey, (int)m.ex, (int)m.ez}, 1.0f, x0all,
+                                    active_stream());
+            } else {
+                for (double *&p : x0all) {
+                    if (p) (void)hipFree(p);
+                    p = nullptr;
+                }
+                if (hipMalloc(&x0dev, m.ndof * sizeof(double)) != hipSuccess) gy = nullptr;
+            }
+        }
         if (!gy) appendf("    | learned initial guess unavailable for this grid: zero guess\n");
     }
     for (unsigned d : dirs) {
@@ -1126,9 +1148,11 @@ logical hmgSolveResident(var *Cout)
             }
         }
         int rc;
-        if (gy && x0dev) {
-            nn::guess_to_x0(gy, gC, (int)d, m.analysis == HMG_ELASTIC ? 3 : 1,
-                            nn::Grid3{(int)m.ey, (int)m.ex, (int)m.ez}, 1.0f, x0dev, active_stream());
+        if (gy && d < 6 && x0all[d]) {
+            rc = solve_device(x0all[d]);
+        } else if (gy && x0dev) {
+            nn::guess_to_x0(gy, gC, (int)d, gnd, nn::Grid3{(int)m.ey, (int)m.ex, (int)m.ez}, 1.0f, x0dev,
+                            active_stream());
             rc = solve_device(x0dev);
         } else {
             rc = solve_device(nullptr);
@@ -1168,6 +1192,8 @@ logical hmgSolveResident(var *Cout)
         }
     }
     if (x0dev) (void)hipFree(x0dev);
+    for (double *p : x0all)
+        if (p) (void)hipFree(p);
     m.texp_ok = false;
     if (ok && m.thermal_expansion) {
         if (m.analysis != HMG_ELASTIC)
@@ -1693,12 +1719,13 @@ static bool batch_guess(BatchRun *B, const batch::Batch &bt, const std::vector<b
         int gC = 0;
         const float *gy = guess_predict_device(B->net, m.elem_map.data(), m.props, m.nmat, m.ex, m.ey, m.ez, &gC);
         if (!gy || hipStreamSynchronize(guess_stream(B->net)) != hipSuccess) continue;
+        double *xs[6] = {};
         for (int s = 0; s < nsys; s++) {
             if (owner[s].first != (int)si || fabs(R(s, SB_DELTA0)) < 1e-12) continue;
-            nn::guess_to_x0(gy, gC, owner[s].second, nd, nn::Grid3{(int)m.ey, (int)m.ex, (int)m.ez}, 1.0f,
-                            x64 + sys[s].off, st);
+            xs[owner[s].second] = x64 + sys[s].off;
             has[s] = 1;
         }
+        nn::guess_to_x0_all(gy, gC, nd, nn::Grid3{(int)m.ey, (int)m.ex, (int)m.ez}, 1.0f, xs, st);
         // the next prediction reuses the network's output buffer
         if (hipStreamSynchronize(st) != hipSuccess) return false;
     }

@@ -63,6 +63,10 @@ void features(const uint16_t *elem, const float *table, int nf, uint16_t *in, in
 // x0[nd*i + a] = ramp of load case `load` at node i, component a, plus
 // scale * y[i][load*nd + a]
 void guess_to_x0(const float *y, int Cp, int load, int nd, Grid3 g, float scale, double *x0, hipStream_t s);
+// the same for every load case with x0[load] != nullptr (6 elastic, 3
+// thermal), in one pass over y: y holds each node's outputs of all load
+// cases together, so one pass reads it once instead of once per load case
+void guess_to_x0_all(const float *y, int Cp, int nd, Grid3 g, float scale, double *const x0[6], hipStream_t s);
 
 // fill / convert helpers
 void fill_f32(float *p, float v, size_t n, hipStream_t s);

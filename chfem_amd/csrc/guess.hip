@@ -456,6 +456,40 @@ void guess_to_x0(const float *y, int Cp, int load, int nd, Grid3 g, float scale,
     k_guess_x0<<<b, NB, 0, s>>>(y, Cp, load, nd, g, scale, x0);
 }
 
+struct X0Set {
+    double *x[6];
+};
+
+__global__ void __launch_bounds__(NB) k_guess_x0_all(const float *__restrict__ y, int Cp, int nd, Grid3 g, float scale,
+                                                     X0Set xs)
+{
+    const size_t N = (size_t)g.ny * g.nx * g.nz;
+    const int nloads = nd == 3 ? 6 : 3;
+    for (size_t i = (size_t)blockIdx.x * NB + threadIdx.x; i < N; i += (size_t)gridDim.x * NB) {
+        const int r = (int)(i % g.ny), c = (int)((i / g.ny) % g.nx), l = (int)(i / ((size_t)g.ny * g.nx));
+        const double ramps[3] = {(double)c, (double)((g.ny - r) % g.ny), (double)((g.nz - l) % g.nz)};
+        const float *yi = y + i * Cp;
+        for (int load = 0; load < nloads; load++) {
+            double *x0 = xs.x[load];
+            if (!x0) continue;
+            const int comp = nd == 3 ? cLoadComp[load] : 0;
+            const double ramp = ramps[cLoadAxis[load]];
+            for (int a = 0; a < nd; a++)
+                x0[nd * i + a] = (a == comp ? ramp : 0.0) + (double)(scale * yi[load * nd + a]);
+        }
+    }
+}
+
+void guess_to_x0_all(const float *y, int Cp, int nd, Grid3 g, float scale, double *const x0[6], hipStream_t s)
+{
+    const size_t N = (size_t)g.ny * g.nx * g.nz;
+    unsigned b = (unsigned)((N + NB - 1) / NB);
+    if (b > 65536) b = 65536;
+    X0Set xs;
+    for (int k = 0; k < 6; k++) xs.x[k] = x0[k];
+    k_guess_x0_all<<<b, NB, 0, s>>>(y, Cp, nd, g, scale, xs);
+}
+
 __global__ void k_fill_f32(float *p, float v, size_t n)
 {
     for (size_t j = (size_t)blockIdx.x * NB + threadIdx.x; j < n; j += (size_t)gridDim.x * NB) p[j] = v;
