@@ -11,8 +11,8 @@
 // once for the 4 groups (37-47 % faster than one group per wavefront for
 // the full-resolution layers at 256^3).  The products are the 16x16x16 BF16
 // shape: these layers are bound by their operand gathers, not by the matrix
-// cores (a 16 -> 16 channel layer at 256^3 runs about 230 GFLOP in some
-// 3 ms), so the 16x16x32 shape of gfx950 would not shorten them.
+// cores (a 16 -> 16 channel layer at 256^3 runs about 230 GFLOP in 2.2 ms),
+// so the 16x16x32 shape of gfx950 would not shorten them.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include "guess.h"
