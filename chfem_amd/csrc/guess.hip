@@ -193,39 +193,58 @@ __global__ void __launch_bounds__(NB) k_convw(const uint16_t *__restrict__ in, G
     const size_t v0 = (size_t)blockIdx.x * WCHUNK;
     const size_t v1 = v0 + WCHUNK < V ? v0 + WCHUNK : V;
     const int nci = Ci / 16, ntiles = (Co / 16) * nci;
+    // With 4 tiles or more, wavefront w takes tiles w, w + 4, .. over all the
+    // chunk's voxels.  With fewer, the wavefronts split the voxels as well:
+    // wavefront w takes tile w % ntiles on every nsplit-th step of 16 voxels
+    // from step w / ntiles, and the shares are added through LDS.
+    const bool few = ntiles < 4;
+    const int nsplit = few ? 4 / ntiles : 1;
+    const int grp = few ? wave / ntiles : 0;
+    const int wt = few ? wave % ntiles : wave;
+    const int mmax = few ? 1 : 4;
     float4_t acc[4];
 #pragma unroll
     for (int m = 0; m < 4; m++) acc[m] = float4_t{0.f, 0.f, 0.f, 0.f};
-    for (size_t vk = v0; vk < v1; vk += 16) {
-        size_t u[4];
-        bool ok[4];
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const size_t v = vk + 4 * kg + e;
-            ok[e] = v < v1;
-            const size_t vv = ok[e] ? v : v0;
-            const int r = (int)(vv % go.ny), c = (int)((vv / go.ny) % go.nx), l = (int)(vv / ((size_t)go.ny * go.nx));
-            u[e] = in_index<MODE>(r, c, l, t, gi, go);
-        }
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            const int tt = wave + 4 * m;
-            if (tt >= ntiles) break;
-            const int cot = tt / nci, cit = tt % nci;
-            short4_t a, b;
+    if (grp < nsplit) {
+        for (size_t vk = v0 + 16 * (size_t)grp; vk < v1; vk += 16 * (size_t)nsplit) {
+            size_t u[4];
+            bool ok[4];
 #pragma unroll
             for (int e = 0; e < 4; e++) {
                 const size_t v = vk + 4 * kg + e;
-                a[e] = ok[e] ? (short)dz[v * Co + cot * 16 + i] : (short)0;
-                b[e] = ok[e] ? (short)in[u[e] * Ci + cit * 16 + i] : (short)0;
+                ok[e] = v < v1;
+                const size_t vv = ok[e] ? v : v0;
+                const int r = (int)(vv % go.ny), c = (int)((vv / go.ny) % go.nx), l = (int)(vv / ((size_t)go.ny * go.nx));
+                u[e] = in_index<MODE>(r, c, l, t, gi, go);
             }
-            acc[m] = mfma16(a, b, acc[m]);
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const int tt = wt + 4 * m;
+                if (m >= mmax || tt >= ntiles) break;
+                const int cot = tt / nci, cit = tt % nci;
+                short4_t a, b;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const size_t v = vk + 4 * kg + e;
+                    a[e] = ok[e] ? (short)dz[v * Co + cot * 16 + i] : (short)0;
+                    b[e] = ok[e] ? (short)in[u[e] * Ci + cit * 16 + i] : (short)0;
+                }
+                acc[m] = mfma16(a, b, acc[m]);
+            }
         }
     }
+    if (few && nsplit > 1) {
+        __shared__ float4_t red[4][64];
+        red[wave][lane] = acc[0];
+        __syncthreads();
+        if (wave >= ntiles) return;
+        for (int k = 1; k < nsplit; k++) acc[0] += red[wave + ntiles * k][lane];
+    }
+    if (few && wave >= ntiles) return;
 #pragma unroll
     for (int m = 0; m < 4; m++) {
-        const int tt = wave + 4 * m;
-        if (tt >= ntiles) break;
+        const int tt = wt + 4 * m;
+        if (m >= mmax || tt >= ntiles) break;
         const int cot = tt / nci, cit = tt % nci;
 #pragma unroll
         for (int e = 0; e < 4; e++) {
