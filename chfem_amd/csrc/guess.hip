@@ -179,19 +179,27 @@ void conv(int mode, const uint16_t *in, Grid3 gi, int Ci, const uint16_t *W, con
 // weight gradients: block (chunk of output voxels, tap); the Co x Ci tile of
 // the tap is split into 16x16 products spread over the 4 wavefronts
 // ---------------------------------------------------------------------------
-#define WCHUNK 2048
+// output voxels per block: 2048, halved (down to 256) while the launch
+// would have fewer than 2048 blocks, so the coarse levels' few voxels are
+// still spread over the GPU
+static int wchunk(size_t V, int taps)
+{
+    int c = 2048;
+    while (c > 256 && ((V + c - 1) / c) * (size_t)taps < 2048) c /= 2;
+    return c;
+}
 
 template <int MODE>
 __global__ void __launch_bounds__(NB) k_convw(const uint16_t *__restrict__ in, Grid3 gi, int Ci,
                                               const uint16_t *__restrict__ dz, Grid3 go, int Co, int taps,
-                                              float *__restrict__ part)
+                                              int chunk, float *__restrict__ part)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int i = lane & 15, kg = lane >> 4;
     const int t = blockIdx.y;
     const size_t V = (size_t)go.ny * go.nx * go.nz;
-    const size_t v0 = (size_t)blockIdx.x * WCHUNK;
-    const size_t v1 = v0 + WCHUNK < V ? v0 + WCHUNK : V;
+    const size_t v0 = (size_t)blockIdx.x * chunk;
+    const size_t v1 = v0 + chunk < V ? v0 + chunk : V;
     const int nci = Ci / 16, ntiles = (Co / 16) * nci;
     // With 4 tiles or more, wavefront w takes tiles w, w + 4, .. over all the
     // chunk's voxels.  With fewer, the wavefronts split the voxels as well:
@@ -270,7 +278,8 @@ __global__ void __launch_bounds__(NB) k_wsum(const float *__restrict__ part, int
 size_t wpart_size(Grid3 go, int Co, int Ci, int taps)
 {
     const size_t V = (size_t)go.ny * go.nx * go.nz;
-    return ((V + WCHUNK - 1) / WCHUNK) * (size_t)taps * Co * Ci;
+    const int c = wchunk(V, taps);
+    return ((V + c - 1) / c) * (size_t)taps * Co * Ci;
 }
 
 void conv_wgrad(int mode, const uint16_t *in, Grid3 gi, int Ci, const uint16_t *dz, Grid3 go, int Co, float *part,
@@ -278,11 +287,12 @@ void conv_wgrad(int mode, const uint16_t *in, Grid3 gi, int Ci, const uint16_t *
 {
     const int taps = mode == CONV_DOWN ? 8 : 27;
     const size_t V = (size_t)go.ny * go.nx * go.nz;
-    const int nchunk = (int)((V + WCHUNK - 1) / WCHUNK);
+    const int chunk = wchunk(V, taps);
+    const int nchunk = (int)((V + chunk - 1) / chunk);
     dim3 grid(nchunk, taps);
-    if (mode == CONV_SAME) k_convw<CONV_SAME><<<grid, NB, 0, s>>>(in, gi, Ci, dz, go, Co, taps, part);
-    else if (mode == CONV_DOWN) k_convw<CONV_DOWN><<<grid, NB, 0, s>>>(in, gi, Ci, dz, go, Co, taps, part);
-    else k_convw<CONV_UP><<<grid, NB, 0, s>>>(in, gi, Ci, dz, go, Co, taps, part);
+    if (mode == CONV_SAME) k_convw<CONV_SAME><<<grid, NB, 0, s>>>(in, gi, Ci, dz, go, Co, taps, chunk, part);
+    else if (mode == CONV_DOWN) k_convw<CONV_DOWN><<<grid, NB, 0, s>>>(in, gi, Ci, dz, go, Co, taps, chunk, part);
+    else k_convw<CONV_UP><<<grid, NB, 0, s>>>(in, gi, Ci, dz, go, Co, taps, chunk, part);
     const size_t n = (size_t)taps * Co * Ci;
     k_wsum<<<(unsigned)((n + NB - 1) / NB), NB, 0, s>>>(part, nchunk, taps, Co, Ci, grad);
 }
@@ -310,7 +320,9 @@ void pack_weights(const float *Wm, uint16_t *out, int Co, int Ci, int taps, int 
 }
 
 // ---------------------------------------------------------------------------
-#define ACHUNK 1024
+// voxels per block of the activation backward (enough blocks at 64^3 and
+// below, where training runs)
+#define ACHUNK 256
 
 __global__ void __launch_bounds__(NB) k_act_bwd(const float *__restrict__ dy, const uint16_t *__restrict__ y, int relu,
                                                 uint16_t *__restrict__ dz, float *__restrict__ skip_grad,
@@ -389,7 +401,7 @@ void pool_children(const float *fine, Grid3 gf, float *coarse, int C, int accumu
 }
 
 // ---------------------------------------------------------------------------
-#define LCHUNK 4096
+#define LCHUNK 512
 
 __global__ void __launch_bounds__(NB) k_mse(const float *__restrict__ y, int Cp, const float *__restrict__ target, int nt,
                                             size_t V, double scale, float *__restrict__ dy, double *__restrict__ part)
