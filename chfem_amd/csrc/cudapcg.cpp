@@ -843,7 +843,20 @@ static int run_minres(const OpDesc &op, T *x, T *r, T *d, T *q, T *s, const T *d
     hipMemcpyAsync(d, q, n * sizeof(T), hipMemcpyDeviceToDevice, st);
     hipMemcpyAsync(q, s, n * sizeof(T), hipMemcpyDeviceToDevice, st);
     launch_precond<T>(dinv, r, r, n, nullptr, st);
+    // L2 rule: iterations are queued cg_batch() at a time, as in the CG
+    // loops: the update's finishing kernel counts the iteration and raises
+    // S_DONE once stop <= tol^2 res0, after which the queued launches return
+    // at once (the iterate and the count are those of a loop that
+    // synchronises every iteration).  The INF and error rules evaluate their
+    // metric on the host after every iteration.
+    const bool queued = rule == CUDAPCG_L2_NORM;
     Reduce ra = make_reduce(FIN_ALPHA, 0), rs = make_reduce(FIN_STORE, S_STOP), rdl = make_reduce(FIN_DELTA, 0);
+    if (queued) {
+        ra = make_guarded(FIN_ALPHA, 0);
+        rs = make_guarded(FIN_STOP, 0);
+        rdl = make_guarded(FIN_DELTA_NOCOUNT, 0);
+        if (!arm_guard(tol * tol * fabs(res0))) return -1;
+    }
     launch_precond<T>(dinv, q, nullptr, n, &ra, st);
     it = 1;
     launch_minres_update<T>(x, r, d, q, dinv, &rs, n, st);
@@ -858,13 +871,16 @@ static int run_minres(const OpDesc &op, T *x, T *r, T *d, T *q, T *s, const T *d
     double best = fabs(stop);
     unsigned best_it = it;
     while (cont && it < maxit) {
-        it++;
-        launch_minres_dir<T>(d, q, r, s, dinv, g->S, 0, &ra, n, st);
-        launch_minres_update<T>(x, r, d, q, dinv, &rs, n, st);
-        timed_apply<T>(op, r, s, &rdl);
+        const unsigned nb = queued ? std::min(cg_batch(), maxit - it) : 1u;
+        for (unsigned b = 0; b < nb; b++) {
+            launch_minres_dir<T>(d, q, r, s, dinv, g->S, 0, &ra, n, st);
+            launch_minres_update<T>(x, r, d, q, dinv, &rs, n, st);
+            timed_apply<T>(op, r, s, &rdl);
+        }
         if (!sync_loop()) return -1;
+        it = queued ? (unsigned)g->Sh[S_ITER] : it + 1;
         stop = g->Sh[S_STOP];
-        cont = above(stop, res0, tol);
+        cont = queued ? g->Sh[S_DONE] == 0.0 : above(stop, res0, tol);
         if (rule != CUDAPCG_L2_NORM) {
             if (!rule_metric(metric)) return -1;
             cont = metric > tol;

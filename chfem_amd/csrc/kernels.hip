@@ -431,6 +431,7 @@ __global__ void __launch_bounds__(BS) k_minres_dir(T *__restrict__ d, T *__restr
                                                    const T *__restrict__ sv, const T *__restrict__ dinv,
                                                    const double *__restrict__ S, int first, size_t n, Reduce red)
 {
+    if (stopped(red)) return;
     const T b = first ? T(0) : (T)(S[S_DELTA] / S[S_DELTA_OLD]);
     double acc = 0;
     STREAM_RANGE(T, n)
@@ -466,6 +467,7 @@ __global__ void __launch_bounds__(BS) k_minres_update(T *__restrict__ x, T *__re
                                                       const T *__restrict__ q, const T *__restrict__ dinv, size_t n,
                                                       Reduce red)
 {
+    if (stopped(red)) return;
     const T a = (T)red.S[S_ALPHA];
     double acc = 0;
     STREAM_RANGE(T, n)
