@@ -27,9 +27,15 @@ __global__ void __launch_bounds__(TR *TC + (HALO ? 64 : 0)) k_walsh(const T *__r
     // XCD b % 8), so each XCD is given one contiguous range of tiles (rows of
     // tiles fastest): tiles that share halo nodes then run on the same XCD at
     // about the same time and meet each other's halo lines in its L2.
+    // Single-kernel CG iterations of a guarded loop walk each XCD's range
+    // backwards on odd iterations: an iteration then starts with the tiles
+    // whose d and q the previous one wrote last, some of which the
+    // memory-side cache still holds (1.7 % off the 256^3 FP32 launch).
     constexpr int OR = HALO ? TR : TR - 1;  // node rows per tile
     const unsigned gx = (unsigned)(ny + OR - 1) / OR, gy = (unsigned)(nx + TC - 2) / (TC - 1);
-    const unsigned b = blockIdx.x, nb = gridDim.x, per = nb / 8u, rem = nb % 8u, xcd = b % 8u, k = b / 8u;
+    const unsigned b = blockIdx.x, nb = gridDim.x, per = nb / 8u, rem = nb % 8u, xcd = b % 8u;
+    unsigned k = b / 8u;
+    if (FM == 2 && DOT && red.guard && (((unsigned)red.S[S_ITER]) & 1u)) k = (xcd < rem ? per + 1u : per) - 1u - k;
     const unsigned t = (xcd < rem ? xcd * (per + 1u) : rem * (per + 1u) + (xcd - rem) * per) + k;
     const TileSums ts = walsh_tile<T, ND, TR, TC, DOT, FM, HALO>(v, q, keys, lamu, ebe, ny, nx, nz, LZ, (int)(t % gx),
                                                            (int)((t / gx) % gy), (int)(t / (gx * gy)), fz);
