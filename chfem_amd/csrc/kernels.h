@@ -202,10 +202,12 @@ template <typename T> void launch_pcg_update(T *x, T *r, const T *d, T *z, const
 // r -= q ; z = dinv r ; reduce r.z
 template <typename T> void launch_sub_precond(T *r, const T *q, const T *dinv, T *z, const Reduce *red, size_t n,
                                               hipStream_t s);
-// MINRES: d = rt + b d ; q = s + b q ; reduce q.(dinv q) ; b = S[DELTA]/S[DELTA_OLD] (first: d=rt, q=s)
-template <typename T> void launch_minres_dir(T *d, T *q, const T *rt, const T *sv, const T *dinv, const double *S,
-                                             int first, const Reduce *red, size_t n, hipStream_t s);
-// MINRES: rn = rt/dinv - a q ; rt = dinv rn ; x += a d ; reduce rn.rt
+// MINRES: dnew = rt + b dold (dnew may be dold) ; q = s + b q ; reduce q.(dinv q) ;
+// b = S[DELTA]/S[DELTA_OLD] (first: d = rt, q = s)
+template <typename T> void launch_minres_dir(const T *dold, T *dnew, T *q, const T *rt, const T *sv, const T *dinv,
+                                             const double *S, int first, const Reduce *red, size_t n, hipStream_t s);
+// MINRES: rn = rt/dinv - a q ; rt = dinv rn ; x += a d (x == nullptr: the
+// iterate is left to the direction ring's flush) ; reduce rn.rt
 template <typename T> void launch_minres_update(T *x, T *rt, const T *d, const T *q, const T *dinv, const Reduce *red,
                                                 size_t n, hipStream_t s);
 // FP64 outer refinement: x64 += e32

@@ -425,11 +425,13 @@ __global__ void __launch_bounds__(BS) k_sub_precond(T *__restrict__ r, const T *
     store_partials<1>(red, vv);
 }
 
-// MINRES: d = rt + b d ; q = s + b q ; reduce q.(dinv q)
+// MINRES: d = rt + b dold (written to dnew, which may be dold) ; q = s + b q ;
+// reduce q.(dinv q)
 template <typename T>
-__global__ void __launch_bounds__(BS) k_minres_dir(T *__restrict__ d, T *__restrict__ q, const T *__restrict__ rt,
-                                                   const T *__restrict__ sv, const T *__restrict__ dinv,
-                                                   const double *__restrict__ S, int first, size_t n, Reduce red)
+__global__ void __launch_bounds__(BS) k_minres_dir(const T *dold, T *dnew, T *__restrict__ q,
+                                                   const T *__restrict__ rt, const T *__restrict__ sv,
+                                                   const T *__restrict__ dinv, const double *__restrict__ S, int first,
+                                                   size_t n, Reduce red)
 {
     if (stopped(red)) return;
     const T b = first ? T(0) : (T)(S[S_DELTA] / S[S_DELTA_OLD]);
@@ -439,21 +441,21 @@ __global__ void __launch_bounds__(BS) k_minres_dir(T *__restrict__ d, T *__restr
     {
         const size_t o = j_ * W_;
         const V16<T> rv = ld16(rt + o), s_ = ld16(sv + o), mv = ld16(dinv + o);
-        V16<T> dv = ld16(d + o), qv = ld16(q + o);
+        V16<T> dv = ld16(dold + o), qv = ld16(q + o);
 #pragma unroll
         for (int k = 0; k < W_; k++) {
             dv.e[k] = first ? rv.e[k] : rv.e[k] + b * dv.e[k];
             qv.e[k] = first ? s_.e[k] : s_.e[k] + b * qv.e[k];
             acc += (double)qv.e[k] * (double)(mv.e[k] * qv.e[k]);
         }
-        st16(d + o, dv);
+        st16(dnew + o, dv);
         st16(q + o, qv);
     }
     FOR_TAIL(n)
     {
-        const T dn = first ? rt[i] : rt[i] + b * d[i];
+        const T dn = first ? rt[i] : rt[i] + b * dold[i];
         const T qn = first ? sv[i] : sv[i] + b * q[i];
-        d[i] = dn;
+        dnew[i] = dn;
         q[i] = qn;
         acc += (double)qn * (double)(dinv[i] * qn);
     }
@@ -461,8 +463,9 @@ __global__ void __launch_bounds__(BS) k_minres_dir(T *__restrict__ d, T *__restr
     store_partials<1>(red, vv);
 }
 
-// MINRES: rn = rt/dinv - a q ; rt = dinv rn ; x += a d ; reduce rn.rt
-template <typename T>
+// MINRES: rn = rt/dinv - a q ; rt = dinv rn ; (XU) x += a d ; reduce rn.rt.
+// Without XU the iterate is left to the direction ring's flush.
+template <typename T, bool XU>
 __global__ void __launch_bounds__(BS) k_minres_update(T *__restrict__ x, T *__restrict__ rt, const T *__restrict__ d,
                                                       const T *__restrict__ q, const T *__restrict__ dinv, size_t n,
                                                       Reduce red)
@@ -474,16 +477,21 @@ __global__ void __launch_bounds__(BS) k_minres_update(T *__restrict__ x, T *__re
     FOR_VECTORS
     {
         const size_t o = j_ * W_;
-        V16<T> xv = ld16(x + o), rv = ld16(rt + o);
-        const V16<T> dv = ld16(d + o), qv = ld16(q + o), mv = ld16(dinv + o);
+        V16<T> rv = ld16(rt + o);
+        const V16<T> qv = ld16(q + o), mv = ld16(dinv + o);
 #pragma unroll
         for (int k = 0; k < W_; k++) {
             const T rn = rv.e[k] / mv.e[k] - a * qv.e[k];
             rv.e[k] = mv.e[k] * rn;
-            xv.e[k] += a * dv.e[k];
             acc += (double)rn * (double)rv.e[k];
         }
-        st16(x + o, xv);
+        if constexpr (XU) {
+            V16<T> xv = ld16(x + o);
+            const V16<T> dv = ld16(d + o);
+#pragma unroll
+            for (int k = 0; k < W_; k++) xv.e[k] += a * dv.e[k];
+            st16(x + o, xv);
+        }
         st16(rt + o, rv);
     }
     FOR_TAIL(n)
@@ -491,7 +499,7 @@ __global__ void __launch_bounds__(BS) k_minres_update(T *__restrict__ x, T *__re
         const T rn = rt[i] / dinv[i] - a * q[i];
         const T rtn = dinv[i] * rn;
         rt[i] = rtn;
-        x[i] += a * d[i];
+        if constexpr (XU) x[i] += a * d[i];
         acc += (double)rn * (double)rtn;
     }
     double vv[1] = {acc};
@@ -1302,17 +1310,19 @@ void launch_sub_precond(T *r, const T *q, const T *dinv, T *z, const Reduce *red
 }
 
 template <typename T>
-void launch_minres_dir(T *d, T *q, const T *rt, const T *sv, const T *dinv, const double *S, int first,
-                       const Reduce *red, size_t n, hipStream_t s)
+void launch_minres_dir(const T *dold, T *dnew, T *q, const T *rt, const T *sv, const T *dinv, const double *S,
+                       int first, const Reduce *red, size_t n, hipStream_t s)
 {
-    LAUNCH_RED(1, k_minres_dir<T>, stream_grid(n, sizeof(T)), s, *red, d, q, rt, sv, dinv, S, first, n, *red);
+    LAUNCH_RED(1, k_minres_dir<T>, stream_grid(n, sizeof(T)), s, *red, dold, dnew, q, rt, sv, dinv, S, first, n,
+               *red);
 }
 
 template <typename T>
 void launch_minres_update(T *x, T *rt, const T *d, const T *q, const T *dinv, const Reduce *red, size_t n,
                           hipStream_t s)
 {
-    LAUNCH_RED(1, k_minres_update<T>, stream_grid(n, sizeof(T)), s, *red, x, rt, d, q, dinv, n, *red);
+    if (x) LAUNCH_RED(1, (k_minres_update<T, true>), stream_grid(n, sizeof(T)), s, *red, x, rt, d, q, dinv, n, *red);
+    else LAUNCH_RED(1, (k_minres_update<T, false>), stream_grid(n, sizeof(T)), s, *red, x, rt, d, q, dinv, n, *red);
 }
 
 // Key-Jacobi launchers: one block per contiguous range of 4-node groups
@@ -1518,7 +1528,7 @@ void launch_interpl2(const double *coarse, double *fine, int rows, int cols, int
     template void launch_pcg_dir<T>(T *, const T *, const double *, int, size_t, hipStream_t, int);           \
     template void launch_pcg_update<T>(T *, T *, const T *, T *, const T *, const Reduce *, size_t, hipStream_t); \
     template void launch_sub_precond<T>(T *, const T *, const T *, T *, const Reduce *, size_t, hipStream_t); \
-    template void launch_minres_dir<T>(T *, T *, const T *, const T *, const T *, const double *, int,        \
+    template void launch_minres_dir<T>(const T *, T *, T *, const T *, const T *, const T *, const double *, int, \
                                        const Reduce *, size_t, hipStream_t);                                  \
     template void launch_minres_update<T>(T *, T *, const T *, const T *, const T *, const Reduce *, size_t,  \
                                           hipStream_t);                                                       \
