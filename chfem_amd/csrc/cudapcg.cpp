@@ -403,6 +403,7 @@ static int make_ops()
     o.nmat = (int)g->dm.nmat;
     const char *esel = getenv("CHFEM_EBE");
     o.ebe_atomic = (esel && !strcmp(esel, "atomic")) || o.physics == PHY_FLUID;
+    o.ebe_tiles16 = esel && !strcmp(esel, "mfma16");
     g->dm.op64 = o;
     g->dm.op64.K = g->dm.K64;
     g->dm.op64.lamu = use_walsh ? g->lamu64 : nullptr;

@@ -5,10 +5,16 @@
 // thread and scatters them with atomics).
 //
 // Per element layer a block of TR x TC work-items (4 wavefronts for 32x8)
-// owns TR x TC elements.  Each wavefront groups its 64 elements by material
-// into column tiles of 16 and multiplies each tile by its material's local
-// matrix (KE x KE) with v_mfma_{f32,f64}_16x16x4: row tiles of 16 local
-// DOFs, 4 DOFs of the reduction per step.
+// owns TR x TC elements.  FP64 (and FP32 with CHFEM_EBE=mfma16): each
+// wavefront groups its 64 elements by material into column tiles of 16 and
+// multiplies each tile by its material's local matrix (KE x KE) with
+// v_mfma_{f32,f64}_16x16x4: row tiles of 16 local DOFs (24 rows fill two),
+// 4 DOFs of the reduction per step.  FP32 by default: v_mfma_f32_4x4x1
+// runs 16 independent 4x4x1 products per instruction; the wavefront groups
+// its elements by material into groups of 4 (one per 4-lane block), and
+// each block multiplies its group by 4 rows of its material's matrix, one
+// reduction DOF per instruction: no row padding, and only the groups past
+// the 16th (at most one per extra material) cost more instructions.
 // Element forces land in LDS and every interior node sums its 8 element
 // contributions there, as in the Walsh-basis kernel (walsh.hip): tiles
 // overlap by one node row/column, nothing is scattered with atomics, and the
@@ -53,13 +59,14 @@ __device__ __forceinline__ int edr(int k) { return (0x33 >> k) & 1; }
 __device__ __forceinline__ int edc(int k) { return (0x66 >> k) & 1; }
 __device__ __forceinline__ int edl(int k) { return k >> 2; }
 
-template <typename T, int ND, int TR, int TC, bool DOT>
+template <typename T, int ND, int TR, int TC, bool DOT, bool B4>
 __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T *__restrict__ q,
                                                       const uint16_t *__restrict__ keys, const T *__restrict__ K,
                                                       int nmat, int ebe, int ny, int nx, int nz, int LZ, Reduce red,
                                                       Sdf sdf)
 {
     static_assert(TR * TC == 256, "four wavefronts per block");
+    static_assert(!B4 || sizeof(T) == 4, "the 4x4 block form is FP32");
     if (DOT && stopped(red)) return;
     constexpr int KE = 8 * ND;                 // local DOFs
     constexpr int MT = (KE + 15) / 16;         // row tiles
@@ -69,10 +76,14 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
     // padded row stride of the local matrices: the A operand's 16 rows per
     // step then fall on distinct LDS banks (a stride of KE = 24 or 8 put
     // every fourth row on the same bank)
-    constexpr int KP = KE + 1;
+    // 4x4 block form: rows at stride KE (16-byte row reads of 4 reduction
+    // DOFs), materials at KE*KE + 4 so that two materials' rows read by one
+    // instruction fall on different banks
+    constexpr int KP = B4 ? KE : KE + 1;
+    constexpr int MS = B4 ? KE * KE + 4 : KE * KP;
     __shared__ T su[2][NV];
     __shared__ T sf[NT * SF];
-    __shared__ T sK[4 * KE * KP];              // K[m][row][col], row stride KP
+    __shared__ __align__(16) T sK[4 * MS];     // K[m][row][col], row stride KP, material stride MS
     __shared__ int smat[NT];
     // per wavefront: block element at each position of the material-grouped
     // column tiles (at most 64 elements in at most 7 tiles)
@@ -84,7 +95,10 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
     const int r0 = blockIdx.x * (TR - 1), c0 = blockIdx.y * (TC - 1), l0 = blockIdx.z * LZ;
     const int vr = min(TR - 1, ny - r0), vc = min(TC - 1, nx - c0), vl = min(LZ, nz - l0);
     const size_t nxy = (size_t)nx * ny;
-    for (int i = tid; i < nmat * KE * KE; i += NT) sK[(i / KE) * KP + i % KE] = K[i];
+    for (int i = tid; i < nmat * KE * KE; i += NT) {
+        const int m = i / (KE * KE), rc = i % (KE * KE);
+        sK[m * MS + (rc / KE) * KP + rc % KE] = K[i];
+    }
 
     uint32_t goff[SLOTS];
     bool gval[SLOTS];
@@ -141,7 +155,7 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             if (!cnt[m]) continue;
-            const T *Km = sK + m * KE * KP;
+            const T *Km = sK + m * MS;
             for (int t = 0; t < ((cnt[m] + 15) >> 4); t++) {
                 // column li of this tile: the material's element 16 t + li, if any
                 const bool valid = 16 * t + li < cnt[m];
@@ -180,6 +194,102 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
             }
         }
     };
+    // 4x4 block form (FP32).  Material m's c_m elements of the wavefront fill
+    // groups gb[m] .. gb[m] + ceil(c_m / 4) - 1 (lane order; sperm[wave][4 g +
+    // j] is the block element in column j of group g).  Lane l = 4 p + j
+    // takes column j of group p: it supplies row 4 rb + j of group p's
+    // material matrix (A) and its element's local DOF values (B), and
+    // receives rows 4 rb .. 4 rb + 3 of its element's forces.  Groups past the
+    // 16th are spread as (group, row block) jobs over the 16 blocks, in as
+    // few further rounds as they need.
+    auto elements4 = [&](const T *cur, const T *nxt) {
+        constexpr int NRB = KE / 4;  // row blocks of 4 local DOFs
+        const int me = smat[tid];
+        const unsigned long long below = (1ull << lane) - 1ull;
+        int cnt[4], gb[4];
+        int G = 0;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const unsigned long long bal = __ballot(me == m);
+            cnt[m] = __popcll(bal);
+            gb[m] = G;
+            if (me == m) sperm[wave][4 * G + __popcll(bal & below)] = tid;
+            G += (cnt[m] + 3) >> 2;
+        }
+        __syncthreads();  // sperm written before any lane reads it
+        const int p = lane >> 2, j = lane & 3;
+        // material and element of column j of group g (g < G: the materials
+        // with elements cover 0 .. G - 1 in order)
+        auto locate = [&](int g, int &m, int &e) -> bool {
+            m = 0;
+#pragma unroll
+            for (int k = 1; k < 4; k++)
+                if (cnt[k] && g >= gb[k]) m = k;
+            const bool ok = g < G && 4 * (g - gb[m]) + j < cnt[m];
+            e = ok ? sperm[wave][4 * g + j] : 0;
+            return ok;
+        };
+        auto gather = [&](int e, bool ok, T (&u)[KE]) {
+            const int eb = ((e / TR) * NR + (e % TR)) * ND;
+            const T esc = ok ? sesc[e] : T(0);
+#pragma unroll
+            for (int d = 0; d < KE; d++) {
+                const int kn = d / ND, comp = d % ND;
+                const T *pl = edl(kn) ? nxt : cur;
+                u[d] = esc * pl[eb + (edc(kn) * NR + edr(kn)) * ND + comp];
+            }
+        };
+        auto put = [&](int e, int rb, f4_t acc) {
+            T *dst = sf + e * SF + 4 * rb;
+#pragma unroll
+            for (int i = 0; i < 4; i++) dst[i] = acc[i];
+        };
+        {
+            int m, e;
+            const bool ok = locate(p, m, e);
+            T u[KE];
+            gather(e, ok, u);
+            const T *Kb = sK + m * MS + j * KP;
+#pragma unroll
+            for (int rb = 0; rb < NRB; rb += 2) {
+                // two row blocks at a time: independent accumulation chains
+                const T *K0 = Kb + 4 * rb * KP, *K1 = K0 + 4 * KP;
+                f4_t a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0};
+#pragma unroll
+                for (int k4 = 0; k4 < KE / 4; k4++) {
+                    const f4_t x0 = *(const f4_t *)(K0 + 4 * k4), x1 = *(const f4_t *)(K1 + 4 * k4);
+#pragma unroll
+                    for (int kk = 0; kk < 4; kk++) {
+                        a0 = __builtin_amdgcn_mfma_f32_4x4x1f32(x0[kk], u[4 * k4 + kk], a0, 0, 0, 0);
+                        a1 = __builtin_amdgcn_mfma_f32_4x4x1f32(x1[kk], u[4 * k4 + kk], a1, 0, 0, 0);
+                    }
+                }
+                if (ok) {
+                    put(e, rb, a0);
+                    put(e, rb + 1, a1);
+                }
+            }
+        }
+        const int O = (G - 16) * NRB;  // row-block jobs of the groups past the 16th
+        for (int x = 0; 16 * x < O; x++) {
+            const int qj = 16 * x + p;
+            const int rb = qj % NRB;
+            int m = 0, e = 0;
+            const bool ok = qj < O && locate(16 + qj / NRB, m, e);
+            T u[KE];
+            gather(e, ok, u);
+            const T *Kr = sK + m * MS + (4 * rb + j) * KP;
+            f4_t a0 = {0, 0, 0, 0};
+#pragma unroll
+            for (int k4 = 0; k4 < KE / 4; k4++) {
+                const f4_t x0 = *(const f4_t *)(Kr + 4 * k4);
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++)
+                    a0 = __builtin_amdgcn_mfma_f32_4x4x1f32(x0[kk], u[4 * k4 + kk], a0, 0, 0, 0);
+            }
+            if (ok) put(e, rb, a0);
+        }
+    };
     const bool own = tr >= 1 && tc >= 1 && tr - 1 < vr && tc - 1 < vc;
     auto fslot = [&](int k) { return ((tc - EDC(k)) * TR + (tr - EDR(k))) * SF + ND * k; };
 
@@ -192,7 +302,8 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
     sesc[tid] = escale(l0 - 1);
     __syncthreads();
     fetch(l0 + 1);
-    elements(cur, nxt);
+    if constexpr (B4) elements4(cur, nxt);
+    else elements(cur, nxt);
     const int mnext = material(l0);
     const T enext = escale(l0);
     __syncthreads();
@@ -227,7 +338,8 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
 #pragma unroll
             for (int a = 0; a < ND; a++) own_u[a] = uu[a];
         }
-        elements(cur, nxt);
+        if constexpr (B4) elements4(cur, nxt);
+        else elements(cur, nxt);
         const int mn = more ? material(l + 1) : -1;
         const T en = more ? escale(l + 1) : T(1);
         __syncthreads();
@@ -281,13 +393,30 @@ void launch_ebe_mfma(const OpDesc &op, const T *v, T *q, const Reduce *red, hipS
     while (cols * ((op.nz + LZ - 1) / LZ) > MAX_PARTIAL_BLOCKS) LZ *= 2;
     dim3 grid((op.ny + TR - 2) / (TR - 1), (op.nx + TC - 2) / (TC - 1), (op.nz + LZ - 1) / LZ);
     const T *K = (const T *)op.K;
-    if (op.physics == PHY_ELASTIC) {
-        if (red) k_ebe_mfma<T, 3, TR, TC, true><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr, op.sdf);
-        else k_ebe_mfma<T, 3, TR, TC, false><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr, op.sdf);
-    } else {
-        if (red) k_ebe_mfma<T, 1, TR, TC, true><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr, op.sdf);
-        else k_ebe_mfma<T, 1, TR, TC, false><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, op.nz, LZ, rr, op.sdf);
+#define EBE_LAUNCH(ND_, DOT_, B4_)                                                                                  \
+    k_ebe_mfma<T, ND_, TR, TC, DOT_, B4_><<<grid, TR * TC, 0, s>>>(v, q, op.keys, K, op.nmat, op.ebe, op.ny, op.nx, \
+                                                                   op.nz, LZ, rr, op.sdf)
+    if constexpr (sizeof(T) == 4) {
+        if (!op.ebe_tiles16) {
+            if (op.physics == PHY_ELASTIC) {
+                if (red) EBE_LAUNCH(3, true, true);
+                else EBE_LAUNCH(3, false, true);
+            } else {
+                if (red) EBE_LAUNCH(1, true, true);
+                else EBE_LAUNCH(1, false, true);
+            }
+            if (red) launch_reduce_finish(rr, grid.x * grid.y * grid.z, 1, s);
+            return;
+        }
     }
+    if (op.physics == PHY_ELASTIC) {
+        if (red) EBE_LAUNCH(3, true, false);
+        else EBE_LAUNCH(3, false, false);
+    } else {
+        if (red) EBE_LAUNCH(1, true, false);
+        else EBE_LAUNCH(1, false, false);
+    }
+#undef EBE_LAUNCH
     if (red) launch_reduce_finish(rr, grid.x * grid.y * grid.z, 1, s);
 }
 
@@ -308,7 +437,34 @@ __global__ void k_mfma_check_f(const float *A, const float *B, float *D, const d
     }
 }
 
+// lane-layout check of the FP32 4x4x1 16-block product: lane l supplies
+// A[l] and B[l] and receives its 4 result items at D[4 l .. 4 l + 3]
+__global__ void k_mfma_check4(const float *A, const float *B, float *D)
+{
+    const int l = threadIdx.x;
+    const f4_t d = __builtin_amdgcn_mfma_f32_4x4x1f32(A[l], B[l], f4_t{0, 0, 0, 0}, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; i++) D[4 * l + i] = d[i];
+}
+
 }  // namespace chfem
+
+extern "C" int chfemMfmaCheck4(const float *A, const float *B, float *D)
+{
+    float *dA = nullptr, *dB = nullptr, *dD = nullptr;
+    int ok = hipMalloc(&dA, 64 * 4) == hipSuccess && hipMalloc(&dB, 64 * 4) == hipSuccess &&
+             hipMalloc(&dD, 256 * 4) == hipSuccess;
+    ok = ok && hipMemcpy(dA, A, 64 * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(dB, B, 64 * 4, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        chfem::k_mfma_check4<<<1, 64>>>(dA, dB, dD);
+        ok = hipDeviceSynchronize() == hipSuccess && hipMemcpy(D, dD, 256 * 4, hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (dA) (void)hipFree(dA);
+    if (dB) (void)hipFree(dB);
+    if (dD) (void)hipFree(dD);
+    return ok;
+}
 
 extern "C" int chfemMfmaCheckF(const float *A, const float *B, float *D, const double *A2, const double *B2,
                                double *D2)

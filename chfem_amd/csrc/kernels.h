@@ -98,6 +98,9 @@ struct OpDesc {
     // EBE: 1 selects the reference-style element kernel with atomics instead
     // of the matrix-core kernel (ebe_mfma.hip; up to 4 materials)
     int ebe_atomic;
+    // EBE matrix-core kernel in FP32: 1 selects the 16x16x4 column-tile form
+    // (the FP64 form) instead of the 4x4x1 block form
+    int ebe_tiles16;
     Sdf sdf;
 };
 

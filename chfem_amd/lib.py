@@ -127,6 +127,7 @@ SIGNATURES = [
     ("hmgSetInitialGuessNetwork", C.c_int, [vp]),
     ("chfemGuessMfmaCheck", C.c_int, [vp, vp, vp]),
     ("chfemMfmaCheckF", C.c_int, [vp, vp, vp, vp, vp, vp]),
+    ("chfemMfmaCheck4", C.c_int, [vp, vp, vp]),
     ("cudapcgGetOperatorTile", u8, [u8, vp]),
     ("cudapcgGetLoopForm", C.c_int, [u8]),
     ("chfemBatchCreate", vp, []),

@@ -184,6 +184,10 @@ int chfemGuessMfmaCheck(const uint16_t *A, const uint16_t *B, float *D);
 /* FP32 / FP64 16x16x4 matrix-core lane-layout check used by the EBE
  * operator: D = A (16x4) * B (4x16), row-major. */
 int chfemMfmaCheckF(const float *A, const float *B, float *D, const double *A2, const double *B2, double *D2);
+/* FP32 4x4x1 16-block matrix-core lane-layout check used by the FP32 EBE
+ * operator: lane l of one wavefront supplies A[l] and B[l]; its 4 result
+ * items land at D[4 l .. 4 l + 3] (64 inputs each, 256 outputs). */
+int chfemMfmaCheck4(const float *A, const float *B, float *D);
 
 /* Operator-launch timing inside the Krylov loops: with timing on, every
  * operator launch of the inner solve is bracketed with HIP events on the

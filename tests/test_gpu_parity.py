@@ -314,6 +314,23 @@ def test_mfma_f32_f64_layout():
     np.testing.assert_allclose(D2, A @ B, rtol=1e-13, atol=1e-13)
 
 
+def test_mfma_4x4_layout():
+    """Lane layout of the FP32 4x4x1 16-block matrix-core product the FP32
+    EBE operator is built on: lane l = 4 b + j supplies row j of block b's A
+    and column j of its B, and holds column j of D_b = A_b B_b (rows 0..3
+    in its 4 result items)."""
+    from chfem_amd.lib import lib, ptr
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal(64).astype(np.float32)
+    B = rng.standard_normal(64).astype(np.float32)
+    D = np.zeros(256, np.float32)
+    assert lib().chfemMfmaCheck4(ptr(A), ptr(B), ptr(D)) == 1
+    Ab, Bb = A.reshape(16, 4).astype(np.float64), B.reshape(16, 4).astype(np.float64)
+    Db = np.einsum("bi,bj->bij", Ab, Bb)  # D_b[i][j] = A_b[i] B_b[j]
+    got = D.reshape(16, 4, 4)  # [block][column j][row i]
+    np.testing.assert_allclose(got.transpose(0, 2, 1), Db, rtol=1e-6, atol=1e-6)
+
+
 @pytest.mark.parametrize("kernel", ["mfma", "atomic"])
 @pytest.mark.parametrize("which", ["thermal", "elastic"])
 @pytest.mark.parametrize("nphase", [1, 4])
