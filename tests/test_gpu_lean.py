@@ -242,3 +242,31 @@ def test_lean_solvers_hold_fewer_vectors(elastic):
     print(f"solver vectors allocated (units of one FP64 vector, {vec / 2**20:.1f} MiB): {used}")
     assert used[CG3] <= 2.5 and used[CG2] <= 1.5
     assert used[CG] >= used[CG3] + 3.5
+
+
+@pytest.mark.parametrize("ring", ["2", "3", "5", "auto"])
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_minres_direction_ring(porous, ring, precision, monkeypatch):
+    """MINRES with the L2 rule keeps its directions in a ring and adds
+    alpha_j d_j to the iterate once per window: CHFEM_RING=2 leaves it no
+    ring (the update kernel updates x), 3 gives 2 slots (a window flush every
+    2 iterations, the tightest reuse), 5 gives 4, and the default 8.  The
+    FP64 iterations and solution are the oracle's MINRES ones; the mixed
+    solve (FP32 sweeps that end on stagnation, flushing their last window)
+    reaches the same permeability column."""
+    if ring != "auto":
+        monkeypatch.setenv("CHFEM_RING", ring)
+    d = 2
+    xr, okr, itr, _ = porous.solve(d, solver=MINRES, tol=1e-10, maxit=20000)
+    assert okr
+    s = H.GpuSolver(porous, solver=MINRES, precision=precision, tol=1e-10 if precision == 0 else 1e-9,
+                    maxit=20000)
+    try:
+        x, ok, st = s.solve(porous.rhs(d))
+    finally:
+        s.close()
+    assert ok
+    if precision == 0:
+        assert abs(int(st.iterations) - itr) <= 2
+        assert rel(x, xr) < 1e-7
+    assert rel(porous.update_c(d, x), porous.update_c(d, xr)) < 1e-7
