@@ -136,3 +136,16 @@ def test_synthetic_generators():
     p, seed = synthetic.porous_spheres((40, 40, 40), rmin=4, rmax=8, porosity=0.35, seed=4)
     assert (p == 0).mean() <= 0.35 + 0.05
     assert synthetic.percolates(p == 0)
+
+
+@pytest.mark.parametrize("module", ["opbench", "batchbench", "guessbench", "membench", "configbench"])
+def test_measurement_modules_parse_arguments(module):
+    """The measurement entry points import and parse their arguments without
+    a GPU (--help exits before any library call)."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, "-m", f"chfem_amd.{module}", "--help"], cwd=root, capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert "usage" in p.stdout
