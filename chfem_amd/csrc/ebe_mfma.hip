@@ -4,8 +4,8 @@
 // matrix_free.cu:1097-1335 computes the same products one element per
 // thread and scatters them with atomics).
 //
-// Per element layer a block of TR x TC work-items (4 wavefronts for 32x8)
-// owns TR x TC elements.  FP64 (and FP32 with CHFEM_EBE=mfma16): each
+// Per element layer a block of TR x TC work-items (4 wavefronts for 16x16
+// or 32x8) owns TR x TC elements.  FP64 (and FP32 with CHFEM_EBE=mfma16): each
 // wavefront groups its 64 elements by material into column tiles of 16 and
 // multiplies each tile by its material's local matrix (KE x KE) with
 // v_mfma_{f32,f64}_16x16x4: row tiles of 16 local DOFs (24 rows fill two),
@@ -20,6 +20,8 @@
 // overlap by one node row/column, nothing is scattered with atomics, and the
 // result is deterministic.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
+#include <string.h>
 #include "kernels.h"
 #include "reduce.h"
 
@@ -382,10 +384,9 @@ __global__ void __launch_bounds__(TR *TC) k_ebe_mfma(const T *__restrict__ v, T 
     }
 }
 
-template <typename T>
-void launch_ebe_mfma(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s)
+template <typename T, int TR, int TC>
+static void ebe_mfma_tiles(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s)
 {
-    constexpr int TR = 32, TC = 8;
     Reduce rr = {nullptr, nullptr, 0, 0, 0};
     if (red) rr = *red;
     int LZ = 16;
@@ -418,6 +419,19 @@ void launch_ebe_mfma(const OpDesc &op, const T *v, T *q, const Reduce *red, hipS
     }
 #undef EBE_LAUNCH
     if (red) launch_reduce_finish(rr, grid.x * grid.y * grid.z, 1, s);
+}
+
+// element tiles of 16 x 16 (a wavefront: 16 rows x 4 columns), or 32 x 8
+// (32 rows x 2 columns) with CHFEM_EBE_TILE=32x8.  At 256^3 the square tile
+// evaluates 1.27 element slots per element against 1.30 (halo and grid
+// tail), its compact wavefront patches hold fewer two-material groups, and
+// it ran 3 % faster in both precisions
+template <typename T>
+void launch_ebe_mfma(const OpDesc &op, const T *v, T *q, const Reduce *red, hipStream_t s)
+{
+    const char *e = getenv("CHFEM_EBE_TILE");
+    if (e && !strcmp(e, "32x8")) ebe_mfma_tiles<T, 32, 8>(op, v, q, red, s);
+    else ebe_mfma_tiles<T, 16, 16>(op, v, q, red, s);
 }
 
 template void launch_ebe_mfma<float>(const OpDesc &, const float *, float *, const Reduce *, hipStream_t);

@@ -331,15 +331,18 @@ def test_mfma_4x4_layout():
     np.testing.assert_allclose(got.transpose(0, 2, 1), Db, rtol=1e-6, atol=1e-6)
 
 
-@pytest.mark.parametrize("kernel", ["mfma", "atomic"])
+@pytest.mark.parametrize("kernel", ["mfma", "mfma-32x8", "atomic"])
 @pytest.mark.parametrize("which", ["thermal", "elastic"])
 @pytest.mark.parametrize("nphase", [1, 4])
 def test_ebe_kernels(kernel, which, nphase, monkeypatch):
-    """EBE operator on the matrix cores (default) and the reference-style
-    element kernel with atomics, with 1 and 4 materials on a grid of several
-    blocks in every direction, FP64 and FP32 against the oracle."""
+    """EBE operator on the matrix cores (16x16 element tiles by default, and
+    32x8) and the reference-style element kernel with atomics, with 1 and 4
+    materials on a grid of several blocks in every direction, FP64 and FP32
+    against the oracle."""
     if kernel == "atomic":
         monkeypatch.setenv("CHFEM_EBE", "atomic")
+    if kernel == "mfma-32x8":
+        monkeypatch.setenv("CHFEM_EBE_TILE", "32x8")
     rng = np.random.default_rng(51 + nphase)
     colors = (0, 60, 120, 255)[:nphase]
     img = rng.choice(np.array(colors, dtype=np.uint8), size=(20, 40, 35))
