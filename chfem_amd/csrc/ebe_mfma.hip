@@ -21,6 +21,7 @@
 // result is deterministic.
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
+#include <algorithm>
 #include <string.h>
 #include "kernels.h"
 #include "reduce.h"
@@ -389,7 +390,12 @@ static void ebe_mfma_tiles(const OpDesc &op, const T *v, T *q, const Reduce *red
 {
     Reduce rr = {nullptr, nullptr, 0, 0, 0};
     if (red) rr = *red;
-    int LZ = 16;
+    // element layers per block (each block also evaluates the layer below
+    // its first one): 32 for FP64 elastic, 16 otherwise (at 256^3: FP64
+    // elastic 1.660 -> 1.627 ms with 32; FP32 elastic the same; conduction
+    // 1-4 % slower with 32, 4-10 % with 64); CHFEM_EBE_LZ overrides
+    const char *lz_env = getenv("CHFEM_EBE_LZ");
+    int LZ = lz_env ? std::max(1, atoi(lz_env)) : (sizeof(T) == 8 && op.physics == PHY_ELASTIC ? 32 : 16);
     const long cols = (long)((op.ny + TR - 2) / (TR - 1)) * ((op.nx + TC - 2) / (TC - 1));
     while (cols * ((op.nz + LZ - 1) / LZ) > MAX_PARTIAL_BLOCKS) LZ *= 2;
     dim3 grid((op.ny + TR - 2) / (TR - 1), (op.nx + TC - 2) / (TC - 1), (op.nz + LZ - 1) / LZ);
