@@ -372,6 +372,44 @@ def test_ebe_kernels(kernel, which, nphase, monkeypatch):
             s.close()
 
 
+@pytest.mark.parametrize("which", ["thermal", "elastic"])
+def test_ebe_many_materials(which):
+    """More than 4 materials (7 thermal, 5 elastic): the EBE operator runs
+    the element kernel with atomics, since the matrix-core kernel keeps at
+    most 4 local matrices in LDS.  FP64 and FP32 operators and the mixed
+    solve's C column against the oracle."""
+    rng = np.random.default_rng(77)
+    if which == "thermal":
+        colors = (0, 40, 80, 120, 160, 200, 255)
+        props = [1.0, 10.0, 3.5, 0.5, 2.0, 7.0, 4.5]
+        an = H.THERMAL
+    else:
+        colors = (0, 60, 120, 180, 255)
+        props = [1.0, 0.3, 10.0, 0.2, 4.0, 0.25, 2.0, 0.35, 6.0, 0.28]
+        an = H.ELASTIC
+    img = rng.choice(np.array(colors, dtype=np.uint8), size=(12, 18, 15))
+    m = H.model(an, img, props, colors=colors)
+    v = rng.standard_normal(m.ndof)
+    q_ref = m.apply(v, strategy=1)
+    v32 = v.astype(np.float32)
+    q_ref32 = m.apply(v32.astype(np.float64), strategy=1)
+    xr, okr, _, _ = m.solve(1, tol=1e-10, strategy=1)
+    assert okr
+    for precision in (0, 1):
+        s = H.GpuSolver(m, strategy=1, precision=precision, tol=1e-9)
+        try:
+            if precision == 0:
+                assert rel(s.apply(v), q_ref) < 1e-12
+            else:
+                q32 = s.apply32(v32)
+                assert np.abs(q32 - q_ref32).max() < 1e-5 * np.abs(q_ref32).max()
+            x, ok, st = s.solve(m.rhs(1))
+            assert ok
+            assert rel(m.update_c(1, x), m.update_c(1, xr)) < 1e-7
+        finally:
+            s.close()
+
+
 @pytest.mark.parametrize("which", ["elastic", "thermal"])
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("fuse", ["2", "1", "0"])
