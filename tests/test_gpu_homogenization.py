@@ -164,3 +164,14 @@ def test_sdf_vs_oracle(which, kernel, monkeypatch):
         assert relmax(m.update_c(m.cdim - 1, x), m.update_c(m.cdim - 1, xr)) < 1e-7
     finally:
         s.close()
+    # the FP32 operator of mixed precision (the matrix-core kernel's 4x4
+    # block form scales each element's gathered values) on an
+    # FP32-representable input
+    s = H.GpuSolver(m, strategy=1, precision=1, tol=1e-9)
+    try:
+        assert lib().cudapcgSetParametricDensityField(ptr(dens), fmin, fmax) == 1
+        v32 = rng.standard_normal(m.ndof).astype(np.float32)
+        q_ref = m.apply(v32.astype(np.float64), strategy=1)
+        assert np.abs(s.apply32(v32) - q_ref).max() < 1e-5 * np.abs(q_ref).max()
+    finally:
+        s.close()
