@@ -270,3 +270,24 @@ def test_minres_direction_ring(porous, ring, precision, monkeypatch):
         assert abs(int(st.iterations) - itr) <= 2
         assert rel(x, xr) < 1e-7
     assert rel(porous.update_c(d, x), porous.update_c(d, xr)) < 1e-7
+
+
+def test_minres_then_cg_same_solver(elastic):
+    """The direction buffers are allocated on first use up to the loop's
+    width: MINRES takes 9 slots, a later CG solve on the same solver grows
+    them to its own width.  Both solves against the oracle's."""
+    d = 1
+    b = elastic.rhs(d)
+    xm, okm, _, _ = elastic.solve(d, solver=MINRES, tol=1e-10, maxit=20000)
+    xc, okc, itc, _ = elastic.solve(d, solver=CG, tol=1e-10, maxit=20000)
+    assert okm and okc
+    s = H.GpuSolver(elastic, solver=MINRES, tol=1e-10, maxit=20000)
+    try:
+        x1, ok1, _ = s.solve(b)
+        assert ok1 and rel(x1, xm) < 1e-7
+        assert s.L.cudapcgSetSolver(CG) == 1
+        x2, ok2, st2 = s.solve(b)
+        assert ok2 and rel(x2, xc) < 1e-7
+        assert abs(int(st2.iterations) - itc) <= 2
+    finally:
+        s.close()
