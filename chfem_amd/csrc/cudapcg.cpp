@@ -477,23 +477,42 @@ static void autotune_tiles()
                 launch_apply<T>(op, d, q, nullptr, g->stream);
             }
         };
-        float best_ms = 1e30f;
+        // two stages: every (form, tile) timed over 2 launches after one
+        // untimed, then the 3 fastest over 8 launches (about half the
+        // launches of timing all of them over 8)
+        auto timed = [&](int mode, int c, int reps, float &ms) -> bool {
+            op.tile_tr = cand[c][0]; op.tile_tc = cand[c][1]; op.tile_lz = cand[c][2];
+            hipEventRecord(g->ev0, g->stream);
+            for (int k = 0; k < reps; k++) run(mode);
+            hipEventRecord(g->ev1, g->stream);
+            if (hipEventSynchronize(g->ev1) != hipSuccess) return false;
+            ms = 0;
+            hipEventElapsedTime(&ms, g->ev0, g->ev1);
+            return true;
+        };
+        struct Cand { float ms; int mode, c; };
+        Cand all[2 * 16];
+        int na = 0;
         for (int mode = m0; mode <= m1; mode++) {
             if (mode > 0 && !fz) continue;
-            for (int c = 0; c < nc; c++) {
+            for (int c = 0; c < nc && na < 32; c++) {
                 op.tile_tr = cand[c][0]; op.tile_tc = cand[c][1]; op.tile_lz = cand[c][2];
                 run(mode);
-                hipEventRecord(g->ev0, g->stream);
-                for (int k = 0; k < 8; k++) run(mode);
-                hipEventRecord(g->ev1, g->stream);
-                if (hipEventSynchronize(g->ev1) != hipSuccess) return;
-                float ms = 0;
-                hipEventElapsedTime(&ms, g->ev0, g->ev1);
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best_mode = mode > 0 ? mode : 1;
-                    best[0] = cand[c][0]; best[1] = cand[c][1]; best[2] = cand[c][2];
-                }
+                float ms;
+                if (!timed(mode, c, 2, ms)) return;
+                all[na++] = {ms, mode, c};
+            }
+        }
+        if (na == 0) return;
+        std::sort(all, all + na, [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+        float best_ms = 1e30f;
+        for (int i = 0; i < std::min(na, 3); i++) {
+            float ms;
+            if (!timed(all[i].mode, all[i].c, 8, ms)) return;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_mode = all[i].mode > 0 ? all[i].mode : 1;
+                best[0] = cand[all[i].c][0]; best[1] = cand[all[i].c][1]; best[2] = cand[all[i].c][2];
             }
         }
         op.tile_tr = best[0]; op.tile_tc = best[1]; op.tile_lz = best[2];
