@@ -248,3 +248,27 @@ def test_c5_learned_guess_512():
     assert all(s["x0_used"] == 1 for s in s1)
     assert diff < 1e-5
     assert it1 < it0
+
+
+# ---------------------------------------------------------------------------
+# the measurement entry point of these configs, end to end at a small size
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("config,size", [("c2", 24), ("c3", 32)])
+def test_configbench_small(config, size):
+    """python -m chfem_amd.configbench runs every method of its config and
+    prints one JSON line whose runs converged with the same effective
+    tensor in FP64 and mixed precision (to the solver tolerance)."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, "-m", "chfem_amd.configbench", "--config", config, "--size", str(size)],
+                       cwd=root, capture_output=True, text=True, timeout=600)
+    assert p.returncode == 0, p.stderr[-2000:]
+    out = json.loads(p.stdout.strip().splitlines()[-1])
+    runs = out["runs"]
+    assert len(runs) == (4 if config == "c2" else 2)
+    for r in runs.values():
+        assert all(r["converged"]) and r["seconds"] > 0
+    cols = [np.array(r["C_columns"]) for r in runs.values()]
+    for c in cols[1:]:
+        assert relmax(c, cols[0]) < 1e-4
